@@ -1,0 +1,327 @@
+"""FP64 CPU oracle for the AVIONIC primal-dual loop (ctypes front end).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the timed
+CPU baseline.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package.  The product (``avionic_amd``)
+never imports it.
+
+All arrays are numpy ``complex128`` (images, k-space, coil maps) or
+``float64`` (masks, trajectories), in the reference's CUDA layout:
+image ``[T, H, W]``, k-space ``[T, C, H, W]``, mask ``[T, H, W]``,
+coil maps ``[C, H, W]``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+TV, TGV2, ICTGV2 = 0, 1, 2
+METHODS = {"tv": TV, "tgv2": TGV2, "ictgv2": ICTGV2}
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int),
+        ("maxIt", ctypes.c_int),
+        ("sigma", ctypes.c_double),
+        ("tau", ctypes.c_double),
+        ("sigmaTauRatio", ctypes.c_double),
+        ("tsw", ctypes.c_double),
+        ("tsw2", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+        ("alpha0", ctypes.c_double),
+        ("alpha1", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("schedule", ctypes.c_int),
+        ("gapEvery", ctypes.c_int),
+        ("centered", ctypes.c_int),
+    ]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def _declare(L):
+    L.orc_fft1.argtypes = [_P, _I, _I]
+    L.orc_fft2.argtypes = [_P, _I, _I, _I, _I]
+    L.orc_cart_K.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _P]
+    L.orc_cart_KH.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _P]
+    for f in ("orc_grad", "orc_symgrad", "orc_div", "orc_symdiv"):
+        getattr(L, f).argtypes = [_P, _I, _I, _I, _D, _D, _D, _P]
+    L.orc_prox3.argtypes = [_P, ctypes.c_long, _D]
+    L.orc_prox6.argtypes = [_P, ctypes.c_long, _D]
+    L.orc_ellipke.argtypes = [_D]
+    L.orc_ellipke.restype = _D
+    L.orc_time_space_weights.argtypes = [_D, _P, _P]
+    L.orc_time_space_weights.restype = _I
+    L.orc_adapt_lambda_cart.argtypes = [_P, _I, _I, _I, _D, _D]
+    L.orc_adapt_lambda_cart.restype = _D
+    L.orc_adapt_step.argtypes = [_D, _D, _D, _P, _P]
+    L.orc_tv_norm.argtypes = [_P, _I, _I, _I, _D, _D, _D]
+    L.orc_tv_norm.restype = _D
+    L.orc_tgv2_norm.argtypes = [_P, _P, _I, _I, _I, _D, _D, _D, _D, _D]
+    L.orc_tgv2_norm.restype = _D
+    L.orc_ictgv2_norm.argtypes = [_P, _P, _P, _P, _I, _I, _I] + [_D] * 9
+    L.orc_ictgv2_norm.restype = _D
+    L.orc_gstar.argtypes = [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _P]
+    L.orc_gstar.restype = _D
+    L.orc_pdgap.argtypes = [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I,
+                            _D, _D, _D, _D, _P]
+    L.orc_pdgap.restype = _D
+    L.orc_solver_adapt_step.argtypes = [_I, _P, _P, _P, _I, _I, _I, _I, _I, _P,
+                                        _D, _P, _P, _P, _P]
+    L.orc_recon.argtypes = [ctypes.POINTER(OrcParams), _I, _I, _I, _I, _P, _P,
+                            _P, _P, _P, _P, _P, _P, _P, _I]
+    L.orc_recon.restype = _I
+    L.orc_primal_components.argtypes = [_I]
+    L.orc_primal_components.restype = _I
+    L.orc_dual_components.argtypes = [_I]
+    L.orc_dual_components.restype = _I
+    if hasattr(L, "orc_nufft_K"):
+        L.orc_kb_beta.argtypes = [_I, _D]
+        L.orc_kb_beta.restype = _D
+        for f in ("orc_nufft_K", "orc_nufft_KH"):
+            getattr(L, f).argtypes = [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P]
+        L.orc_nudft_K.argtypes = [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P]
+
+
+def _c(a, dtype=np.complex128):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# --------------------------------------------------------------------------- FFT
+def fft1(a, sign):
+    a = _c(a).copy()
+    lib().orc_fft1(_ptr(a), a.shape[0], int(sign))
+    return a
+
+
+def fft2(a, sign, centered=False):
+    a = _c(a).copy()
+    H, W = a.shape
+    lib().orc_fft2(_ptr(a), H, W, int(sign), int(bool(centered)))
+    return a
+
+
+# ---------------------------------------------------------------------- operator
+def cart_K(x, b1, mask=None, centered=False):
+    x, b1 = _c(x), _c(b1)
+    T, H, W = x.shape
+    C = b1.shape[0]
+    m = None if mask is None else _c(mask, np.float64)
+    z = np.zeros((T, C, H, W), np.complex128)
+    lib().orc_cart_K(_ptr(x), _ptr(b1), _ptr(m), W, H, C, T, int(bool(centered)), _ptr(z))
+    return z
+
+
+def cart_KH(z, b1, mask=None, centered=False):
+    z, b1 = _c(z), _c(b1)
+    T, C, H, W = z.shape
+    m = None if mask is None else _c(mask, np.float64)
+    x = np.zeros((T, H, W), np.complex128)
+    lib().orc_cart_KH(_ptr(z), _ptr(b1), _ptr(m), W, H, C, T, int(bool(centered)), _ptr(x))
+    return x
+
+
+# ---------------------------------------------------------------------- stencils
+def _stencil(name, inp, ncomp_out, dims, dx, dy, dt):
+    T, H, W = dims
+    out = np.zeros((ncomp_out, T, H, W), np.complex128)
+    getattr(lib(), name)(_ptr(inp), W, H, T, float(dx), float(dy), float(dt), _ptr(out))
+    return out
+
+
+def grad(u, dx=1.0, dy=1.0, dt=1.0):
+    u = _c(u)
+    return _stencil("orc_grad", u, 3, u.shape, dx, dy, dt)
+
+
+def symgrad(v, dx=1.0, dy=1.0, dt=1.0):
+    v = _c(v)
+    return _stencil("orc_symgrad", v, 6, v.shape[1:], dx, dy, dt)
+
+
+def div(p, dx=1.0, dy=1.0, dt=1.0):
+    p = _c(p)
+    return _stencil("orc_div", p, 1, p.shape[1:], dx, dy, dt)[0]
+
+
+def symdiv(q, dx=1.0, dy=1.0, dt=1.0):
+    q = _c(q)
+    return _stencil("orc_symdiv", q, 3, q.shape[1:], dx, dy, dt)
+
+
+def prox3(y, radius):
+    y = _c(y).copy()
+    lib().orc_prox3(_ptr(y), y[0].size, float(radius))
+    return y
+
+
+def prox6(q, radius):
+    q = _c(q).copy()
+    lib().orc_prox6(_ptr(q), q[0].size, float(radius))
+    return q
+
+
+# ----------------------------------------------------------------------- scalars
+def ellipke(m):
+    return lib().orc_ellipke(float(m))
+
+
+def time_space_weights(tsw):
+    ds, dt = ctypes.c_double(), ctypes.c_double()
+    rc = lib().orc_time_space_weights(float(tsw), ctypes.byref(ds), ctypes.byref(dt))
+    if rc != 0:
+        raise ValueError("ComputeTimeSpaceWeights: Invalid ratio.")
+    return ds.value, dt.value
+
+
+def adapt_lambda_cart(mask, W, H, T, k, d):
+    m = None if mask is None else _c(mask, np.float64)
+    return lib().orc_adapt_lambda_cart(_ptr(m), W, H, T, float(k), float(d))
+
+
+def adapt_step(nKx, nx, ratio, sigma, tau):
+    s, t = ctypes.c_double(sigma), ctypes.c_double(tau)
+    lib().orc_adapt_step(float(nKx), float(nx), float(ratio), ctypes.byref(s), ctypes.byref(t))
+    return s.value, t.value
+
+
+def solver_adapt_step(method, delta, b1, mask, centered, d6, ratio, sigma, tau):
+    """delta: [P, T, H, W].  Returns (sigma, tau, nKx, nx)."""
+    delta, b1 = _c(delta), _c(b1)
+    P, T, H, W = delta.shape
+    C = b1.shape[0]
+    m = None if mask is None else _c(mask, np.float64)
+    d6 = np.ascontiguousarray(d6, np.float64)
+    s, t = ctypes.c_double(sigma), ctypes.c_double(tau)
+    nk, nx = ctypes.c_double(), ctypes.c_double()
+    lib().orc_solver_adapt_step(int(method), _ptr(delta), _ptr(b1), _ptr(m), W, H, C, T,
+                                int(bool(centered)), _ptr(d6), float(ratio),
+                                ctypes.byref(s), ctypes.byref(t), ctypes.byref(nk),
+                                ctypes.byref(nx))
+    return s.value, t.value, nk.value, nx.value
+
+
+# ------------------------------------------------------------------- functionals
+def tv_norm(u, dx, dy, dt):
+    u = _c(u)
+    T, H, W = u.shape
+    return lib().orc_tv_norm(_ptr(u), W, H, T, dx, dy, dt)
+
+
+def tgv2_norm(u, v, alpha0, alpha1, dx, dy, dt):
+    u, v = _c(u), _c(v)
+    T, H, W = u.shape
+    return lib().orc_tgv2_norm(_ptr(u), _ptr(v), W, H, T, alpha0, alpha1, dx, dy, dt)
+
+
+def ictgv2_norm(u, v, w, v2, alpha0, alpha1, alpha, d6):
+    u, v, w, v2 = _c(u), _c(v), _c(w), _c(v2)
+    T, H, W = u.shape
+    return lib().orc_ictgv2_norm(_ptr(u), _ptr(v), _ptr(w), _ptr(v2), W, H, T,
+                                 alpha0, alpha1, alpha, *[float(d) for d in d6])
+
+
+def gstar(method, x1, y, z, data, b1, mask, centered, lam, d6):
+    x1, y, z, data, b1 = _c(x1), _c(y), _c(z), _c(data), _c(b1)
+    T, C, H, W = data.shape
+    m = None if mask is None else _c(mask, np.float64)
+    d6 = np.ascontiguousarray(d6, np.float64)
+    return lib().orc_gstar(int(method), _ptr(x1), _ptr(y), _ptr(z), _ptr(data), _ptr(b1),
+                           _ptr(m), W, H, C, T, int(bool(centered)), float(lam), _ptr(d6))
+
+
+def pdgap(method, x, y, z, data, b1, mask, centered, lam, alpha0, alpha1, alpha, d6):
+    x, y, z, data, b1 = _c(x), _c(y), _c(z), _c(data), _c(b1)
+    T, C, H, W = data.shape
+    m = None if mask is None else _c(mask, np.float64)
+    d6 = np.ascontiguousarray(d6, np.float64)
+    return lib().orc_pdgap(int(method), _ptr(x), _ptr(y), _ptr(z), _ptr(data), _ptr(b1),
+                           _ptr(m), W, H, C, T, int(bool(centered)), float(lam),
+                           float(alpha0), float(alpha1), float(alpha), _ptr(d6))
+
+
+# ------------------------------------------------------------------------ solver
+def default_params(method):
+    """The CUDA InitParams defaults (tv.cpp:39-57, tgv2.cpp:36-56,
+    ictgv2.cpp:37-64); lambda is left for the caller (adaptLambda)."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    p = dict(method=m, maxIt=500, sigma=1.0 / 3.0, tau=1.0 / 3.0, sigmaTauRatio=1.0,
+             tsw=5.0, tsw2=1.0, lambda_=1.0, alpha0=float(np.sqrt(2.0)), alpha1=1.0,
+             alpha=0.5, schedule=0, gapEvery=0, centered=0)
+    if m == ICTGV2:
+        p.update(tsw=6.5, tsw2=2.5, alpha=0.6423)
+    return p
+
+
+def adapt_lambda_defaults(method):
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    if m == ICTGV2:
+        return 0.4 / (5.1191 * 0.35), 10.0 / (5.1191 * 0.35)
+    return 0.4 * 0.2991, 10.0 * 0.2991
+
+
+def recon(params: dict, data, b1, mask=None, x0=None, nthreads=0, logs=False):
+    """Run the PD loop.  Returns x (u) or, with logs=True, a dict with
+    x, all primal components, sigma/tau logs and gap log."""
+    pr = OrcParams(**params)
+    data, b1 = _c(data), _c(b1)
+    T, C, H, W = data.shape
+    P = lib().orc_primal_components(pr.method)
+    m = None if mask is None else _c(mask, np.float64)
+    x = np.zeros((T, H, W), np.complex128) if x0 is None else _c(x0).copy()
+    xall = np.zeros((P, T, H, W), np.complex128)
+    sig = np.zeros(max(pr.maxIt, 1))
+    tau = np.zeros(max(pr.maxIt, 1))
+    ng_cap = pr.maxIt // max(pr.gapEvery, 1) + 2
+    gaps = np.zeros(ng_cap)
+    ng = ctypes.c_int(0)
+    rc = lib().orc_recon(ctypes.byref(pr), W, H, C, T, _ptr(data), _ptr(b1), _ptr(m),
+                         _ptr(x), _ptr(xall), _ptr(sig), _ptr(tau), _ptr(gaps),
+                         ctypes.byref(ng), int(nthreads))
+    if rc < 0:
+        raise ValueError("invalid parameters")
+    if not logs:
+        return x
+    return dict(x=x, primal=xall, sigma=sig[: pr.maxIt], tau=tau[: pr.maxIt],
+                gap=gaps[: ng.value])
+
+
+def primal_components(method):
+    return lib().orc_primal_components(int(method))
+
+
+def dual_components(method):
+    return lib().orc_dual_components(int(method))

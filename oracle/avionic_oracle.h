@@ -1,0 +1,134 @@
+/*
+ * FP64 CPU oracle for the AVIONIC primal-dual reconstruction loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
+ * path is compared against, and the timed CPU baseline.  Only tests/, the
+ * smoke check and the benchmark's cpu_baseline leg may load it.  The product
+ * library (libavionic) never links, loads or calls anything in this directory.
+ *
+ * It restates, in double precision, the reference MATLAB scripts
+ * (matlab/tv_dmri.m, tgv2_dmri.m, ictgv2_dmri.m, forward_opt.m, backward_opt.m
+ * and their helpers) with the CUDA solver conventions (CUDA/src/tv.cpp,
+ * tgv2.cpp, ictgv2.cpp, pd_recon.cpp, utils.cpp, cartesian_operator.cpp),
+ * which the reference gtests pin.  Each function cites the lines it follows.
+ *
+ * Layout (the reference's CUDA layout, cartesian_operator.cpp:77-81,136-140):
+ *   image   [t][y][x]      complex, x fastest, width W, height H, frames T
+ *   k-space [t][c][y][x]   complex, C coils
+ *   mask    [t][y][x]      real (binary in practice)
+ *   b1      [c][y][x]      complex coil sensitivities
+ * Complex arrays are interleaved (re, im) doubles.
+ */
+#ifndef AVIONIC_ORACLE_H_
+#define AVIONIC_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Method identifiers, same numbering as the product C-ABI. */
+enum { ORC_TV = 0, ORC_TGV2 = 1, ORC_ICTGV2 = 2 };
+
+typedef struct orc_params {
+  int method;            /* ORC_TV / ORC_TGV2 / ORC_ICTGV2 */
+  int maxIt;             /* PDParams::maxIt (pd_recon.h:37) */
+  double sigma, tau;     /* initial step sizes (tv.cpp:44-45) */
+  double sigmaTauRatio;  /* pd_recon.h:64 */
+  double tsw, tsw2;      /* timeSpaceWeight(2): (ds,dt) derived inside, like
+                            ictgv2.cpp:300-306 */
+  double lambda;         /* data-fidelity weight */
+  double alpha0, alpha1; /* TGV2 weights (tgv2.h:11-14) */
+  double alpha;          /* ICTGV2 balance (ictgv2.h:12) */
+  int schedule;          /* 0: adapt when k<10 || k%50==0 (tv.cpp:172);
+                            1: every iteration (tv_dmri.m:198, debug=1) */
+  int gapEvery;          /* >0: record the normalised PD gap every gapEvery
+                            iterations (tv.cpp:180-193, debugstep) */
+  int centered;          /* 1: centred FFTs (cartesian_operator.cpp:90-99) */
+} orc_params;
+
+/* ---- FFT (unnormalised, sign -1 = e^{-i...}, +1 = e^{+i...}) ---- */
+void orc_fft1(double *a, int n, int sign);
+void orc_fft2(double *a, int H, int W, int sign, int centered);
+
+/* ---- Cartesian operator pair ---- */
+void orc_cart_K(const double *x, const double *b1, const double *mask, int W,
+                int H, int C, int T, int centered, double *z);
+void orc_cart_KH(const double *z, const double *b1, const double *mask, int W,
+                 int H, int C, int T, int centered, double *x);
+
+/* ---- stencils (single-frame-stack, N = W*H*T) ---- */
+void orc_grad(const double *u, int W, int H, int T, double dx, double dy,
+              double dt, double *g);
+void orc_symgrad(const double *v, int W, int H, int T, double dx, double dy,
+                 double dt, double *e);
+void orc_div(const double *p, int W, int H, int T, double dx, double dy,
+             double dt, double *out);
+void orc_symdiv(const double *q, int W, int H, int T, double dx, double dy,
+                double dt, double *out);
+void orc_prox3(double *y, long N, double radius);
+void orc_prox6(double *y, long N, double radius);
+
+/* ---- scalars ---- */
+double orc_ellipke(double m);
+int orc_time_space_weights(double tsw, double *ds, double *dt);
+double orc_adapt_lambda_cart(const double *mask, int W, int H, int T, double k,
+                             double d);
+void orc_adapt_step(double nKx, double nx, double ratio, double *sigma,
+                    double *tau);
+
+/* ---- functionals ---- */
+double orc_tv_norm(const double *u, int W, int H, int T, double dx, double dy,
+                   double dt);
+double orc_tgv2_norm(const double *u, const double *v, int W, int H, int T,
+                     double alpha0, double alpha1, double dx, double dy,
+                     double dt);
+double orc_ictgv2_norm(const double *u, const double *v, const double *w,
+                       const double *v2, int W, int H, int T, double alpha0,
+                       double alpha1, double alpha, double dx, double dy,
+                       double dt, double dx2, double dy2, double dt2);
+/* G* and PD gap with explicit (dx,dy,dt[,2]) like the reference methods. */
+double orc_gstar(int method, const double *x1, const double *y, const double *z,
+                 const double *data, const double *b1, const double *mask,
+                 int W, int H, int C, int T, int centered, double lambda,
+                 const double *d6);
+double orc_pdgap(int method, const double *x, const double *y, const double *z,
+                 const double *data, const double *b1, const double *mask,
+                 int W, int H, int C, int T, int centered, double lambda,
+                 double alpha0, double alpha1, double alpha, const double *d6);
+/* Step-size adaptation of one solver (tv.cpp:79-97, tgv2.cpp:89-128,
+   ictgv2.cpp:97-161) on primal difference `dx` (P components). */
+void orc_solver_adapt_step(int method, const double *delta, const double *b1,
+                           const double *mask, int W, int H, int C, int T,
+                           int centered, const double *d6, double ratio,
+                           double *sigma, double *tau, double *nKx, double *nx);
+
+/* ---- the PD loop ----
+ * x_inout: the image u (N complex).  x_out_all (nullable): all P primal
+ * components at exit.  sig_log/tau_log (nullable): sigma/tau after each
+ * iteration (maxIt entries).  gap_log (nullable, >= maxIt/gapEvery+1 entries):
+ * normalised gaps; *n_gap receives the count.  Returns iterations run. */
+int orc_recon(const orc_params *p, int W, int H, int C, int T,
+              const double *data, const double *b1, const double *mask,
+              double *x_inout, double *x_out_all, double *sig_log,
+              double *tau_log, double *gap_log, int *n_gap, int nthreads);
+
+/* Primal / dual component counts per method. */
+int orc_primal_components(int method);
+int orc_dual_components(int method);
+
+/* ---- non-Cartesian (radial) operator, Kaiser-Bessel gridding ---- */
+double orc_kb_beta(int width, double osf);
+void orc_nufft_K(const double *x, const double *b1, const double *traj,
+                 const double *dcf, int W, int H, int C, int T, int M,
+                 int width, double osf, double *kdata);
+void orc_nufft_KH(const double *kdata, const double *b1, const double *traj,
+                  const double *dcf, int W, int H, int C, int T, int M,
+                  int width, double osf, double *x);
+void orc_nudft_K(const double *x, const double *b1, const double *traj,
+                 const double *dcf, int W, int H, int C, int T, int M,
+                 double *kdata);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

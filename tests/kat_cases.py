@@ -1,0 +1,331 @@
+"""Known-answer cases transcribed from the reference gtests.
+
+Each ``kat_<name>(backend, spec)`` rebuilds the inputs of the reference test
+(``arange`` vectors, constant coil maps, all-ones masks), runs it on a backend
+(the FP64 oracle or the HIP library) and returns ``[(label, got, expected,
+tol), ...]``.  ``spec`` is the matching entry of ``golden/reference_kats.json``,
+which cites the reference test lines.
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KATS = json.load(open(os.path.join(HERE, "golden", "reference_kats.json")))
+EPS = KATS["eps"]
+
+AR = np.arange(4000, dtype=np.float64).astype(np.complex128)
+SQRT2 = math.sqrt(2.0)
+
+
+def tol_for(expected):
+    """EPS of the reference tests, widened to 1e-6 relative where that is
+    larger: the reference computes these values with float32 sums, which do
+    not resolve EPS at magnitudes above ~1e3."""
+    return max(EPS, 1e-6 * abs(expected))
+
+
+def img(W, H, T, off=0):
+    return AR[off:off + W * H * T].reshape(T, H, W).copy()
+
+
+def b1_const(C, H, W, vals):
+    b = np.ones((C, H, W), np.complex128)
+    for i, v in enumerate(vals):
+        b[i] *= complex(v[0], v[1]) if isinstance(v, (list, tuple)) else v
+    return b
+
+
+def _at(a, x, y, t):
+    return a[t, y, x]
+
+
+def _norm3(g):
+    return np.sqrt((np.abs(g) ** 2).sum(0))
+
+
+def _norm6(e):
+    w = np.array([1, 1, 1, 2, 2, 2], np.float64)[:, None, None, None]
+    return np.sqrt((w * np.abs(e) ** 2).sum(0))
+
+
+def kat_gradient_norm(be, s):
+    W, H, T = s["dims"]
+    g = be.grad(img(W, H, T), s["d"])
+    n = _norm3(g)
+    return [(f"|grad|{c[:3]}", _at(n, *c[:3]), c[3], tol_for(c[3])) for c in s["checks"]]
+
+
+kat_scaled_gradient_norm = kat_gradient_norm
+
+
+def kat_tv_divergence(be, s):
+    W, H, T = s["dims"]
+    g = be.grad(img(W, H, T), [1, 1, 1])
+    n = np.maximum(_norm3(g), 1.0)
+    d = be.div(g / n, [1, 1, 1])
+    return [(f"div{c[:3]}", _at(d, *c[:3]).real, c[3], tol_for(c[3])) for c in s["checks"]]
+
+
+def kat_tv_adapt_lambda(be, s):
+    W, H, T = s["dims"]
+    lam = be.adapt_lambda_cart(np.ones((T, H, W)), W, H, T, s["k"], s["d"])
+    return [("lambda", lam, s["expected"], EPS)]
+
+
+def _step_case(be, s, method, delta, d6):
+    W, H, T = s["dims"]
+    C = s["coils"]
+    b1 = b1_const(C, H, W, s["b1"])
+    sig, tau = be.solver_adapt_step(method, delta, b1, np.ones((T, H, W)), True, d6, 1.0,
+                                    s["sigma0"], 1.0 / 3.0)
+    return [("sigma", sig, s["expected_sigma"], EPS), ("tau", tau, s["expected_tau"], EPS)]
+
+
+def kat_tv_adapt_step(be, s):
+    W, H, T = s["dims"]
+    return _step_case(be, s, 0, (1j * img(W, H, T))[None], [1, 1, 1, 1, 1, 1])
+
+
+def kat_tgv_adapt_step(be, s):
+    W, H, T = s["dims"]
+    x = img(W, H, T)
+    return _step_case(be, s, 1, np.stack([x, x, x, x]), [1, 1, 1, 1, 1, 1])
+
+
+def kat_ictgv_adapt_step(be, s):
+    W, H, T = s["dims"]
+    N = W * H * T
+    a, b = img(W, H, T), img(W, H, T, N)
+    return _step_case(be, s, 2, np.stack([a, a, a, a, b, b, b, b]), s["d6"])
+
+
+def kat_tv_norm(be, s):
+    W, H, T = s["dims"]
+    out = []
+    for ds, dt, f, exp in s["cases"]:
+        u = img(W, H, T) * ((1 + 2j) if f == 3.0 else 1.0)
+        out.append((f"tvnorm{ds},{dt}", be.tv_norm(u, [ds, ds, dt]), exp, tol_for(exp)))
+    return out
+
+
+def _gap_inputs(s, ncomp_dual):
+    W, H, T = s["dims"]
+    C = s["coils"]
+    N = W * H * T
+    data = AR[:N * C].reshape(T, C, H, W).copy()
+    x = img(W, H, T)
+    z = np.ones((T, C, H, W), np.complex128)
+    y = np.zeros((ncomp_dual, T, H, W), np.complex128)
+    b1 = b1_const(C, H, W, [[1, 0], [0, 2], [1, 0]])
+    return data, x, z, y, b1, np.ones((T, H, W))
+
+
+def kat_tv_gstar(be, s):
+    data, x, z, y, b1, m = _gap_inputs(s, 3)
+    g = be.gstar(0, x, y, z, data, b1, m, False, s["lambda"], [1] * 6)
+    return [("gstar", g, s["expected"], tol_for(s["expected"]))]
+
+
+def kat_tv_pdgap(be, s):
+    data, x, z, y, b1, m = _gap_inputs(s, 3)
+    g = be.pdgap(0, x[None], y, z, data, b1, m, False, s["lambda"], 0, 0, 0, [1] * 6)
+    return [("pdgap", g, s["expected"], tol_for(s["expected"]))]
+
+
+def kat_tgv_gstar(be, s):
+    data, x, z, y, b1, m = _gap_inputs(s, 9)
+    g = be.gstar(1, x, y, z, data, b1, m, False, s["lambda"], [1] * 6)
+    return [("gstar", g, s["expected"], tol_for(s["expected"]))]
+
+
+def kat_tgv_pdgap(be, s):
+    data, x, z, y, b1, m = _gap_inputs(s, 9)
+    g = be.pdgap(1, np.stack([x] * 4), y, z, data, b1, m, False, s["lambda"], SQRT2, 1.0, 0,
+                 [1] * 6)
+    return [("pdgap", g, s["expected"], tol_for(s["expected"]))]
+
+
+def kat_ictgv_gstar(be, s):
+    data, x, z, y, b1, m = _gap_inputs(s, 18)
+    g = be.gstar(2, x, y, z, data, b1, m, False, s["lambda"], s["d6"])
+    return [("gstar", g, s["expected"], tol_for(s["expected"]))]
+
+
+def kat_ictgv_pdgap(be, s):
+    data, x, z, y, b1, m = _gap_inputs(s, 18)
+    W, H, T = s["dims"]
+    x3 = img(W, H, T, W * H * T)
+    xa = np.stack([x, x, x, x, x3, x3, x3, x3])
+    g = be.pdgap(2, xa, y, z, data, b1, m, False, s["lambda"], s["alpha0"], s["alpha1"],
+                 s["alpha"], s["d6"])
+    return [("pdgap", g, s["expected"], tol_for(s["expected"]))]
+
+
+def _iteration(be, s, method):
+    W, H, T = s["dims"]
+    C = s["coils"]
+    data = AR[:W * H * C * T].reshape(T, C, H, W).copy()
+    b1 = b1_const(C, H, W, [[1, 0], [0, 2]])
+    mask = np.ones((T, H, W)) if s["mask"] else None
+    p = be.default_params(method)
+    k, d = be.adapt_lambda_defaults(method)
+    p["lambda_"] = be.adapt_lambda_cart(mask, W, H, T, k, d)
+    p["centered"] = int(s["centered"])
+    x = be.recon(p, data, b1, mask, None)
+    return [(f"|x|{c[:3]}", abs(_at(x, *c[:3])), c[3], tol_for(c[3])) for c in s["checks"]]
+
+
+def kat_tv_iteration(be, s):
+    return _iteration(be, s, 0)
+
+
+def kat_tgv_iteration(be, s):
+    return _iteration(be, s, 1)
+
+
+def kat_ictgv_iteration(be, s):
+    return _iteration(be, s, 2)
+
+
+def _tgv_inputs(W, H, T):
+    N = W * H * T
+    u = img(W, H, T)
+    v = np.stack([img(W, H, T, N * (k + 1)) for k in range(3)])
+    return u, v
+
+
+def kat_tgv_gradient_norm(be, s):
+    W, H, T = s["dims"]
+    u, v = _tgv_inputs(W, H, T)
+    n1 = _norm3(be.grad(u, s["d"]) - v)
+    n2 = _norm6(be.symgrad(v, s["d"]))
+    out = [(f"|grad-v|{c[:3]}", _at(n1, *c[:3]), c[3], tol_for(c[3])) for c in s["grad_checks"]]
+    out += [(f"|Ev|{c[:3]}", _at(n2, *c[:3]), c[3], tol_for(c[3])) for c in s["symgrad_checks"]]
+    return out
+
+
+kat_tgv_scaled_gradient_norm = kat_tgv_gradient_norm
+
+
+def kat_tgv_divergence(be, s):
+    W, H, T = s["dims"]
+    d = s["d"]
+    u, v = _tgv_inputs(W, H, T)
+    y1 = be.prox3(be.grad(u, d) - v, s["alpha1"])
+    y2 = be.prox6(be.symgrad(v, d), s["alpha0"])
+    sd = be.symdiv(y2, d)
+    return [(f"|divsym|{c[:4]}", abs(sd[c[0], c[3], c[2], c[1]]), c[4], tol_for(c[4]))
+            for c in s["symdiv_abs_checks"]]
+
+
+def kat_tgv2_norm(be, s):
+    W, H, T = s["dims"]
+    u0 = img(W, H, T)
+    v = np.stack([u0, u0, u0])
+    out = []
+    for ds, dt, f, exp in s["cases"]:
+        u = u0 * ((1 + 2j) if f == 3.0 else 1.0)
+        g = be.tgv2_norm(u, v, s["alpha0"], s["alpha1"], [ds, ds, dt])
+        out.append((f"tgv2norm{ds},{dt}", g, exp, tol_for(exp)))
+    return out
+
+
+def _ictgv_inputs(W, H, T):
+    N = W * H * T
+    u = img(W, H, T)
+    v = np.stack([img(W, H, T, (k + 1) * N) for k in range(3)])
+    w = img(W, H, T, 4 * N)
+    v2 = np.stack([img(W, H, T, (k + 5) * N) for k in range(3)])
+    return u, v, w, v2
+
+
+def kat_ictgv_gradient_norm(be, s):
+    W, H, T = s["dims"]
+    u, v, w, v2 = _ictgv_inputs(W, H, T)
+    dd = s.get("d", [1.0, 1.0, 1.0, 1.0])
+    d1 = [dd[0], dd[0], dd[1]]
+    d2 = [dd[2], dd[2], dd[3]]
+    out = []
+    if "y1_checks" in s:
+        n1 = _norm3(be.grad(u - w, d1) - v)
+        out += [(f"|y1|{c[:3]}", _at(n1, *c[:3]), c[3], tol_for(c[3])) for c in s["y1_checks"]]
+    n3 = _norm3(be.grad(w, d2) - v2)
+    n4 = _norm6(be.symgrad(v2, d2))
+    out += [(f"|y3|{c[:3]}", _at(n3, *c[:3]), c[3], tol_for(c[3])) for c in s["y3_checks"]]
+    out += [(f"|y4|{c[:3]}", _at(n4, *c[:3]), c[3], tol_for(c[3])) for c in s["y4_checks"]]
+    return out
+
+
+kat_ictgv_scaled_gradient_norm = kat_ictgv_gradient_norm
+
+
+def kat_ictgv_divergence(be, s):
+    W, H, T = s["dims"]
+    u, v, w, v2 = _ictgv_inputs(W, H, T)
+    dd = s["d"]
+    d1 = [dd[0], dd[0], dd[1]]
+    d2 = [dd[2], dd[2], dd[3]]
+    a, a0, a1 = s["alpha"], s["alpha0"], s["alpha1"]
+    m = min(a, 1 - a)
+    y1 = be.prox3(be.grad(u - w, d1) - v, a1 * a / m)
+    y2 = be.prox6(be.symgrad(v, d1), a0 * a / m)
+    y3 = be.prox3(be.grad(w, d2) - v2, a1 * (1 - a) / m)
+    y4 = be.prox6(be.symgrad(v2, d2), a0 * (1 - a) / m)
+    dv1, dv3 = be.div(y1, d1), be.div(y3, d2)
+    sd2, sd4 = be.symdiv(y2, d1), be.symdiv(y4, d2)
+    out = [(f"div1{c[:3]}", _at(dv1, *c[:3]).real, c[3], EPS) for c in s["div1_checks"]]
+    out += [(f"div3{c[:3]}", _at(dv3, *c[:3]).real, c[3], EPS) for c in s["div3_checks"]]
+    out += [(f"symdiv2{c}", sd2[c[0], c[3], c[2], c[1]].real, c[4], EPS) for c in s["symdiv2_checks"]]
+    out += [(f"symdiv4{c}", sd4[c[0], c[3], c[2], c[1]].real, c[4], EPS) for c in s["symdiv4_checks"]]
+    return out
+
+
+def kat_ictgv2_norm(be, s):
+    W, H, T = s["dims"]
+    N = W * H * T
+    u0, w = img(W, H, T), img(W, H, T, N)
+    v, v2 = np.stack([u0] * 3), np.stack([w] * 3)
+    out = []
+    for d6, f, exp in s["cases"]:
+        u = u0 * ((1 + 2j) if f == 3.0 else 1.0)
+        g = be.ictgv2_norm(u, v, w, v2, s["alpha0"], s["alpha1"], s["alpha"], d6)
+        out.append((f"ictgv2norm{d6[:3]}", g, exp, tol_for(exp)))
+    return out
+
+
+def kat_cart_forward_operation(be, s):
+    W, H, T = s["dims"]
+    C = s["coils"]
+    k = AR[:W * H * C * T].reshape(T, C, H, W).copy()
+    x = np.abs(be.cart_KH(k, b1_const(C, H, W, s["b1"]), None, s["centered"]))
+    return [(f"|KHz|{c[:3]}", _at(x, *c[:3]), c[3], tol_for(c[3])) for c in s["checks"]]
+
+
+def kat_cart_backward_operation(be, s):
+    W, H, T = s["dims"]
+    C = s["coils"]
+    z = np.abs(be.cart_K(img(W, H, T), b1_const(C, H, W, s["b1"]), None, s["centered"]))
+    z = z.reshape(T * C, H, W)
+    return [(f"|Kx|{c[:3]}", z[c[2], c[1], c[0]], c[3], tol_for(c[3])) for c in s["checks"]]
+
+
+def kat_fft_centered_forward(be, s):
+    W, H = s["dims"]
+    a = AR[:W * H].reshape(H, W).copy()
+    f = np.abs(be.fft2(a, -1, True)) / math.sqrt(W * H)
+    return [(f"|F|{c[:2]}", f[c[1], c[0]], c[2], tol_for(c[2])) for c in s["checks"]]
+
+
+def kat_ellipke(be, s):
+    return [(f"E({m})", be.ellipke(m), e, EPS) for m, e in s["cases"]]
+
+
+def kat_noncart_adapt_lambda(be, s):
+    lam = be.adapt_lambda_noncart(s["width"], s["spokes_per_frame"], s["k"], s["d"])
+    return [("lambda", lam, s["expected"], EPS)]
+
+
+NAMES = [k for k in KATS if not k.startswith("_") and k != "eps"]
