@@ -1,0 +1,485 @@
+"""MI355X-native AVIONIC primal-dual reconstruction (Python host mirror).
+
+Python front end over ``libavionic.so`` (HIP kernels for gfx950) through its
+C-ABI (``include/avionic.h``).  The classes mirror the reference's C++ API:
+
+* ``CartesianOperator`` / ``NoncartesianOperator``  — ``BaseOperator``
+  (CUDA/include/base_operator.h:24-112): ``ForwardOperation`` (k-space ->
+  image, K^H), ``BackwardOperation`` (image -> k-space, K), ``AdaptLambda``.
+* ``TV`` / ``TGV2`` / ``ICTGV2``  — ``PDRecon`` (CUDA/include/pd_recon.h:88-185):
+  ``IterativeReconstruction``, ``GetParams``, ``AdaptLambda``, ``SetVerbose``,
+  ``SetDebug``, ``AdaptStepSize``, ``ComputeGStar``, ``ComputePDGap``,
+  ``ExportAdditionalResults``.
+* ``gradient``, ``symmetric_gradient``, ``divergence``, ... — ``utils::``
+  (CUDA/include/utils.h).
+
+Device memory is handled with torch tensors (complex64 / float32 on
+``cuda``); torch is plumbing only, every computation runs in libavionic.
+There is no CPU fallback: without the built library or a HIP device the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libavionic.so")
+
+TV_METHOD, TGV2_METHOD, ICTGV2_METHOD = 0, 1, 2
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U = ctypes.c_uint
+_D = ctypes.c_double
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+
+
+class AvionicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"avionic error {code}: {msg}")
+        self.code = code
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("maxIt", ctypes.c_uint),
+        ("dx", _D), ("dy", _D), ("dz", _D),
+        ("stopPDGap", _D),
+        ("ds", _D), ("dt", _D),
+        ("sigma", _D), ("tau", _D), ("sigmaTauRatio", _D),
+        ("timeSpaceWeight", _D),
+        ("lambda", _D),
+        ("adaptLambda", _I),
+        ("adaptLambdaK", _D), ("adaptLambdaD", _D),
+        ("alpha0", _D), ("alpha1", _D),
+        ("alpha", _D),
+        ("timeSpaceWeight2", _D),
+        ("dx2", _D), ("dy2", _D), ("dt2", _D),
+        ("adaptSchedule", _I),
+        ("dualStorage", _I),
+    ]
+
+
+def _to_dict(p: _Params) -> dict:
+    return {name: getattr(p, name) for name, _ in _Params._fields_}
+
+
+def _from_dict(d: dict) -> _Params:
+    p = _Params()
+    for name, typ in _Params._fields_:
+        if name in d:
+            setattr(p, name, typ(d[name]).value if typ in (_I, ctypes.c_uint) else float(d[name]))
+    return p
+
+
+_EXPORT_CB = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.POINTER(ctypes.c_float), _SZ, _P)
+
+_lib = None
+
+
+def lib():
+    """Load libavionic.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m avionic_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    sig = {
+        "avionic_last_error": (ctypes.c_char_p, []),
+        "avionic_version": (ctypes.c_char_p, []),
+        "avionic_cartesian_operator_create": (_I, [_U, _U, _U, _U, _P, _I, ctypes.POINTER(_P)]),
+        "avionic_noncartesian_operator_create": (_I, [_U, _U, _U, _U, _U, _U, _P, _P, _I, _D,
+                                                      ctypes.POINTER(_P)]),
+        "avionic_operator_destroy": (_I, [_P]),
+        "avionic_operator_forward": (_I, [_P, _P, _P, _P, _P]),
+        "avionic_operator_backward": (_I, [_P, _P, _P, _P, _P]),
+        "avionic_operator_adapt_lambda": (_I, [_P, _D, _D, ctypes.POINTER(_D)]),
+        "avionic_operator_kspace_size": (_SZ, [_P]),
+        "avionic_default_params": (_I, [_I, ctypes.POINTER(_Params)]),
+        "avionic_recon_create": (_I, [_I, _U, _U, _U, _U, ctypes.POINTER(_Params), _P,
+                                      ctypes.POINTER(_P)]),
+        "avionic_recon_destroy": (_I, [_P]),
+        "avionic_recon_get_params": (_I, [_P, ctypes.POINTER(_Params)]),
+        "avionic_recon_set_params": (_I, [_P, ctypes.POINTER(_Params)]),
+        "avionic_recon_set_verbose": (_I, [_P, _I]),
+        "avionic_recon_set_debug": (_I, [_P, _I, _I]),
+        "avionic_recon_adapt_lambda": (_I, [_P, _D, _D, ctypes.POINTER(_D)]),
+        "avionic_recon_run": (_I, [_P, _P, _P, _P, _P]),
+        "avionic_recon_adapt_step_size": (_I, [_P, _P, _P, _P]),
+        "avionic_recon_compute_gstar": (_I, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
+        "avionic_recon_compute_pdgap": (_I, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
+        "avionic_recon_get_step_log": (_I, [_P, _P, _P, ctypes.POINTER(_SZ)]),
+        "avionic_recon_export": (_I, [_P, _EXPORT_CB, _P]),
+        "avionic_gradient": (_I, [_P, _P, _U, _U, _U, _F, _F, _F, _P]),
+        "avionic_symmetric_gradient": (_I, [_P, _P, _U, _U, _U, _F, _F, _F, _P]),
+        "avionic_divergence": (_I, [_P, _P, _U, _U, _U, _F, _F, _F, _P]),
+        "avionic_symmetric_divergence": (_I, [_P, _P, _U, _U, _U, _F, _F, _F, _P]),
+        "avionic_proximal_map3": (_I, [_P, _SZ, _F, _P]),
+        "avionic_proximal_map6": (_I, [_P, _SZ, _F, _P]),
+        "avionic_tv_norm": (_I, [_P, _U, _U, _U, _F, _F, _F, ctypes.POINTER(_D), _P]),
+        "avionic_tgv2_norm": (_I, [_P, _P, _U, _U, _U, _D, _D, _F, _F, _F, ctypes.POINTER(_D), _P]),
+        "avionic_ictgv2_norm": (_I, [_P, _P, _P, _P, _U, _U, _U, _D, _D, _D, _F, _F, _F, _F, _F, _F,
+                                     ctypes.POINTER(_D), _P]),
+        "avionic_ellipke": (_D, [_D]),
+        "avionic_time_space_weights": (_I, [_D, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+        "avionic_fft2": (_I, [_P, _U, _U, _U, _I, _I, _I, _P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _check(rc):
+    if rc != 0:
+        raise AvionicError(rc, lib().avionic_last_error().decode())
+
+
+def version() -> str:
+    return lib().avionic_version().decode()
+
+
+# ------------------------------------------------------------------ plumbing
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t, dtype=None):
+    """Device pointer of a contiguous cuda tensor (checked)."""
+    if t is None:
+        return None
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("expected a cuda tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _c64(t):
+    return _dptr(t, _torch().complex64)
+
+
+def _f32(t):
+    return _dptr(t, _torch().float32)
+
+
+def _empty_c64(*shape):
+    torch = _torch()
+    return torch.empty(*shape, dtype=torch.complex64, device="cuda")
+
+
+# ----------------------------------------------------------------- operators
+class BaseOperator:
+    """BaseOperator (CUDA/include/base_operator.h:24-112)."""
+
+    def __init__(self, handle, width, height, coils, frames):
+        self._h = handle
+        self.width, self.height, self.coils, self.frames = width, height, coils, frames
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().avionic_operator_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def kspace_shape(self):
+        n = lib().avionic_operator_kspace_size(self._h)
+        if isinstance(self, CartesianOperator):
+            return (self.frames, self.coils, self.height, self.width)
+        return (self.frames, self.coils, n)
+
+    def ForwardOperation(self, kdata, b1, out=None):
+        """k-space -> coil-combined image (K^H)."""
+        if out is None:
+            out = _empty_c64(self.frames, self.height, self.width)
+        _check(lib().avionic_operator_forward(self._h, _c64(kdata), _c64(out), _c64(b1), _stream()))
+        return out
+
+    def BackwardOperation(self, x, b1, out=None):
+        """image -> coil-wise k-space (K)."""
+        if out is None:
+            out = _empty_c64(*self.kspace_shape())
+        _check(lib().avionic_operator_backward(self._h, _c64(x), _c64(out), _c64(b1), _stream()))
+        return out
+
+    def AdaptLambda(self, k, d):
+        lam = ctypes.c_double()
+        _check(lib().avionic_operator_adapt_lambda(self._h, float(k), float(d), ctypes.byref(lam)))
+        return lam.value
+
+
+class CartesianOperator(BaseOperator):
+    """CartesianOperator(width, height, coils, frames[, mask][, centered])
+    (CUDA/include/cartesian_operator.h:18-25).  Without a mask the operator is
+    fully sampled (ZeroMask); ``centered`` defaults to True as in the
+    reference's (W, H, C, T[, mask]) constructors."""
+
+    def __init__(self, width, height, coils, frames, mask=None, centered=True):
+        h = ctypes.c_void_p()
+        _check(lib().avionic_cartesian_operator_create(width, height, coils, frames, _f32(mask),
+                                                       int(bool(centered)), ctypes.byref(h)))
+        super().__init__(h, width, height, coils, frames)
+        self.centered = bool(centered)
+
+
+class NoncartesianOperator(BaseOperator):
+    """NoncartesianOperator (CUDA/include/noncartesian_operator.h): radial /
+    arbitrary trajectories by Kaiser-Bessel gridding.  traj: float32
+    [frames, 2, spokesPerFrame*nFE] in [-0.5, 0.5); dens: float32
+    [frames, spokesPerFrame*nFE] or None."""
+
+    def __init__(self, width, height, coils, frames, nFE, spokesPerFrame, traj, dens=None,
+                 kernel_width=3, osf=1.5):
+        h = ctypes.c_void_p()
+        _check(lib().avionic_noncartesian_operator_create(width, height, coils, frames, nFE,
+                                                          spokesPerFrame, _f32(traj), _f32(dens),
+                                                          int(kernel_width), float(osf),
+                                                          ctypes.byref(h)))
+        super().__init__(h, width, height, coils, frames)
+        self.nFE, self.spokesPerFrame = nFE, spokesPerFrame
+
+
+def noncartesian_adapt_lambda(width, spokes_per_frame, k, d):
+    """NoncartesianOperator::AdaptLambda (noncartesian_operator.cpp:88-95)
+    evaluated by the library on a minimal operator of that geometry."""
+    torch = _torch()
+    nFE = width
+    traj = torch.zeros(1, 2, spokes_per_frame * nFE, dtype=torch.float32, device="cuda")
+    op = NoncartesianOperator(width, width, 1, 1, nFE, spokes_per_frame, traj)
+    return op.AdaptLambda(k, d)
+
+
+# ------------------------------------------------------------------- solvers
+class PDRecon:
+    """PDRecon (CUDA/include/pd_recon.h:88-185)."""
+
+    METHOD = None
+
+    def __init__(self, width, height, coils, frames, mrOp, params=None):
+        self.width, self.height, self.coils, self.frames = width, height, coils, frames
+        self.op = mrOp
+        h = ctypes.c_void_p()
+        pp = None
+        if params is not None:
+            base = _Params()
+            _check(lib().avionic_default_params(self.METHOD, ctypes.byref(base)))
+            merged = _to_dict(base)
+            merged.update(params)
+            pp = ctypes.byref(_from_dict(merged))
+        _check(lib().avionic_recon_create(self.METHOD, width, height, coils, frames, pp,
+                                          mrOp.handle, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().avionic_recon_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def N(self):
+        return self.width * self.height * self.frames
+
+    def GetParams(self) -> dict:
+        p = _Params()
+        _check(lib().avionic_recon_get_params(self._h, ctypes.byref(p)))
+        return _to_dict(p)
+
+    def SetParams(self, params: dict):
+        cur = self.GetParams()
+        cur.update(params)
+        _check(lib().avionic_recon_set_params(self._h, ctypes.byref(_from_dict(cur))))
+
+    def AdaptLambda(self, k, d):
+        lam = ctypes.c_double()
+        _check(lib().avionic_recon_adapt_lambda(self._h, float(k), float(d), ctypes.byref(lam)))
+        return lam.value
+
+    def SetVerbose(self, verbose):
+        _check(lib().avionic_recon_set_verbose(self._h, int(bool(verbose))))
+
+    def SetDebug(self, debug, debugstep=1):
+        _check(lib().avionic_recon_set_debug(self._h, int(bool(debug)), int(debugstep)))
+
+    def IterativeReconstruction(self, data, x, b1):
+        """x is updated in place (initial image in, reconstruction out)."""
+        _check(lib().avionic_recon_run(self._h, _c64(data), _c64(x), _c64(b1), _stream()))
+        return x
+
+    def AdaptStepSize(self, delta, b1):
+        _check(lib().avionic_recon_adapt_step_size(self._h, _c64(delta), _c64(b1), _stream()))
+
+    def ComputeGStar(self, x1, y, z, data, b1):
+        out = ctypes.c_double()
+        _check(lib().avionic_recon_compute_gstar(self._h, _c64(x1), _c64(y), _c64(z), _c64(data),
+                                                 _c64(b1), ctypes.byref(out), _stream()))
+        return out.value
+
+    def ComputePDGap(self, x, y, z, data, b1):
+        out = ctypes.c_double()
+        _check(lib().avionic_recon_compute_pdgap(self._h, _c64(x), _c64(y), _c64(z), _c64(data),
+                                                 _c64(b1), ctypes.byref(out), _stream()))
+        return out.value
+
+    def _step_log(self):
+        n = ctypes.c_size_t(1 << 30)
+        _check(lib().avionic_recon_get_step_log(self._h, None, None, ctypes.byref(n)))
+        s = np.zeros(max(n.value, 1))
+        t = np.zeros(max(n.value, 1))
+        m = ctypes.c_size_t(n.value)
+        _check(lib().avionic_recon_get_step_log(self._h, s.ctypes.data_as(_P), t.ctypes.data_as(_P),
+                                                ctypes.byref(m)))
+        return s[: m.value], t[: m.value]
+
+    def sigma_log(self):
+        return self._step_log()[0]
+
+    def tau_log(self):
+        return self._step_log()[1]
+
+    def ExportAdditionalResults(self, outputDir, callback):
+        """callback(outputDir, name, numpy complex64 array)."""
+        def _cb(name, data, count, _user):
+            arr = np.ctypeslib.as_array(data, shape=(2 * count,)).copy()
+            callback(outputDir, name.decode(), arr.view(np.complex64))
+        cb = _EXPORT_CB(_cb)
+        _check(lib().avionic_recon_export(self._h, cb, None))
+
+
+class TV(PDRecon):
+    """TV (CUDA/include/tv.h)."""
+    METHOD = TV_METHOD
+
+
+class TGV2(PDRecon):
+    """TGV2 (CUDA/include/tgv2.h)."""
+    METHOD = TGV2_METHOD
+
+
+class ICTGV2(PDRecon):
+    """ICTGV2 (CUDA/include/ictgv2.h)."""
+    METHOD = ICTGV2_METHOD
+
+
+def default_params(method) -> dict:
+    p = _Params()
+    _check(lib().avionic_default_params(int(method), ctypes.byref(p)))
+    return _to_dict(p)
+
+
+# ------------------------------------------------------------------- utils::
+def _dims(t, lead):
+    s = tuple(t.shape)
+    T, H, W = s[lead:]
+    return W, H, T
+
+
+def gradient(u, dx=1.0, dy=1.0, dt=1.0):
+    W, H, T = _dims(u, 0)
+    out = _empty_c64(3, T, H, W)
+    _check(lib().avionic_gradient(_c64(u), _c64(out), W, H, T, dx, dy, dt, _stream()))
+    return out
+
+
+def symmetric_gradient(v, dx=1.0, dy=1.0, dt=1.0):
+    W, H, T = _dims(v, 1)
+    out = _empty_c64(6, T, H, W)
+    _check(lib().avionic_symmetric_gradient(_c64(v), _c64(out), W, H, T, dx, dy, dt, _stream()))
+    return out
+
+
+def divergence(p, dx=1.0, dy=1.0, dt=1.0):
+    W, H, T = _dims(p, 1)
+    out = _empty_c64(T, H, W)
+    _check(lib().avionic_divergence(_c64(p), _c64(out), W, H, T, dx, dy, dt, _stream()))
+    return out
+
+
+def symmetric_divergence(q, dx=1.0, dy=1.0, dt=1.0):
+    W, H, T = _dims(q, 1)
+    out = _empty_c64(3, T, H, W)
+    _check(lib().avionic_symmetric_divergence(_c64(q), _c64(out), W, H, T, dx, dy, dt, _stream()))
+    return out
+
+
+def proximal_map3(y, scale):
+    """In place: y /= max(1, |y|_3 * scale) (utils::ProximalMap3)."""
+    n = y[0].numel()
+    _check(lib().avionic_proximal_map3(_c64(y), n, float(scale), _stream()))
+    return y
+
+
+def proximal_map6(q, scale):
+    n = q[0].numel()
+    _check(lib().avionic_proximal_map6(_c64(q), n, float(scale), _stream()))
+    return q
+
+
+def tv_norm(u, dx, dy, dt):
+    W, H, T = _dims(u, 0)
+    out = ctypes.c_double()
+    _check(lib().avionic_tv_norm(_c64(u), W, H, T, dx, dy, dt, ctypes.byref(out), _stream()))
+    return out.value
+
+
+def tgv2_norm(u, v, alpha0, alpha1, dx, dy, dt):
+    W, H, T = _dims(u, 0)
+    out = ctypes.c_double()
+    _check(lib().avionic_tgv2_norm(_c64(u), _c64(v), W, H, T, float(alpha0), float(alpha1), dx, dy,
+                                   dt, ctypes.byref(out), _stream()))
+    return out.value
+
+
+def ictgv2_norm(u, v, w, v2, alpha0, alpha1, alpha, dx, dy, dt, dx2, dy2, dt2):
+    W, H, T = _dims(u, 0)
+    out = ctypes.c_double()
+    _check(lib().avionic_ictgv2_norm(_c64(u), _c64(v), _c64(w), _c64(v2), W, H, T, float(alpha0),
+                                     float(alpha1), float(alpha), dx, dy, dt, dx2, dy2, dt2,
+                                     ctypes.byref(out), _stream()))
+    return out.value
+
+
+def ellipke(m):
+    return lib().avionic_ellipke(float(m))
+
+
+def time_space_weights(tsw):
+    ds, dt = ctypes.c_double(), ctypes.c_double()
+    _check(lib().avionic_time_space_weights(float(tsw), ctypes.byref(ds), ctypes.byref(dt)))
+    return ds.value, dt.value
+
+
+def fft2(a, sign=-1, centered=False, unitary=False):
+    """In-place 2D FFT over the last two axes of a complex64 cuda tensor."""
+    H, W = a.shape[-2], a.shape[-1]
+    batch = int(a.numel() // (H * W))
+    _check(lib().avionic_fft2(_c64(a), H, W, batch, int(sign), int(bool(centered)),
+                              int(bool(unitary)), _stream()))
+    return a

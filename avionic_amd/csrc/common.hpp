@@ -1,0 +1,111 @@
+// Shared device/host definitions of libavionic (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace avionic {
+
+// ----------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define AV_HIP(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw ::avionic::Error(-2, std::string(#expr) + ": " +                  \
+                                     hipGetErrorString(e_));                  \
+  } while (0)
+
+#define AV_CHECK(cond, msg)                                                   \
+  do {                                                                        \
+    if (!(cond)) throw ::avionic::Error(-1, msg);                             \
+  } while (0)
+
+#define AV_LAUNCH_CHECK() AV_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- complex
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__host__ __device__ inline float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+__host__ __device__ inline float2 czero() { return make_float2(0.f, 0.f); }
+
+// ------------------------------------------------------ dual storage types
+// bfloat16 complex pair: round-to-nearest-even from float32.
+struct bf16x2 {
+  uint16_t re, im;
+};
+
+__host__ __device__ inline uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+__host__ __device__ inline float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t u = f2u(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN
+  uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h) { return u2f(((uint32_t)h) << 16); }
+
+template <typename DT>
+struct DualIO;
+template <>
+struct DualIO<float2> {
+  __device__ static inline float2 load(const float2 *p, long i) { return p[i]; }
+  __device__ static inline void store(float2 *p, long i, float2 v) { p[i] = v; }
+};
+template <>
+struct DualIO<bf16x2> {
+  __device__ static inline float2 load(const bf16x2 *p, long i) {
+    uint32_t w = *reinterpret_cast<const uint32_t *>(p + i);
+    return make_float2(u2f(w << 16), u2f(w & 0xffff0000u));
+  }
+  __device__ static inline void store(bf16x2 *p, long i, float2 v) {
+    uint32_t w = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
+    *reinterpret_cast<uint32_t *>(p + i) = w;
+  }
+};
+
+// ------------------------------------------------------------ FFT tiling
+// Elements (complex) held in LDS by one FFT tile, and threads per tile.
+constexpr int kFftThreads = 256;
+constexpr int kFftTileElems = 4096;  // 32 KiB of float2
+constexpr int kMaxStages = 12;
+
+// A 1D FFT plan for one length: Stockham stages (radix, Ns) and twiddles.
+struct FftPlan {
+  int L;
+  int nst;
+  int radix[kMaxStages];
+  int ns[kMaxStages];
+  const float2 *tw;  // tw[m] = exp(-2 pi i m / L), m < L (device)
+};
+
+// ----------------------------------------------------- reductions / state
+constexpr int kRedThreads = 256;
+
+// Per-reconstruction scalars kept on the device so that the step-size rule
+// runs without a host round trip (and inside captured graphs).
+struct DevScalars {
+  double sigma, tau, lambda, ratio;
+  double nKx2, nx2;  // last step-rule reductions
+  int iter;          // iteration counter (incremented once per iteration)
+  int diverged;      // set when a reduction produced NaN/Inf
+};
+
+}  // namespace avionic

@@ -1,0 +1,435 @@
+// Cartesian operator kernels: column pass, row pass (with the fused z dual
+// update and the adjoint row transform), coil-summing adjoint column pass.
+#include <cmath>
+#include <vector>
+
+#include "cartesian.hpp"
+#include "fft_device.hpp"
+
+namespace avionic {
+
+constexpr int NT = kFftThreads;
+constexpr int EMAX = kFftTileElems;
+
+__device__ inline long kidx(const CartGeom &g, int t, int c, int k, int x) {
+  const int r0 = g.row_off[t];
+  const int S = g.row_off[t + 1] - r0;
+  return ((long)g.C * r0 + (long)c * S + k) * g.W + x;
+}
+
+// ---------------------------------------------------------------------------
+// Column pass of K.  grid (nxb, T); each block holds an H x XB tile and loops
+// over the coils: tile = b1_c . u (ifftshifted in y and x when centred),
+// inverse FFT along y, keep the sampled rows (fftshifted in y).  The image
+// tile is re-read per coil from L2; HBM sees it once.
+__global__ __launch_bounds__(NT) void k_col_forward(CartGeom g, const float2 *__restrict__ u,
+                                                    const float2 *__restrict__ usub,
+                                                    const float2 *__restrict__ b1,
+                                                    float2 *__restrict__ kcol) {
+  __shared__ float2 tile[EMAX];
+  const int tid = threadIdx.x;
+  const int xb0 = blockIdx.x * g.XB;
+  const int t = blockIdx.y;
+  const int H = g.H, W = g.W, XB = g.XB;
+  const int HX = H * XB;
+  const long HW = (long)H * W;
+  const float2 *ut = u + (long)t * HW;
+  const float2 *st = usub ? usub + (long)t * HW : nullptr;
+  const int r0 = g.row_off[t];
+  const int S = g.row_off[t + 1] - r0;
+  const int hc = (H + 1) / 2;
+  for (int c = 0; c < g.C; ++c) {
+    const float2 *bc = b1 + (long)c * HW;
+    for (int e = tid; e < HX; e += NT) {
+      const int y = e / XB, x = xb0 + e % XB;
+      float2 v = czero();
+      if (x < W) {
+        const int sy = g.centered ? (y + H / 2) % H : y;
+        const int sx = g.centered ? (x + W / 2) % W : x;
+        const long si = (long)sy * W + sx;
+        const float2 uu = st ? csub(ut[si], st[si]) : ut[si];
+        v = cmul(bc[si], uu);
+      }
+      tile[e] = v;
+    }
+    __syncthreads();
+    fft_lds<NT, EMAX, true>(tile, g.ph, XB, 1, XB, +1.f, tid);
+    for (int e = tid; e < S * XB; e += NT) {
+      const int k = e / XB, xb = e % XB, x = xb0 + xb;
+      if (x < W) {
+        const int ky = g.rows[r0 + k];
+        const int src = g.centered ? (ky + hc) % H : ky;
+        // x stays in ifftshifted order; the row pass fftshifts its output.
+        kcol[kidx(g, t, c, k, x)] = tile[src * XB + xb];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass.  Each block transforms RB compact rows of length W along x.  The
+// adjoint's row transform (modes DualUpdate / AdjDense / AdjCompact) uses a
+// second LDS tile.
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_row_pass(CartGeom g, const float2 *__restrict__ kcol,
+                                                 float2 *__restrict__ z,
+                                                 const float2 *__restrict__ d,
+                                                 float2 *__restrict__ zhat,
+                                                 float2 *__restrict__ out,
+                                                 const float2 *__restrict__ zin,
+                                                 const DevScalars *__restrict__ sc,
+                                                 double *__restrict__ partial) {
+  __shared__ float2 tile[EMAX];
+  __shared__ float2 tile2[(MODE == kRowDualUpdate) ? EMAX : 1];
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  const int W = g.W;
+  const long g0 = (long)blockIdx.x * g.RB;
+  const int nr = (int)min((long)g.RB, g.nrows_c - g0);
+  const int n = nr * W;
+  const int wc = (W + 1) / 2;
+  double acc = 0.0;
+  float2 *adj = (MODE == kRowDualUpdate) ? tile2 : tile;
+
+  if (MODE == kRowAdjDense || MODE == kRowAdjCompact) {
+    // a[x'] = (M z)[(x' + W/2) % W] when centred.
+    for (int e = tid; e < n; e += NT) {
+      const int r = e / W, xp = e % W;
+      const int4 ri = g.rowinfo[g0 + r];
+      const int kx = g.centered ? (xp + W / 2) % W : xp;
+      float2 v;
+      if (MODE == kRowAdjDense)
+        v = zin[(((long)ri.x * g.C + ri.y) * g.H + ri.w) * W + kx];
+      else
+        v = zin[(g0 + r) * W + kx];
+      tile[e] = cscale(v, g.mrows[(long)ri.z * W + kx]);
+    }
+    __syncthreads();
+  } else {
+    for (int e = tid; e < n; e += NT) tile[e] = kcol[g0 * W + e];
+    __syncthreads();
+    fft_lds<NT, EMAX, false>(tile, g.pw, nr, W, 1, +1.f, tid);
+    float sigma = 0.f, inv = 0.f;
+    if (MODE == kRowDualUpdate) {
+      sigma = (float)sc->sigma;
+      inv = (float)(1.0 / (1.0 + sc->sigma / sc->lambda));
+    }
+    for (int e = tid; e < n; e += NT) {
+      const int r = e / W, kx = e % W;
+      const int src = g.centered ? (kx + wc) % W : kx;
+      const int4 ri = g.rowinfo[g0 + r];
+      const float mk = g.mrows[(long)ri.z * W + kx];
+      const float2 val = cscale(tile[r * W + src], g.scale * mk);
+      const long gi = (g0 + r) * W + kx;
+      if (MODE == kRowApplyK) {
+        if (out)
+          out[(((long)ri.x * g.C + ri.y) * g.H + ri.w) * W + kx] = val;
+        else
+          zhat[gi] = val;
+      } else if (MODE == kRowNorm2) {
+        acc += (double)cabs2(val);
+      } else if (MODE == kRowResidual) {
+        acc += (double)cabs2(csub(val, d[gi]));
+      } else if (MODE == kRowDualUpdate) {
+        float2 zz = z[gi];
+        zz = cadd(zz, cscale(val, sigma));
+        zz = cscale(csub(zz, cscale(d[gi], sigma)), inv);
+        z[gi] = zz;
+        const int xp = g.centered ? (kx + wc) % W : kx;
+        tile2[r * W + xp] = cscale(zz, mk);
+      }
+    }
+    if (MODE == kRowDualUpdate) __syncthreads();
+  }
+
+  if (MODE == kRowDualUpdate || MODE == kRowAdjDense || MODE == kRowAdjCompact) {
+    fft_lds<NT, EMAX, false>(adj, g.pw, nr, W, 1, -1.f, tid);
+    for (int e = tid; e < n; e += NT) {
+      const int r = e / W, kxp = e % W;
+      const int src = g.centered ? (kxp + wc) % W : kxp;
+      zhat[g0 * W + e] = adj[r * W + src];
+    }
+  }
+
+  if (MODE == kRowNorm2 || MODE == kRowResidual) {
+    red[tid] = acc;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) partial[blockIdx.x] = red[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass of K^H with the coil reduction.  grid (nxb, T).  The coil sum
+// accumulates in a second LDS tile.
+__global__ __launch_bounds__(NT) void k_col_adjoint(CartGeom g, const float2 *__restrict__ zhat,
+                                                    const float2 *__restrict__ b1,
+                                                    float2 *__restrict__ out) {
+  __shared__ float2 tile[EMAX];
+  __shared__ float2 acc[EMAX];
+  const int tid = threadIdx.x;
+  const int xb0 = blockIdx.x * g.XB;
+  const int t = blockIdx.y;
+  const int H = g.H, W = g.W, XB = g.XB;
+  const int HX = H * XB;
+  const long HW = (long)H * W;
+  const int r0 = g.row_off[t];
+  const int S = g.row_off[t + 1] - r0;
+  const int hc = (H + 1) / 2;
+  for (int e = tid; e < HX; e += NT) acc[e] = czero();
+  for (int c = 0; c < g.C; ++c) {
+    for (int e = tid; e < HX; e += NT) tile[e] = czero();
+    __syncthreads();
+    for (int e = tid; e < S * XB; e += NT) {
+      const int k = e / XB, xb = e % XB, x = xb0 + xb;
+      if (x < W) {
+        const int ky = g.rows[r0 + k];
+        const int py = g.centered ? (ky + hc) % H : ky;
+        tile[py * XB + xb] = zhat[kidx(g, t, c, k, x)];
+      }
+    }
+    __syncthreads();
+    fft_lds<NT, EMAX, true>(tile, g.ph, XB, 1, XB, -1.f, tid);
+    const float2 *bc = b1 + (long)c * HW;
+    for (int e = tid; e < HX; e += NT) {
+      const int y = e / XB, x = xb0 + e % XB;
+      if (x < W) {
+        const int src = g.centered ? (y + hc) % H : y;
+        acc[e] = cadd(acc[e], cmulc(bc[(long)y * W + x], tile[src * XB + e % XB]));
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < HX; e += NT) {
+    const int y = e / XB, x = xb0 + e % XB;
+    if (x < W) out[(long)t * HW + (long)y * W + x] = cscale(acc[e], g.scale);
+  }
+}
+
+// Dense [t][c][H][W] -> compact rows, times the mask.
+__global__ void k_compact_gather(CartGeom g, const float2 *__restrict__ dense,
+                                 float2 *__restrict__ compact) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = g.nrows_c * g.W;
+  if (i >= n) return;
+  const long row = i / g.W;
+  const int x = (int)(i % g.W);
+  const int4 ri = g.rowinfo[row];
+  compact[i] = cscale(dense[(((long)ri.x * g.C + ri.y) * g.H + ri.w) * g.W + x],
+                      g.mrows[(long)ri.z * g.W + x]);
+}
+
+// Standalone batched 2D FFT (used by avionic_fft2, the AGILE FFT
+// replacement): rows out of place (the centred y shift permutes rows between
+// blocks), then columns in place.
+__global__ __launch_bounds__(NT) void k_fft_rows_oop(const float2 *in, float2 *outp, FftPlan pw,
+                                                     int H, int W, int RB, int sign,
+                                                     int centered) {
+  __shared__ float2 tile[EMAX];
+  const int tid = threadIdx.x;
+  const long img = blockIdx.y;
+  const int y0 = blockIdx.x * RB;
+  const int nr = min(RB, H - y0);
+  const float2 *base = in + img * (long)H * W;
+  float2 *ob = outp + img * (long)H * W;
+  for (int e = tid; e < nr * W; e += NT) {
+    const int r = e / W, x = e % W;
+    const int y = y0 + r;
+    const int sy = centered ? (y + H / 2) % H : y;
+    const int sx = centered ? (x + W / 2) % W : x;
+    tile[e] = base[(long)sy * W + sx];
+  }
+  __syncthreads();
+  fft_lds<NT, EMAX, false>(tile, pw, nr, W, 1, (float)sign, tid);
+  const int wc = (W + 1) / 2;
+  for (int e = tid; e < nr * W; e += NT) {
+    const int r = e / W, kx = e % W;
+    const int src = centered ? (kx + wc) % W : kx;
+    ob[(long)(y0 + r) * W + kx] = tile[r * W + src];
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_fft_cols(float2 *data, FftPlan ph, int H, int W, int XB,
+                                                 int sign, int centered, float scale) {
+  __shared__ float2 tile[EMAX];
+  const int tid = threadIdx.x;
+  const long img = blockIdx.y;
+  const int xb0 = blockIdx.x * XB;
+  float2 *base = data + img * (long)H * W;
+  for (int e = tid; e < H * XB; e += NT) {
+    const int y = e / XB, x = xb0 + e % XB;
+    tile[e] = (x < W) ? base[(long)y * W + x] : czero();
+  }
+  __syncthreads();
+  fft_lds<NT, EMAX, true>(tile, ph, XB, 1, XB, (float)sign, tid);
+  const int hc = (H + 1) / 2;
+  for (int e = tid; e < H * XB; e += NT) {
+    const int y = e / XB, x = xb0 + e % XB;
+    const int src = centered ? (y + hc) % H : y;
+    if (x < W) base[(long)y * W + x] = cscale(tile[src * XB + e % XB], scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+static int pick_xb(int H, int W) {
+  int xb = 1;
+  while (xb * 2 * H <= EMAX && xb * 2 <= 64 && xb < W) xb *= 2;
+  return xb;
+}
+
+void make_fft_plan(int L, FftPlan &p, void **d_tw) {
+  AV_CHECK(L >= 1 && L <= EMAX, "FFT length out of range");
+  p = FftPlan{};
+  p.L = L;
+  int m = L, ns = 1;
+  const int order[] = {16, 8, 4, 2, 3, 5, 7};
+  for (int r : order) {
+    while (m % r == 0) {
+      AV_CHECK(p.nst < kMaxStages, "FFT plan too long");
+      p.radix[p.nst] = r;
+      p.ns[p.nst] = ns;
+      ns *= r;
+      m /= r;
+      p.nst++;
+    }
+  }
+  AV_CHECK(m == 1, "FFT length must factor into 2, 3, 5 and 7");
+  std::vector<float2> tw(L);
+  for (int k = 0; k < L; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)L;
+    tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  AV_HIP(hipMalloc(d_tw, sizeof(float2) * L));
+  AV_HIP(hipMemcpy(*d_tw, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+  p.tw = (const float2 *)*d_tw;
+}
+
+CartPlan::~CartPlan() {
+  for (void *p : {d_row_off, d_rows, d_mrows, d_rowinfo, d_twx, d_twy})
+    if (p) (void)hipFree(p);
+}
+
+void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered, hipStream_t s) {
+  AV_CHECK(W >= 1 && H >= 1 && C >= 1 && T >= 1, "invalid operator dimensions");
+  AV_CHECK(H <= EMAX && W <= EMAX, "image dimensions exceed the FFT tile");
+  g = CartGeom{};
+  g.W = W; g.H = H; g.C = C; g.T = T;
+  g.centered = centered ? 1 : 0;
+  g.XB = pick_xb(H, W);
+  g.RB = EMAX / W;
+  g.nxb = (W + g.XB - 1) / g.XB;
+  g.scale = (float)(1.0 / std::sqrt((double)H * W));
+  const long HW = (long)H * W;
+  std::vector<float> hm;
+  has_mask = mask != nullptr;
+  if (mask) {
+    hm.resize((size_t)HW * T);
+    AV_HIP(hipMemcpyAsync(hm.data(), mask, sizeof(float) * hm.size(), hipMemcpyDeviceToHost, s));
+    AV_HIP(hipStreamSynchronize(s));
+  }
+  h_row_off.assign(T + 1, 0);
+  h_rows.clear();
+  std::vector<float> mrows;
+  mask_norm2 = 0.0;
+  for (int t = 0; t < T; ++t) {
+    for (int y = 0; y < H; ++y) {
+      bool any = !mask;
+      if (mask)
+        for (int x = 0; x < W; ++x) {
+          const float v = hm[(size_t)t * HW + (size_t)y * W + x];
+          mask_norm2 += (double)v * v;
+          if (v != 0.f) any = true;
+        }
+      if (any) {
+        h_rows.push_back(y);
+        for (int x = 0; x < W; ++x) mrows.push_back(mask ? hm[(size_t)t * HW + (size_t)y * W + x] : 1.f);
+      }
+    }
+    h_row_off[t + 1] = (int)h_rows.size();
+  }
+  g.nrows = (long)h_rows.size();
+  g.nrows_c = g.nrows * C;
+  std::vector<int4> info;
+  info.reserve(g.nrows_c);
+  for (int t = 0; t < T; ++t) {
+    const int r0 = h_row_off[t], S = h_row_off[t + 1] - r0;
+    for (int c = 0; c < C; ++c)
+      for (int k = 0; k < S; ++k) info.push_back(make_int4(t, c, r0 + k, h_rows[r0 + k]));
+  }
+  auto up = [&](void **dst, const void *src, size_t bytes) {
+    AV_HIP(hipMalloc(dst, bytes > 0 ? bytes : 16));
+    if (bytes) AV_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  };
+  up(&d_row_off, h_row_off.data(), sizeof(int) * h_row_off.size());
+  up(&d_rows, h_rows.data(), sizeof(int) * h_rows.size());
+  up(&d_mrows, mrows.data(), sizeof(float) * mrows.size());
+  up(&d_rowinfo, info.data(), sizeof(int4) * info.size());
+  g.row_off = (const int *)d_row_off;
+  g.rows = (const int *)d_rows;
+  g.mrows = (const float *)d_mrows;
+  g.rowinfo = (const int4 *)d_rowinfo;
+  make_fft_plan(W, g.pw, &d_twx);
+  make_fft_plan(H, g.ph, &d_twy);
+}
+
+void launch_col_forward(const CartGeom &g, const float2 *u, const float2 *usub, const float2 *b1,
+                        float2 *kcol, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_forward, dim3(g.nxb, g.T), dim3(NT), 0, s, g, u, usub, b1, kcol);
+  AV_LAUNCH_CHECK();
+}
+
+void launch_row_pass(int mode, const CartGeom &g, const float2 *kcol, float2 *z, const float2 *d,
+                     float2 *zhat, float2 *out, const float2 *zin, const DevScalars *sc,
+                     double *partial, hipStream_t s) {
+  const int nb = (int)((g.nrows_c + g.RB - 1) / g.RB);
+  if (nb == 0) return;
+  dim3 grid(nb), block(NT);
+  switch (mode) {
+    case kRowApplyK: hipLaunchKernelGGL(k_row_pass<kRowApplyK>, grid, block, 0, s, g, kcol, z, d, zhat, out, zin, sc, partial); break;
+    case kRowDualUpdate: hipLaunchKernelGGL(k_row_pass<kRowDualUpdate>, grid, block, 0, s, g, kcol, z, d, zhat, out, zin, sc, partial); break;
+    case kRowNorm2: hipLaunchKernelGGL(k_row_pass<kRowNorm2>, grid, block, 0, s, g, kcol, z, d, zhat, out, zin, sc, partial); break;
+    case kRowAdjDense: hipLaunchKernelGGL(k_row_pass<kRowAdjDense>, grid, block, 0, s, g, kcol, z, d, zhat, out, zin, sc, partial); break;
+    case kRowResidual: hipLaunchKernelGGL(k_row_pass<kRowResidual>, grid, block, 0, s, g, kcol, z, d, zhat, out, zin, sc, partial); break;
+    case kRowAdjCompact: hipLaunchKernelGGL(k_row_pass<kRowAdjCompact>, grid, block, 0, s, g, kcol, z, d, zhat, out, zin, sc, partial); break;
+    default: AV_CHECK(false, "bad row-pass mode");
+  }
+  AV_LAUNCH_CHECK();
+}
+
+void launch_col_adjoint(const CartGeom &g, const float2 *zhat, const float2 *b1, float2 *out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_col_adjoint, dim3(g.nxb, g.T), dim3(NT), 0, s, g, zhat, b1, out);
+  AV_LAUNCH_CHECK();
+}
+
+void launch_compact_gather(const CartGeom &g, const float2 *dense, float2 *compact, hipStream_t s) {
+  const long n = g.nrows_c * g.W;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_compact_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, dense,
+                     compact);
+  AV_LAUNCH_CHECK();
+}
+
+void launch_fft2(float2 *data, const FftPlan &pw, const FftPlan &ph, int H, int W, int batch,
+                 int sign, int centered, float scale, hipStream_t s) {
+  const int RB = EMAX / W;
+  float2 *tmp = nullptr;
+  AV_HIP(hipMallocAsync((void **)&tmp, sizeof(float2) * (size_t)H * W * batch, s));
+  hipLaunchKernelGGL(k_fft_rows_oop, dim3((H + RB - 1) / RB, batch), dim3(NT), 0, s, data, tmp, pw, H,
+                     W, RB, sign, centered);
+  AV_LAUNCH_CHECK();
+  const int XB = pick_xb(H, W);
+  hipLaunchKernelGGL(k_fft_cols, dim3((W + XB - 1) / XB, batch), dim3(NT), 0, s, tmp, ph, H, W, XB,
+                     sign, centered, scale);
+  AV_LAUNCH_CHECK();
+  AV_HIP(hipMemcpyAsync(data, tmp, sizeof(float2) * (size_t)H * W * batch, hipMemcpyDeviceToDevice, s));
+  AV_HIP(hipFreeAsync(tmp, s));
+}
+
+}  // namespace avionic

@@ -1,0 +1,299 @@
+/*
+ * avionic.h — C-ABI of the MI355X-native AVIONIC primal-dual reconstruction
+ * library (libavionic.so, HIP kernels for gfx950).
+ *
+ * Drop-in boundary for the reference's L3 solver / L2 operator / L1 utility
+ * APIs.  Every entry point names the reference interface it replaces.
+ * Paths are relative to the AVIONIC source tree.
+ *
+ * Conventions
+ *  - All array pointers are DEVICE pointers (the reference's CVector/RVector
+ *    are GPU vectors, CUDA/include/types.h:17-23).  Complex data are
+ *    interleaved float pairs (std::complex<float> / hipFloatComplex layout).
+ *  - Layout (CUDA/src/cartesian_operator.cpp:77-81,136-140; main.cpp:509-519):
+ *      image   [frames][height][width]           width fastest
+ *      k-space [frames][coils][height][width]
+ *      mask    [frames][height][width]           float
+ *      coils   [coils][height][width]
+ *    A multi-component primal/dual vector is its components one after the
+ *    other, each N = width*height*frames long (the reference's
+ *    std::vector<CVector> of equal-length vectors, concatenated).
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Calls enqueue work on `stream`; functions that return a host scalar
+ *    synchronise that stream.
+ *  - Errors: every function returns AVIONIC_OK (0) or a negative status; no
+ *    C++ exception crosses this boundary (the reference throws
+ *    std::invalid_argument / runtime_error, pd_recon.cpp:94, utils.cpp:21).
+ *    avionic_last_error() returns a thread-local message for the last failure.
+ */
+#ifndef AVIONIC_H_
+#define AVIONIC_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVIONIC_OK 0
+#define AVIONIC_E_INVALID (-1)   /* bad argument (invalid_argument in the reference) */
+#define AVIONIC_E_HIP (-2)       /* HIP runtime error */
+#define AVIONIC_E_NOMEM (-3)     /* device allocation failed */
+#define AVIONIC_E_DIVERGED (-4)  /* NaN/Inf in a step-size or gap reduction */
+#define AVIONIC_E_CAPACITY (-5)  /* serving: request does not fit the HBM budget */
+#define AVIONIC_E_NOTFOUND (-6)  /* serving: unknown request id */
+#define AVIONIC_E_UNSUPPORTED (-7)
+
+typedef enum avionic_method {
+  AVIONIC_TV = 0,     /* TV     (CUDA/include/tv.h)     */
+  AVIONIC_TGV2 = 1,   /* TGV2   (CUDA/include/tgv2.h)   */
+  AVIONIC_ICTGV2 = 2  /* ICTGV2 (CUDA/include/ictgv2.h) */
+} avionic_method;
+
+/* Parameters: the union of PDParams (pd_recon.h:34-74), TGV2Params
+ * (tgv2.h:9-16) and ICTGV2Params (ictgv2.h:9-21), same field names, held in
+ * double.  dx/dy/dt (and dx2/dy2/dt2) are overwritten from timeSpaceWeight(2)
+ * when a reconstruction starts, as the reference does (tv.cpp:103-104,
+ * ictgv2.cpp:300-306).  The last two fields have no reference counterpart. */
+typedef struct avionic_params {
+  unsigned maxIt;
+  double dx, dy, dz;
+  double stopPDGap;
+  double ds, dt;
+  double sigma, tau, sigmaTauRatio;
+  double timeSpaceWeight;
+  double lambda;
+  int adaptLambda;             /* AdaptLambdaParams (pd_recon.h:18-31) */
+  double adaptLambdaK, adaptLambdaD;
+  double alpha0, alpha1;       /* TGV2 */
+  double alpha;                /* ICTGV2 */
+  double timeSpaceWeight2;
+  double dx2, dy2, dt2;
+  int adaptSchedule;           /* 0: adapt when k<10 || k%50==0 (tv.cpp:172);
+                                  1: every iteration (tv_dmri.m:198, debug=1) */
+  int dualStorage;             /* 0: float32 duals; 1: bfloat16 duals with
+                                  float32 arithmetic */
+} avionic_params;
+
+typedef struct avionic_operator avionic_operator;
+typedef struct avionic_recon avionic_recon;
+
+/* Last error message of this thread ("" if none). */
+const char *avionic_last_error(void);
+/* Library version string. */
+const char *avionic_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* L2 operators (CUDA/include/base_operator.h:24-112)                        */
+/* ------------------------------------------------------------------------ */
+
+/* CartesianOperator(width, height, coils, frames, mask, centered)
+ * (cartesian_operator.h:18-25, cartesian_operator.cpp:3-45).  mask may be
+ * NULL (the reference's ZeroMask: fully sampled).  The mask is copied; the
+ * caller may free it afterwards (the reference keeps a reference,
+ * cartesian_operator.h:114). */
+int avionic_cartesian_operator_create(unsigned width, unsigned height,
+                                      unsigned coils, unsigned frames,
+                                      const float *mask, int centered,
+                                      avionic_operator **op);
+
+/* NoncartesianOperator(width, height, coils, frames, nSpokes, nFE,
+ * spokesPerFrame, kTraj, dens, sens) (noncartesian_operator.h,
+ * noncartesian_operator.cpp:34-86).  traj: [frames][2][spokesPerFrame*nFE]
+ * (x then y per frame, in [-0.5, 0.5)); dens: [frames][spokesPerFrame*nFE]
+ * (may be NULL = 1); Kaiser-Bessel gridding with kernel width `kernel_width`
+ * on a grid oversampled by `osf`. */
+int avionic_noncartesian_operator_create(unsigned width, unsigned height,
+                                         unsigned coils, unsigned frames,
+                                         unsigned nFE, unsigned spokesPerFrame,
+                                         const float *traj, const float *dens,
+                                         int kernel_width, double osf,
+                                         avionic_operator **op);
+
+int avionic_operator_destroy(avionic_operator *op);
+
+/* ForwardOperation (k-space -> image, K^H), cartesian_operator.cpp:65-111.
+ * Unlike the reference the input is not modified. */
+int avionic_operator_forward(avionic_operator *op, const float *kdata,
+                             float *image, const float *b1, void *stream);
+
+/* BackwardOperation (image -> k-space, K), cartesian_operator.cpp:122-160. */
+int avionic_operator_backward(avionic_operator *op, const float *image,
+                              float *kdata, const float *b1, void *stream);
+
+/* AdaptLambda(k, d): cartesian_operator.cpp:47-62,
+ * noncartesian_operator.cpp:88-95. */
+int avionic_operator_adapt_lambda(avionic_operator *op, double k, double d,
+                                  double *lambda);
+
+/* Number of k-space samples per (frame, coil) the operator expects. */
+size_t avionic_operator_kspace_size(const avionic_operator *op);
+
+/* ------------------------------------------------------------------------ */
+/* L3 solvers (CUDA/include/pd_recon.h:88-185, tv.h, tgv2.h, ictgv2.h)       */
+/* ------------------------------------------------------------------------ */
+
+/* InitParams() defaults of each solver (tv.cpp:39-57, tgv2.cpp:36-56,
+ * ictgv2.cpp:37-64); lambda is left at 0 and adaptLambda = 1. */
+int avionic_default_params(avionic_method method, avionic_params *params);
+
+/* TV/TGV2/ICTGV2(width, height, coils, frames, [params,] mrOp).
+ * params == NULL selects the defaults and adapts lambda through the operator
+ * (InitParams + InitLambda(true)); otherwise lambda is adapted only when
+ * params->adaptLambda is set (InitLambda, tv.cpp:25-37).  The operator must
+ * outlive the solver. */
+int avionic_recon_create(avionic_method method, unsigned width, unsigned height,
+                         unsigned coils, unsigned frames,
+                         const avionic_params *params, avionic_operator *op,
+                         avionic_recon **recon);
+int avionic_recon_destroy(avionic_recon *recon);
+
+/* GetParams() (pd_recon.h:115) and writes through its reference. */
+int avionic_recon_get_params(const avionic_recon *recon, avionic_params *params);
+int avionic_recon_set_params(avionic_recon *recon, const avionic_params *params);
+
+/* SetVerbose / SetDebug(debug, debugstep) (pd_recon.h:148-151). */
+int avionic_recon_set_verbose(avionic_recon *recon, int verbose);
+int avionic_recon_set_debug(avionic_recon *recon, int debug, int debugstep);
+
+/* AdaptLambda(k, d) (pd_recon.h:98). */
+int avionic_recon_adapt_lambda(avionic_recon *recon, double k, double d,
+                               double *lambda);
+
+/* IterativeReconstruction(data, x, b1) (pd_recon.h:134-135).  data: k-space
+ * of the operator's shape, already masked (main.cpp:509-519); x: in = initial
+ * image, out = reconstruction.  Synchronises `stream` at the end. */
+int avionic_recon_run(avionic_recon *recon, const float *data, float *x,
+                      const float *b1, void *stream);
+
+/* AdaptStepSize(extDiff..., b1) (tv.cpp:79-97, tgv2.cpp:89-128,
+ * ictgv2.cpp:97-161) on a primal difference of P components (1 TV, 4 TGV2,
+ * 8 ICTGV2), updating sigma/tau in the solver parameters.  Uses dx..dt2 as
+ * currently set in the parameters. */
+int avionic_recon_adapt_step_size(avionic_recon *recon, const float *delta,
+                                  const float *b1, void *stream);
+
+/* ComputeGStar / ComputePDGap (tv.cpp:202-235, tgv2.cpp:130-176,
+ * ictgv2.cpp:178-249).  x: primal (P components for the gap, the image only
+ * for G*), y: dual (3 / 9 / 18 components), z: k-space dual.  Returned as
+ * double (the reference returns float). */
+int avionic_recon_compute_gstar(avionic_recon *recon, const float *x1,
+                                const float *y, const float *z,
+                                const float *data, const float *b1,
+                                double *gstar, void *stream);
+int avionic_recon_compute_pdgap(avionic_recon *recon, const float *x,
+                                const float *y, const float *z,
+                                const float *data, const float *b1,
+                                double *pdgap, void *stream);
+
+/* Step-size history of the last run: sigma/tau after each iteration.
+ * n: capacity in, entries written out. */
+int avionic_recon_get_step_log(const avionic_recon *recon, double *sigma,
+                               double *tau, size_t *n);
+
+/* ExportAdditionalResults(outputDir, callback) (pd_recon.h:142,
+ * ictgv2.cpp:494-516): calls cb(name, host data, complex count, user) for
+ * "x3_component" (ICTGV2), and with debug on "PDGap", "ICTGVnorm",
+ * "DATAfidelity".  Host buffers are valid during the call only. */
+typedef void (*avionic_export_cb)(const char *name, const float *data,
+                                  size_t count, void *user);
+int avionic_recon_export(avionic_recon *recon, avionic_export_cb cb,
+                         void *user);
+
+/* ------------------------------------------------------------------------ */
+/* L1 utilities (CUDA/include/utils.h; CUDA/src/utils.cpp)                   */
+/* ------------------------------------------------------------------------ */
+
+/* utils::Gradient (utils.cpp:99-118): u (N) -> g (3N). */
+int avionic_gradient(const float *u, float *g, unsigned width, unsigned height,
+                     unsigned frames, float dx, float dy, float dt,
+                     void *stream);
+/* utils::SymmetricGradient (utils.cpp:397-456): v (3N) -> e (6N). */
+int avionic_symmetric_gradient(const float *v, float *e, unsigned width,
+                               unsigned height, unsigned frames, float dx,
+                               float dy, float dt, void *stream);
+/* utils::Divergence (utils.cpp:733-757): p (3N) -> d (N). */
+int avionic_divergence(const float *p, float *d, unsigned width,
+                       unsigned height, unsigned frames, float dx, float dy,
+                       float dt, void *stream);
+/* utils::SymmetricDivergence (utils.cpp:961-1012): q (6N) -> d (3N). */
+int avionic_symmetric_divergence(const float *q, float *d, unsigned width,
+                                 unsigned height, unsigned frames, float dx,
+                                 float dy, float dt, void *stream);
+/* utils::ProximalMap3/6 (utils.cpp:1341-1351): y /= max(1, |y| * scale). */
+int avionic_proximal_map3(float *y, size_t n, float scale, void *stream);
+int avionic_proximal_map6(float *y, size_t n, float scale, void *stream);
+/* utils::TVNorm / TGV2Norm / ICTGV2Norm (utils.cpp:1119-1287). */
+int avionic_tv_norm(const float *u, unsigned width, unsigned height,
+                    unsigned frames, float dx, float dy, float dt,
+                    double *norm, void *stream);
+int avionic_tgv2_norm(const float *u, const float *v, unsigned width,
+                      unsigned height, unsigned frames, double alpha0,
+                      double alpha1, float dx, float dy, float dt,
+                      double *norm, void *stream);
+int avionic_ictgv2_norm(const float *u, const float *v, const float *w,
+                        const float *v2, unsigned width, unsigned height,
+                        unsigned frames, double alpha0, double alpha1,
+                        double alpha, float dx, float dy, float dt, float dx2,
+                        float dy2, float dt2, double *norm, void *stream);
+/* utils::Ellipke (utils.cpp:18-47); returns NaN outside [0, 1]. */
+double avionic_ellipke(double m);
+/* PDRecon::ComputeTimeSpaceWeights (pd_recon.cpp:72-98). */
+int avionic_time_space_weights(double timeSpaceWeight, double *ds, double *dt);
+
+/* In-place 2D FFT of `batch` height x width images (AGILE FFT, as used by
+ * cartesian_operator.cpp:90-99,142-151).  sign -1: e^{-i}, +1: e^{+i};
+ * unitary scales by 1/sqrt(width*height). */
+int avionic_fft2(float *data, unsigned height, unsigned width, unsigned batch,
+                 int sign, int centered, int unitary, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched execution and serving (no reference counterpart: the reference   */
+/* runs one reconstruction per process, main.cpp:789)                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct avionic_engine avionic_engine;
+
+typedef struct avionic_engine_config {
+  size_t hbm_budget_bytes;  /* admission limit for resident requests */
+  int use_graphs;           /* capture each iteration in a hipGraph */
+  int max_batch;            /* max requests in one batched launch */
+  int iterations_per_tick;  /* iterations run per engine step */
+} avionic_engine_config;
+
+typedef struct avionic_request_desc {
+  avionic_method method;
+  unsigned width, height, coils, frames;
+  const avionic_params *params;  /* NULL = defaults + adapted lambda */
+  const float *mask;             /* device, [frames][height][width] or NULL */
+  int centered;
+  const float *data;             /* device, [frames][coils][height][width] */
+  const float *b1;               /* device, [coils][height][width] */
+  const float *x0;               /* device, [frames][height][width] or NULL */
+} avionic_request_desc;
+
+int avionic_engine_create(const avionic_engine_config *cfg,
+                          avionic_engine **engine);
+int avionic_engine_destroy(avionic_engine *engine);
+/* HBM bytes a request of this shape keeps resident. */
+size_t avionic_request_footprint(const avionic_request_desc *desc);
+/* Admit a request (copies its inputs).  AVIONIC_E_CAPACITY if it does not fit
+ * the remaining budget now; the caller may retry after others finish. */
+int avionic_engine_submit(avionic_engine *engine,
+                          const avionic_request_desc *desc, long long *id);
+/* Run up to iterations_per_tick iterations of every resident request as one
+ * batched launch sequence. Returns the number of resident requests. */
+int avionic_engine_step(avionic_engine *engine, void *stream);
+/* 1 done (result copied to x_out, device [frames][height][width]), 0 running,
+ * AVIONIC_E_NOTFOUND unknown.  A finished request is released on retrieval. */
+int avionic_engine_poll(avionic_engine *engine, long long id, float *x_out,
+                        void *stream);
+int avionic_engine_cancel(avionic_engine *engine, long long id);
+/* Requests resident / bytes in use. */
+int avionic_engine_stats(const avionic_engine *engine, int *resident,
+                         size_t *bytes_in_use);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AVIONIC_H_ */
