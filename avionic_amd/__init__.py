@@ -483,3 +483,153 @@ def fft2(a, sign=-1, centered=False, unitary=False):
     _check(lib().avionic_fft2(_c64(a), H, W, batch, int(sign), int(bool(centered)),
                               int(bool(unitary)), _stream()))
     return a
+
+
+# ------------------------------------------------------- batching / serving
+class _EngineConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", _SZ), ("use_graphs", _I), ("max_batch", _I),
+                ("iterations_per_tick", _I)]
+
+
+class _ReqDesc(ctypes.Structure):
+    _fields_ = [("method", _I), ("width", _U), ("height", _U), ("coils", _U), ("frames", _U),
+                ("params", ctypes.POINTER(_Params)), ("mask", _P), ("centered", _I),
+                ("data", _P), ("b1", _P), ("x0", _P), ("stream", _P)]
+
+
+_ENGINE_SIGS = {
+    "avionic_engine_create": (_I, [ctypes.POINTER(_EngineConfig), ctypes.POINTER(_P)]),
+    "avionic_engine_destroy": (_I, [_P]),
+    "avionic_request_footprint": (_SZ, [ctypes.POINTER(_ReqDesc)]),
+    "avionic_engine_submit": (_I, [_P, ctypes.POINTER(_ReqDesc), ctypes.POINTER(ctypes.c_longlong)]),
+    "avionic_engine_step": (_I, [_P, _P]),
+    "avionic_engine_poll": (_I, [_P, ctypes.c_longlong, _P, _P]),
+    "avionic_engine_cancel": (_I, [_P, ctypes.c_longlong]),
+    "avionic_engine_stats": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_SZ)]),
+    "avionic_engine_get_step_log": (_I, [_P, ctypes.c_longlong, _P, _P, ctypes.POINTER(_SZ)]),
+}
+
+AVIONIC_E_CAPACITY = -5
+AVIONIC_E_NOTFOUND = -6
+
+
+def _engine_lib():
+    L = lib()
+    if not getattr(L, "_engine_declared", False):
+        for name, (res, args) in _ENGINE_SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._engine_declared = True
+    return L
+
+
+class Request:
+    """Inputs of one reconstruction request (device tensors)."""
+
+    def __init__(self, method, data, b1, mask=None, x0=None, centered=False, params=None):
+        self.method = int(method)
+        T, C, H, W = data.shape
+        self.shape = (W, H, C, T)
+        self.data, self.b1, self.mask, self.x0 = data, b1, mask, x0
+        self.centered = bool(centered)
+        self._params = None
+        if params is not None:
+            base = default_params(self.method)
+            base.update(params)
+            self._params = _from_dict(base)
+
+    def desc(self):
+        W, H, C, T = self.shape
+        d = _ReqDesc()
+        d.method = self.method
+        d.width, d.height, d.coils, d.frames = W, H, C, T
+        d.params = ctypes.pointer(self._params) if self._params is not None else None
+        d.mask = _f32(self.mask)
+        d.centered = int(self.centered)
+        d.data = _c64(self.data)
+        d.b1 = _c64(self.b1)
+        d.x0 = _c64(self.x0)
+        d.stream = _stream()
+        return d
+
+
+class Engine:
+    """Continuous-batching executor (avionic_engine_*): requests are admitted
+    against an HBM budget, stay resident, and advance together in task-list
+    launches (hipGraph-replayed when use_graphs)."""
+
+    def __init__(self, hbm_budget_bytes=0, use_graphs=True, max_batch=0, iterations_per_tick=1):
+        L = _engine_lib()
+        cfg = _EngineConfig(int(hbm_budget_bytes), int(bool(use_graphs)), int(max_batch),
+                            int(iterations_per_tick))
+        h = ctypes.c_void_p()
+        _check(L.avionic_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _engine_lib().avionic_engine_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @staticmethod
+    def footprint(req: Request) -> int:
+        d = req.desc()
+        return int(_engine_lib().avionic_request_footprint(ctypes.byref(d)))
+
+    def submit(self, req: Request):
+        """Returns the request id, or None when it does not fit right now."""
+        d = req.desc()
+        rid = ctypes.c_longlong()
+        rc = _engine_lib().avionic_engine_submit(self._h, ctypes.byref(d), ctypes.byref(rid))
+        if rc == AVIONIC_E_CAPACITY:
+            return None
+        _check(rc)
+        return rid.value
+
+    def step(self):
+        rc = _engine_lib().avionic_engine_step(self._h, _stream())
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def poll(self, rid, out):
+        rc = _engine_lib().avionic_engine_poll(self._h, int(rid), _c64(out), _stream())
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def cancel(self, rid):
+        _check(_engine_lib().avionic_engine_cancel(self._h, int(rid)))
+
+    def stats(self):
+        r, b = ctypes.c_int(), ctypes.c_size_t()
+        _check(_engine_lib().avionic_engine_stats(self._h, ctypes.byref(r), ctypes.byref(b)))
+        return r.value, b.value
+
+    def step_log(self, rid):
+        n = ctypes.c_size_t(1 << 30)
+        _check(_engine_lib().avionic_engine_get_step_log(self._h, int(rid), None, None,
+                                                         ctypes.byref(n)))
+        s, t = np.zeros(max(1, n.value)), np.zeros(max(1, n.value))
+        m = ctypes.c_size_t(n.value)
+        _check(_engine_lib().avionic_engine_get_step_log(self._h, int(rid), s.ctypes.data_as(_P),
+                                                         t.ctypes.data_as(_P), ctypes.byref(m)))
+        return s[: m.value], t[: m.value]
+
+
+def engine_time_kernels(engine: Engine, reps=20):
+    """{kernel name: average ms per launch} (see avionic_engine_time_kernels)."""
+    L = _engine_lib()
+    if not hasattr(L, "_tk"):
+        L.avionic_engine_time_kernels.restype = _I
+        L.avionic_engine_time_kernels.argtypes = [_P, _I, _P, _P, ctypes.POINTER(_SZ)]
+        L._tk = True
+    n = ctypes.c_size_t(16)
+    ms = (ctypes.c_double * 16)()
+    names = (ctypes.c_char_p * 16)()
+    _check(L.avionic_engine_time_kernels(engine._h, int(reps), ms, names, ctypes.byref(n)))
+    return {names[i].decode(): ms[i] for i in range(n.value)}

@@ -27,6 +27,8 @@ struct avionic_recon {
 
 static thread_local std::string g_err;
 
+void avionic_set_error(const char *msg) { g_err = msg ? msg : ""; }
+
 template <class F>
 static int guard(F &&f) {
   try {

@@ -106,6 +106,8 @@ struct DevScalars {
   double nKx2, nx2;  // last step-rule reductions
   int iter;          // iteration counter (incremented once per iteration)
   int diverged;      // set when a reduction produced NaN/Inf
+  int maxit;         // iterations to run (task-list execution idles past it)
+  int schedule;      // 0: adapt when iter < 10 || iter % 50 == 0; 1: always
 };
 
 }  // namespace avionic

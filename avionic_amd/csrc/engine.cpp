@@ -1,13 +1,505 @@
-// Placeholder until the batched engine lands.
+// Batched execution and serving of PD reconstructions (no reference
+// counterpart: the reference runs one reconstruction per process).
+//
+// Requests are admitted against an HBM budget, their iterates stay resident,
+// and every engine step advances all resident requests of one method with a
+// single task-list launch sequence per iteration (optionally replayed from a
+// captured hipGraph).  Each request keeps its own step sizes, iteration count
+// and adaptation schedule on the device, so a request's result does not
+// depend on what else is resident.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <vector>
+
 #include "solver.hpp"
+#include "tasks.hpp"
+
+namespace avionic {
+bool time_space_weights(double tsw, double &ds, double &dt);
+}
+void avionic_set_error(const char *msg);
+
+using namespace avionic;
+
+namespace {
+
+struct Slot {
+  long long id = 0;
+  int method = 0;
+  bool bf = false;
+  int W = 0, H = 0, C = 0, T = 0;
+  long N = 0, K = 0;
+  std::unique_ptr<CartesianOperator> op;
+  DevBuf x, ext, y, z, d, khz, kcol, zhat, sc, slog, tlog, preg, pk, b1;
+  avionic_params params{};
+  int maxit = 0, k_host = 0;
+  bool done = false;
+  size_t bytes = 0;
+  Task task{};
+
+  bool adapts(int k) const {
+    return params.adaptSchedule == 1 || k < 10 || (k % 50) == 0;
+  }
+};
+
+size_t slot_bytes(int method, bool bf, long N, long K, long CHW, int maxit) {
+  const int P = primal_components(method), D = dual_components(method);
+  size_t b = 0;
+  b += 2 * sizeof(float2) * (size_t)P * N;                        // x, ext
+  b += (bf ? sizeof(bf16x2) : sizeof(float2)) * (size_t)D * N;    // y
+  b += 4 * sizeof(float2) * (size_t)std::max(1L, K);              // z, d, kcol, zhat
+  b += sizeof(float2) * (size_t)N;                                // khz
+  b += sizeof(float2) * (size_t)CHW;                              // b1 copy
+  b += 2 * sizeof(double) * (size_t)std::max(1, maxit);           // logs
+  b += sizeof(double) * 3 * (size_t)((N + kRedThreads - 1) / kRedThreads + 1);  // partials
+  return b;
+}
+
+struct Group {
+  int method = 0;
+  bool bf = false;
+  std::vector<Slot *> slots;  // active (not done)
+  bool dirty = true;
+  DevBuf table;
+  TaskGrid grid;
+  hipGraphExec_t g_step = nullptr, g_plain = nullptr;
+  void drop_graphs() {
+    if (g_step) (void)hipGraphExecDestroy(g_step);
+    if (g_plain) (void)hipGraphExecDestroy(g_plain);
+    g_step = g_plain = nullptr;
+  }
+};
+
+}  // namespace
+
+struct avionic_engine {
+  avionic_engine_config cfg{};
+  hipStream_t stream = nullptr;       // compute
+  hipStream_t copy_stream = nullptr;  // admissions
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_req = nullptr;
+  long long next_id = 1;
+  size_t used = 0;
+  std::map<long long, std::unique_ptr<Slot>> slots;
+  std::map<int, Group> groups;  // key = method * 2 + bf
+
+  ~avionic_engine() {
+    for (auto &kv : groups) kv.second.drop_graphs();
+    slots.clear();
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (ev_req) (void)hipEventDestroy(ev_req);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+  }
+
+  void rebuild(Group &g) {
+    g.drop_graphs();
+    std::vector<Task> tasks;
+    int bv = 0, bc = 0, br = 0;
+    for (Slot *s : g.slots) {
+      Task t = s->task;
+      t.b_vox = bv;
+      t.b_col = bc;
+      t.b_row = br;
+      bv += t.nvox_blocks;
+      bc += t.ncol_blocks;
+      br += t.nrow_blocks;
+      tasks.push_back(t);
+    }
+    g.grid.vox = bv;
+    g.grid.col = bc;
+    g.grid.row = br;
+    if (!tasks.empty()) {
+      g.table.ensure(sizeof(Task) * tasks.size());
+      AV_HIP(hipMemcpyAsync(g.table.p, tasks.data(), sizeof(Task) * tasks.size(),
+                            hipMemcpyHostToDevice, stream));
+    }
+    g.dirty = false;
+  }
+
+  void run_iteration(Group &g, bool with_step) {
+    const int n = (int)g.slots.size();
+    if (!cfg.use_graphs) {
+      launch_iteration(g.method, g.bf, g.table.as<Task>(), n, g.grid, with_step, stream);
+      return;
+    }
+    hipGraphExec_t &ex = with_step ? g.g_step : g.g_plain;
+    if (!ex) {
+      hipGraph_t graph = nullptr;
+      AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      launch_iteration(g.method, g.bf, g.table.as<Task>(), n, g.grid, with_step, stream);
+      AV_HIP(hipStreamEndCapture(stream, &graph));
+      AV_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+    }
+    AV_HIP(hipGraphLaunch(ex, stream));
+  }
+};
+
+namespace {
+
+template <class F>
+int eguard(F &&f) {
+  try {
+    f();
+    avionic_set_error("");
+    return AVIONIC_OK;
+  } catch (const Error &e) {
+    avionic_set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    avionic_set_error("out of memory");
+    return AVIONIC_E_NOMEM;
+  } catch (const std::exception &e) {
+    avionic_set_error(e.what());
+    return AVIONIC_E_INVALID;
+  }
+}
+
+// Validate and complete a request's parameters (defaults, lambda, weights).
+avionic_params request_params(const avionic_request_desc *d, const Operator &op) {
+  avionic_params p{};
+  if (d->params) {
+    p = *d->params;
+    if (p.adaptLambda) p.lambda = op.adapt_lambda(p.adaptLambdaK, p.adaptLambdaD);
+  } else {
+    Recon::defaults((int)d->method, &p);
+    p.lambda = op.adapt_lambda(p.adaptLambdaK, p.adaptLambdaD);
+  }
+  double ds, dt;
+  AV_CHECK(time_space_weights(p.timeSpaceWeight, ds, dt), "ComputeTimeSpaceWeights: Invalid ratio.");
+  p.dx = p.dy = ds;
+  p.dt = dt;
+  if (d->method == AVIONIC_ICTGV2) {
+    AV_CHECK(time_space_weights(p.timeSpaceWeight2, ds, dt), "ComputeTimeSpaceWeights: Invalid ratio.");
+    p.dx2 = p.dy2 = ds;
+    p.dt2 = dt;
+  }
+  AV_CHECK(p.lambda > 0.0, "lambda must be positive");
+  AV_CHECK(p.dualStorage == 0 || p.dualStorage == 1, "dualStorage must be 0 or 1");
+  return p;
+}
+
+void check_desc(const avionic_request_desc *d) {
+  AV_CHECK(d != nullptr, "request descriptor is null");
+  AV_CHECK(d->method >= 0 && d->method <= 2, "unknown method");
+  AV_CHECK(d->width > 0 && d->height > 0 && d->coils > 0 && d->frames > 0, "empty request");
+  AV_CHECK(d->data != nullptr && d->b1 != nullptr, "request data / b1 missing");
+}
+
+}  // namespace
 
 extern "C" {
-int avionic_engine_create(const avionic_engine_config *, avionic_engine **) { return AVIONIC_E_UNSUPPORTED; }
-int avionic_engine_destroy(avionic_engine *) { return AVIONIC_E_UNSUPPORTED; }
-size_t avionic_request_footprint(const avionic_request_desc *) { return 0; }
-int avionic_engine_submit(avionic_engine *, const avionic_request_desc *, long long *) { return AVIONIC_E_UNSUPPORTED; }
-int avionic_engine_step(avionic_engine *, void *) { return AVIONIC_E_UNSUPPORTED; }
-int avionic_engine_poll(avionic_engine *, long long, float *, void *) { return AVIONIC_E_UNSUPPORTED; }
-int avionic_engine_cancel(avionic_engine *, long long) { return AVIONIC_E_UNSUPPORTED; }
-int avionic_engine_stats(const avionic_engine *, int *, size_t *) { return AVIONIC_E_UNSUPPORTED; }
+
+int avionic_engine_create(const avionic_engine_config *cfg, avionic_engine **engine) {
+  return eguard([&] {
+    AV_CHECK(cfg && engine, "null argument");
+    *engine = nullptr;
+    auto e = std::make_unique<avionic_engine>();
+    e->cfg = *cfg;
+    if (e->cfg.iterations_per_tick <= 0) e->cfg.iterations_per_tick = 1;
+    if (e->cfg.max_batch <= 0) e->cfg.max_batch = 1 << 20;
+    AV_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    AV_HIP(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+    AV_HIP(hipEventCreateWithFlags(&e->ev_req, hipEventDisableTiming));
+    AV_HIP(hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming));
+    AV_HIP(hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming));
+    *engine = e.release();
+  });
 }
+
+int avionic_engine_destroy(avionic_engine *engine) {
+  return eguard([&] {
+    if (!engine) return;
+    (void)hipStreamSynchronize(engine->stream);
+    delete engine;
+  });
+}
+
+size_t avionic_request_footprint(const avionic_request_desc *d) {
+  try {
+    check_desc(d);
+    CartPlan plan;
+    plan.init((int)d->width, (int)d->height, (int)d->coils, (int)d->frames, d->mask, d->centered,
+              nullptr);
+    const long N = (long)d->width * d->height * d->frames;
+    const int maxit = d->params ? (int)d->params->maxIt : 500;
+    const bool bf = d->params && d->params->dualStorage == 1;
+    return slot_bytes((int)d->method, bf, N, plan.kspace_elems(),
+                      (long)d->coils * d->height * d->width, maxit);
+  } catch (...) {
+    return 0;
+  }
+}
+
+int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long long *id) {
+  return eguard([&] {
+    AV_CHECK(e && id, "null argument");
+    check_desc(d);
+    auto s = std::make_unique<Slot>();
+    s->method = (int)d->method;
+    s->W = (int)d->width; s->H = (int)d->height; s->C = (int)d->coils; s->T = (int)d->frames;
+    s->N = (long)s->W * s->H * s->T;
+    s->op = std::make_unique<CartesianOperator>(s->W, s->H, s->C, s->T, d->mask, d->centered);
+    s->K = s->op->compact_elems();
+    s->params = request_params(d, *s->op);
+    s->bf = s->params.dualStorage == 1;
+    s->maxit = (int)s->params.maxIt;
+    s->bytes = slot_bytes(s->method, s->bf, s->N, s->K, (long)s->C * s->H * s->W, s->maxit);
+    if (e->cfg.hbm_budget_bytes && e->used + s->bytes > e->cfg.hbm_budget_bytes)
+      throw Error(AVIONIC_E_CAPACITY, "request does not fit the HBM budget now");
+    int resident = 0;
+    for (auto &kv : e->slots) resident += kv.second->done ? 0 : 1;
+    if (resident >= e->cfg.max_batch) throw Error(AVIONIC_E_CAPACITY, "batch is full");
+
+    const int P = primal_components(s->method), D = dual_components(s->method);
+    // inputs produced on the caller's stream; admissions copy on their own
+    // stream so that they do not wait for the batch in flight
+    hipStream_t st = e->copy_stream;
+    AV_HIP(hipEventRecord(e->ev_req, (hipStream_t)d->stream));
+    AV_HIP(hipStreamWaitEvent(st, e->ev_req, 0));
+    const long N = s->N, K = std::max(1L, s->K);
+    s->x.ensure(sizeof(float2) * P * N);
+    s->ext.ensure(sizeof(float2) * P * N);
+    s->y.ensure((s->bf ? sizeof(bf16x2) : sizeof(float2)) * D * N);
+    s->z.ensure(sizeof(float2) * K);
+    s->d.ensure(sizeof(float2) * K);
+    s->kcol.ensure(sizeof(float2) * K);
+    s->zhat.ensure(sizeof(float2) * K);
+    s->khz.ensure(sizeof(float2) * N);
+    s->b1.ensure(sizeof(float2) * (size_t)s->C * s->H * s->W);
+    s->slog.ensure(sizeof(double) * std::max(1, s->maxit));
+    s->tlog.ensure(sizeof(double) * std::max(1, s->maxit));
+    const Grid3 g{s->W, s->H, s->T, N};
+    const int nvb = step_blocks(g);
+    const int nrb = s->op->plan().row_blocks();
+    s->preg.ensure(sizeof(double) * 2 * nvb);
+    s->pk.ensure(sizeof(double) * std::max(1, nrb));
+    AV_HIP(hipMemsetAsync(s->x.p, 0, s->x.bytes, st));
+    AV_HIP(hipMemsetAsync(s->y.p, 0, s->y.bytes, st));
+    AV_HIP(hipMemsetAsync(s->z.p, 0, s->z.bytes, st));
+    if (d->x0)
+      AV_HIP(hipMemcpyAsync(s->x.p, d->x0, sizeof(float2) * N, hipMemcpyDeviceToDevice, st));
+    AV_HIP(hipMemcpyAsync(s->ext.p, s->x.p, s->x.bytes, hipMemcpyDeviceToDevice, st));
+    AV_HIP(hipMemcpyAsync(s->b1.p, d->b1, s->b1.bytes, hipMemcpyDeviceToDevice, st));
+    s->op->gather(reinterpret_cast<const float2 *>(d->data), s->d.as<float2>(), st);
+    DevScalars hs{};
+    hs.sigma = s->params.sigma;
+    hs.tau = s->params.tau;
+    hs.lambda = s->params.lambda;
+    hs.ratio = s->params.sigmaTauRatio;
+    hs.iter = 0;
+    hs.maxit = s->maxit;
+    hs.schedule = s->params.adaptSchedule;
+    s->sc.ensure(sizeof(DevScalars));
+    AV_HIP(hipMemcpyAsync(s->sc.p, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+    AV_HIP(hipStreamSynchronize(st));  // the caller may release its inputs now
+
+    const double d6[6] = {s->params.dx, s->params.dy, s->params.dt,
+                          s->params.dx2, s->params.dy2, s->params.dt2};
+    Task &t = s->task;
+    t.g = g;
+    t.rp = make_reg_params(s->method, d6, s->params.alpha0, s->params.alpha1, s->params.alpha);
+    t.cg = s->op->plan().g;
+    t.x = s->x.as<float2>();
+    t.ext = s->ext.as<float2>();
+    t.y = s->y.p;
+    t.khz = s->khz.as<float2>();
+    t.z = s->z.as<float2>();
+    t.d = s->d.as<float2>();
+    t.kcol = s->kcol.as<float2>();
+    t.zhat = s->zhat.as<float2>();
+    t.b1 = s->b1.as<float2>();
+    t.sc = s->sc.as<DevScalars>();
+    t.sig_log = s->slog.as<double>();
+    t.tau_log = s->tlog.as<double>();
+    t.part_reg = s->preg.as<double>();
+    t.part_k = s->pk.as<double>();
+    t.nvox_blocks = nvb;
+    t.ncol_blocks = t.cg.nxb * t.cg.T;
+    t.nrow_blocks = nrb;
+    t.schedule = s->params.adaptSchedule;
+
+    s->id = e->next_id++;
+    *id = s->id;
+    e->used += s->bytes;
+    Group &grp = e->groups[s->method * 2 + (s->bf ? 1 : 0)];
+    grp.method = s->method;
+    grp.bf = s->bf;
+    grp.slots.push_back(s.get());
+    grp.dirty = true;
+    if (s->maxit == 0) s->done = true;
+    e->slots[s->id] = std::move(s);
+  });
+}
+
+int avionic_engine_step(avionic_engine *e, void *stream) {
+  int resident = 0;
+  const int rc = eguard([&] {
+    AV_CHECK(e, "engine is null");
+    hipStream_t us = (hipStream_t)stream;
+    AV_HIP(hipEventRecord(e->ev_in, us));
+    AV_HIP(hipStreamWaitEvent(e->stream, e->ev_in, 0));
+    for (auto &kv : e->groups) {
+      Group &g = kv.second;
+      // drop finished requests from the launch set
+      auto it = std::remove_if(g.slots.begin(), g.slots.end(), [](Slot *s) { return s->done; });
+      if (it != g.slots.end()) {
+        g.slots.erase(it, g.slots.end());
+        g.dirty = true;
+      }
+      if (g.slots.empty()) continue;
+      if (g.dirty) e->rebuild(g);
+      for (int i = 0; i < e->cfg.iterations_per_tick; ++i) {
+        bool any = false, step = false;
+        for (Slot *s : g.slots)
+          if (s->k_host < s->maxit) {
+            any = true;
+            step = step || s->adapts(s->k_host);
+          }
+        if (!any) break;
+        e->run_iteration(g, step);
+        for (Slot *s : g.slots)
+          if (s->k_host < s->maxit) s->k_host++;
+      }
+      for (Slot *s : g.slots)
+        if (s->k_host >= s->maxit) s->done = true;
+    }
+    AV_HIP(hipEventRecord(e->ev_out, e->stream));
+    AV_HIP(hipStreamWaitEvent(us, e->ev_out, 0));
+    for (auto &kv : e->slots) resident += 1;
+  });
+  return rc == AVIONIC_OK ? resident : rc;
+}
+
+int avionic_engine_poll(avionic_engine *e, long long id, float *x_out, void *stream) {
+  int result = 0;
+  const int rc = eguard([&] {
+    AV_CHECK(e, "engine is null");
+    auto it = e->slots.find(id);
+    if (it == e->slots.end()) throw Error(AVIONIC_E_NOTFOUND, "unknown request id");
+    Slot *s = it->second.get();
+    if (!s->done) {
+      result = 0;
+      return;
+    }
+    AV_CHECK(x_out != nullptr, "x_out is null");
+    hipStream_t us = (hipStream_t)stream;
+    AV_HIP(hipStreamSynchronize(e->stream));
+    DevScalars hs{};
+    AV_HIP(hipMemcpy(&hs, s->sc.p, sizeof(hs), hipMemcpyDeviceToHost));
+    AV_HIP(hipMemcpyAsync(x_out, s->x.p, sizeof(float2) * s->N, hipMemcpyDeviceToDevice, us));
+    AV_HIP(hipStreamSynchronize(us));
+    e->used -= s->bytes;
+    for (auto &kv : e->groups) {
+      auto &v = kv.second.slots;
+      auto f = std::find(v.begin(), v.end(), s);
+      if (f != v.end()) {
+        v.erase(f);
+        kv.second.dirty = true;
+      }
+    }
+    e->slots.erase(it);
+    if (hs.diverged) throw Error(AVIONIC_E_DIVERGED, "NaN/Inf in a step-size reduction");
+    result = 1;
+  });
+  return rc == AVIONIC_OK ? result : rc;
+}
+
+int avionic_engine_cancel(avionic_engine *e, long long id) {
+  return eguard([&] {
+    AV_CHECK(e, "engine is null");
+    auto it = e->slots.find(id);
+    if (it == e->slots.end()) throw Error(AVIONIC_E_NOTFOUND, "unknown request id");
+    Slot *s = it->second.get();
+    AV_HIP(hipStreamSynchronize(e->stream));
+    for (auto &kv : e->groups) {
+      auto &v = kv.second.slots;
+      auto f = std::find(v.begin(), v.end(), s);
+      if (f != v.end()) {
+        v.erase(f);
+        kv.second.dirty = true;
+      }
+    }
+    e->used -= s->bytes;
+    e->slots.erase(it);
+  });
+}
+
+int avionic_engine_stats(const avionic_engine *e, int *resident, size_t *bytes) {
+  return eguard([&] {
+    AV_CHECK(e, "engine is null");
+    if (resident) *resident = (int)e->slots.size();
+    if (bytes) *bytes = e->used;
+  });
+}
+
+// Average duration (ms) of each kernel family of the first resident group,
+// each launched `reps` times alone between HIP events on the engine stream.
+// The launches advance the resident requests' state: time on a throw-away
+// engine.  Kernels whose tasks skip work at the current iteration (step-rule
+// kernels outside the schedule) are timed at that cost.
+int avionic_engine_time_kernels(avionic_engine *e, int reps, double *ms, const char **names,
+                                size_t *n) {
+  return eguard([&] {
+    AV_CHECK(e && ms && n && reps > 0, "null argument");
+    Group *g = nullptr;
+    for (auto &kv : e->groups)
+      if (!kv.second.slots.empty()) {
+        g = &kv.second;
+        break;
+      }
+    AV_CHECK(g != nullptr, "no resident requests");
+    if (g->dirty) e->rebuild(*g);
+    const size_t m = std::min(*n, (size_t)kNumKernels);
+    hipEvent_t a, b;
+    AV_HIP(hipEventCreate(&a));
+    AV_HIP(hipEventCreate(&b));
+    for (size_t k = 0; k < m; ++k) {
+      launch_kernel((int)k, g->method, g->bf, g->table.as<Task>(), (int)g->slots.size(), g->grid,
+                    e->stream);  // warm
+      AV_HIP(hipEventRecord(a, e->stream));
+      for (int r = 0; r < reps; ++r)
+        launch_kernel((int)k, g->method, g->bf, g->table.as<Task>(), (int)g->slots.size(), g->grid,
+                      e->stream);
+      AV_HIP(hipEventRecord(b, e->stream));
+      AV_HIP(hipEventSynchronize(b));
+      float t = 0.f;
+      AV_HIP(hipEventElapsedTime(&t, a, b));
+      ms[k] = (double)t / reps;
+      if (names) names[k] = kernel_name((int)k);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *n = m;
+  });
+}
+
+int avionic_engine_get_step_log(const avionic_engine *e, long long id, double *sigma, double *tau,
+                                size_t *n) {
+  return eguard([&] {
+    AV_CHECK(e && n, "null argument");
+    auto it = e->slots.find(id);
+    if (it == e->slots.end()) throw Error(AVIONIC_E_NOTFOUND, "unknown request id");
+    const Slot *s = it->second.get();
+    AV_HIP(hipStreamSynchronize(e->stream));
+    const size_t m = std::min(*n, (size_t)s->k_host);
+    std::vector<double> hs(std::max<size_t>(1, s->maxit)), ht(std::max<size_t>(1, s->maxit));
+    AV_HIP(hipMemcpy(hs.data(), s->slog.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost));
+    AV_HIP(hipMemcpy(ht.data(), s->tlog.p, sizeof(double) * ht.size(), hipMemcpyDeviceToHost));
+    double cs = s->params.sigma, ct = s->params.tau;
+    for (size_t k = 0; k < m; ++k) {
+      if (s->adapts((int)k)) {
+        cs = hs[k];
+        ct = ht[k];
+      }
+      if (sigma) sigma[k] = cs;
+      if (tau) tau[k] = ct;
+    }
+    *n = m;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,221 @@
+// Task-list kernels of the PD iteration (see tasks.hpp).  Each block finds
+// its task, checks the task's device-side predicates (uniform per block, so a
+// whole block returns together) and runs the shared block / voxel bodies.
+#include "cart_device.hpp"
+#include "reg_device.hpp"
+#include "tasks.hpp"
+
+namespace avionic {
+
+enum Family { kVox = 0, kCol = 1, kRow = 2 };
+
+template <int F>
+__device__ inline int first_block(const Task &t) {
+  return F == kVox ? t.b_vox : (F == kCol ? t.b_col : t.b_row);
+}
+
+// Last task whose first block does not exceed b (tasks are in block order;
+// a task with no blocks in this family shares its start with the next one).
+template <int F>
+__device__ inline int find_task(const Task *__restrict__ tasks, int n, int b) {
+  int lo = 0;
+  for (int i = 1; i < n; ++i) {
+    if (first_block<F>(tasks[i]) <= b)
+      lo = i;
+    else
+      break;
+  }
+  return lo;
+}
+
+__device__ inline bool task_active(const DevScalars *sc) { return sc->iter < sc->maxit; }
+__device__ inline bool task_adapt(const DevScalars *sc) {
+  const int k = sc->iter;
+  return sc->schedule == 1 || k < 10 || (k % 50) == 0;
+}
+
+template <int M, typename DT>
+__global__ __launch_bounds__(RT) void kt_dual(const Task *__restrict__ tasks, int n) {
+  const Task &T = tasks[find_task<kVox>(tasks, n, blockIdx.x)];
+  if (!task_active(T.sc)) return;
+  const long i = (long)(blockIdx.x - T.b_vox) * RT + threadIdx.x;
+  if (i >= T.g.N) return;
+  dual_voxel<M, DT>(T.g, T.rp, T.ext, (DT *)T.y, (float)T.sc->sigma, vox_of(T.g, i));
+}
+
+template <int M, typename DT>
+__global__ __launch_bounds__(RT) void kt_primal(const Task *__restrict__ tasks, int n) {
+  const Task &T = tasks[find_task<kVox>(tasks, n, blockIdx.x)];
+  if (!task_active(T.sc)) return;
+  const long i = (long)(blockIdx.x - T.b_vox) * RT + threadIdx.x;
+  if (i >= T.g.N) return;
+  primal_voxel<M, DT>(T.g, T.rp, T.x, T.ext, (const DT *)T.y, T.khz, (float)T.sc->tau,
+                      vox_of(T.g, i));
+}
+
+template <int M>
+__global__ __launch_bounds__(RT) void kt_step_reg(const Task *__restrict__ tasks, int n) {
+  const Task &T = tasks[find_task<kVox>(tasks, n, blockIdx.x)];
+  if (!task_active(T.sc) || !task_adapt(T.sc)) return;
+  const int lb = blockIdx.x - T.b_vox;
+  const long i = (long)lb * RT + threadIdx.x;
+  double ak = 0.0, ax = 0.0;
+  if (i < T.g.N) step_voxel<M>(T.g, T.rp, T.ext, T.x, vox_of(T.g, i), ak, ax);
+  block_reduce2(ak, ax, T.part_reg + 2 * lb);
+}
+
+// DELTA = false: K of the extragradient image (dual update);
+// DELTA = true : K of ext - x (step rule, adapting tasks only).
+template <bool DELTA>
+__global__ __launch_bounds__(NT) void kt_col_forward(const Task *__restrict__ tasks, int n) {
+  __shared__ float2 tile[EMAX];
+  const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
+  if (!task_active(T.sc)) return;
+  if (DELTA && !task_adapt(T.sc)) return;
+  const int lb = blockIdx.x - T.b_col;
+  col_forward_block(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.ext, DELTA ? T.x : nullptr, T.b1,
+                    T.kcol, tile);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int n) {
+  __shared__ float2 tile[EMAX];
+  __shared__ float2 tile2[(MODE == kRowDualUpdate) ? EMAX : 1];
+  __shared__ double red[NT];
+  const Task &T = tasks[find_task<kRow>(tasks, n, blockIdx.x)];
+  if (!task_active(T.sc)) return;
+  if (MODE == kRowNorm2 && !task_adapt(T.sc)) return;
+  const int lb = blockIdx.x - T.b_row;
+  row_pass_block<MODE>(T.cg, lb, T.kcol, T.z, T.d, T.zhat, nullptr, nullptr, T.sc, T.part_k, tile,
+                       tile2, red);
+}
+
+__global__ __launch_bounds__(NT) void kt_col_adjoint(const Task *__restrict__ tasks, int n) {
+  __shared__ float2 tile[EMAX];
+  __shared__ float2 acc[EMAX];
+  const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
+  if (!task_active(T.sc)) return;
+  const int lb = blockIdx.x - T.b_col;
+  col_adjoint_block(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.zhat, T.b1, T.khz, tile, acc);
+}
+
+// One block per task: step rule for adapting tasks, then the iteration
+// counter.  The reductions run in a fixed order (deterministic).
+__global__ __launch_bounds__(RT) void kt_tail(const Task *__restrict__ tasks, int n) {
+  const Task &T = tasks[blockIdx.x];
+  DevScalars *sc = T.sc;
+  if (!task_active(sc)) return;
+  if (task_adapt(sc)) {
+    __shared__ double ra[RT], rb[RT];
+    const int tid = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int j = tid; j < T.nvox_blocks; j += RT) {
+      a += T.part_reg[2 * j];
+      b += T.part_reg[2 * j + 1];
+    }
+    for (int j = tid; j < T.nrow_blocks; j += RT) a += T.part_k[j];
+    ra[tid] = a;
+    rb[tid] = b;
+    __syncthreads();
+    for (int s = RT / 2; s > 0; s >>= 1) {
+      if (tid < s) {
+        ra[tid] += ra[tid + s];
+        rb[tid] += rb[tid + s];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      apply_step_rule(sc, ra[0], rb[0]);
+      if (T.sig_log) T.sig_log[sc->iter] = sc->sigma;
+      if (T.tau_log) T.tau_log[sc->iter] = sc->tau;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc->iter += 1;
+}
+
+const char *kernel_name(int id) {
+  static const char *names[kNumKernels] = {"dual", "col_forward", "row_dual_update", "col_adjoint",
+                                           "primal", "step_reg", "col_forward_delta",
+                                           "row_norm2", "tail"};
+  return (id >= 0 && id < kNumKernels) ? names[id] : "?";
+}
+
+void launch_kernel(int id, int method, bool bf, const Task *d_tasks, int n, const TaskGrid &gr,
+                   hipStream_t s) {
+  if (n <= 0) return;
+  const dim3 bRT(RT), bNT(NT);
+#define AV_BYMETHOD2(KER, ...)                                                       \
+  do {                                                                               \
+    if (!bf) {                                                                       \
+      if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
+      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);                   \
+    } else {                                                                         \
+      if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
+      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);                   \
+    }                                                                                \
+  } while (0)
+  switch (id) {
+    case kKDual: if (gr.vox) AV_BYMETHOD2(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
+    case kKColFwd: if (gr.col) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNT, 0, s, d_tasks, n); break;
+    case kKRowDual: if (gr.row) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
+    case kKColAdj: if (gr.col) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNT, 0, s, d_tasks, n); break;
+    case kKPrimal: if (gr.vox) AV_BYMETHOD2(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
+    case kKStepReg:
+      if (gr.vox) {
+        if (method == kTV) hipLaunchKernelGGL(kt_step_reg<kTV>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+        else if (method == kTGV2) hipLaunchKernelGGL(kt_step_reg<kTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+        else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+      }
+      break;
+    case kKColFwdDelta: if (gr.col) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNT, 0, s, d_tasks, n); break;
+    case kKRowNorm2: if (gr.row) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
+    case kKTail: hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n); break;
+    default: break;
+  }
+#undef AV_BYMETHOD2
+  AV_LAUNCH_CHECK();
+}
+
+void launch_iteration(int method, bool bf, const Task *d_tasks, int n, const TaskGrid &gr,
+                      bool with_step, hipStream_t s) {
+  if (n <= 0) return;
+  const dim3 bRT(RT), bNT(NT);
+#define AV_BYMETHOD(KER, ...)                                                        \
+  do {                                                                               \
+    if (!bf) {                                                                       \
+      if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
+      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);                   \
+    } else {                                                                         \
+      if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
+      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);                   \
+    }                                                                                \
+  } while (0)
+  if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+  AV_LAUNCH_CHECK();
+  if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNT, 0, s, d_tasks, n);
+  if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n);
+  if (gr.col > 0) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNT, 0, s, d_tasks, n);
+  AV_LAUNCH_CHECK();
+  if (gr.vox > 0) AV_BYMETHOD(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+  AV_LAUNCH_CHECK();
+  if (with_step) {
+    if (gr.vox > 0) {
+      if (method == kTV) hipLaunchKernelGGL(kt_step_reg<kTV>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+      else if (method == kTGV2) hipLaunchKernelGGL(kt_step_reg<kTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+      else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+    }
+    if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNT, 0, s, d_tasks, n);
+    if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n);
+    AV_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n);
+  AV_LAUNCH_CHECK();
+#undef AV_BYMETHOD
+}
+
+}  // namespace avionic

@@ -1,0 +1,62 @@
+// Task-list execution of the PD iteration.
+//
+// One Task is one reconstruction's device state.  Every hot kernel takes an
+// array of tasks and a flattened grid: block b belongs to the task whose first
+// block (per kernel family) is the largest not exceeding b.  A launch therefore
+// advances any number of reconstructions of one method at once, each with its
+// own shape, step sizes, iteration counter and schedule; a single
+// reconstruction is the one-task case of the same kernels.
+//
+// Per-task predicates evaluated on the device:
+//   active: sc->iter < sc->maxit        (every kernel; a finished task idles)
+//   adapt : schedule == 1 || iter < 10 || iter % 50 == 0
+//           (step-size kernels; tv.cpp:172, tgv2.cpp:306, ictgv2.cpp:448)
+// so the same captured graph can be replayed for any mix of iteration counts.
+#pragma once
+
+#include "cartesian.hpp"
+#include "common.hpp"
+#include "reg.hpp"
+
+namespace avionic {
+
+struct Task {
+  Grid3 g;
+  RegParams rp;
+  CartGeom cg;
+  float2 *x, *ext;     // primal, P components each
+  void *y;             // dual, D components (float2 or bf16x2)
+  float2 *khz;         // K^H z (image)
+  float2 *z;           // compact k-space dual
+  const float2 *d;     // compact data
+  float2 *kcol, *zhat; // operator scratch (compact)
+  const float2 *b1;
+  DevScalars *sc;
+  double *sig_log, *tau_log;  // indexed by iteration
+  double *part_reg;    // 2 per voxel block
+  double *part_k;      // 1 per row block
+  int nvox_blocks, ncol_blocks, nrow_blocks;
+  int b_vox, b_col, b_row;    // first block of this task per kernel family
+  int schedule;               // 0 reference schedule, 1 every iteration
+};
+
+// Launch one iteration's kernels for `n` tasks of one method / dual storage.
+// tasks: device array; the host mirror gives the grid sizes.
+struct TaskGrid {
+  int vox = 0, col = 0, row = 0;
+};
+
+void launch_iteration(int method, bool bf16, const Task *d_tasks, int n, const TaskGrid &grid,
+                      bool with_step, hipStream_t s);
+
+// Kernel families of one iteration, in launch order.
+enum KernelId {
+  kKDual = 0, kKColFwd, kKRowDual, kKColAdj, kKPrimal, kKStepReg, kKColFwdDelta, kKRowNorm2,
+  kKTail, kNumKernels
+};
+const char *kernel_name(int id);
+// Launch one kernel family alone (for timing).
+void launch_kernel(int id, int method, bool bf16, const Task *d_tasks, int n, const TaskGrid &grid,
+                   hipStream_t s);
+
+}  // namespace avionic

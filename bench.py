@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""PD iterations/s of ICTGV2 at 256x256 x 30 frames x 16 coils, batched vs plain.
+
+One step = one primal-dual iteration (dual update + projections, data-term
+z update with K and K^H, primal update + extragradient, and the step-size rule
+on its schedule k < 10 || k % 50 == 0) over every resident reconstruction.
+
+* plain:   one reconstruction, the task-list kernels launched eagerly per
+           iteration (no graph, no batching).
+* batched: --batch reconstructions resident, advanced together by one
+           task-list launch sequence per iteration, replayed from a hipGraph.
+
+value = batched aggregate PD iterations/s (iterations x reconstructions per
+second); inputs are resident in HBM before the timed region.  The roofline
+entry is the dominant kernel's algorithmic bytes per launch over its average
+duration, timed with HIP events on the engine's stream.  cpu_baseline is the
+FP64 oracle (C, OpenMP) on the host cores for a few iterations of the same
+reconstruction.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+                       [--dual fp32|bf16] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_voxel(kernel, method, bf16):
+    """Compulsory HBM bytes per voxel of the regulariser kernels.
+    dual:   read ext (P), read y (D), write y (D)
+    primal: read y (D), read K^H z (1), read x (P), write x (P), write ext (P)"""
+    P = {"tv": 1, "tgv2": 4, "ictgv2": 8}[method]
+    D = {"tv": 3, "tgv2": 9, "ictgv2": 18}[method]
+    bp, bd = 8, (4 if bf16 else 8)
+    if kernel == "dual":
+        return P * bp + 2 * D * bd
+    if kernel == "primal":
+        return D * bd + bp + 3 * P * bp
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--dual", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--workload", default="ictgv2_256x30x16")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested). Reconstructions shard by request; run one "
+              f"process per GPU instead.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import avionic_amd as A
+    from avionic_amd import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    wl = synth.CONFIGS[args.workload]
+    log(f"generating {wl.name} inputs")
+    inp = synth.make(wl)
+    T, C, H, W = inp["data"].shape
+    method = {"tv": 0, "tgv2": 1, "ictgv2": 2}[wl.method]
+    bf16 = args.dual == "bf16"
+
+    def c64(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.complex64)).to(dev)
+
+    data, b1, x0 = c64(inp["data"]), c64(inp["b1"]), c64(inp["x0"])
+    mask = torch.from_numpy(np.ascontiguousarray(inp["mask"], np.float32)).to(dev)
+    total_its = args.warmup + args.steps + 10
+    params = dict(wl.params)
+    params.update(maxIt=total_its, dualStorage=1 if bf16 else 0)
+
+    def request():
+        return A.Request(method, data, b1, mask=mask, x0=x0, centered=False, params=params)
+
+    def timed(engine, n_req):
+        for _ in range(args.warmup):
+            engine.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            engine.step()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # ---- plain: one reconstruction, eager launches
+    log("plain path")
+    eng = A.Engine(use_graphs=False, max_batch=1)
+    rid = eng.submit(request())
+    t_plain = timed(eng, 1)
+    plain_its = args.steps / t_plain
+    del eng
+
+    # ---- batched: B reconstructions, one graph-replayed launch sequence
+    log(f"batched path (B={args.batch})")
+    eng = A.Engine(use_graphs=True, max_batch=args.batch)
+    ids = [eng.submit(request()) for _ in range(args.batch)]
+    assert all(i is not None for i in ids)
+    t_batch = timed(eng, args.batch)
+    batched_its = args.batch * args.steps / t_batch
+    ms_per_step = 1e3 * t_batch / args.steps
+    del eng
+
+    # ---- same result alone and in the mix (60 iterations covers the k<10
+    #      and k%50 step-rule iterations)
+    log("alone vs in-mix check")
+    p60 = dict(params, maxIt=60)
+    alone = A.Engine(use_graphs=False, max_batch=1)
+    ra = alone.submit(A.Request(method, data, b1, mask=mask, x0=x0, params=p60))
+    xa = torch.empty_like(x0)
+    while not alone.poll(ra, xa):
+        alone.step()
+    mix = A.Engine(use_graphs=True, max_batch=4)
+    other = dict(params, maxIt=90)
+    mix.submit(A.Request(method, data, b1, mask=mask, x0=torch.zeros_like(x0), params=other))
+    rm = mix.submit(A.Request(method, data, b1, mask=mask, x0=x0, params=p60))
+    xm = torch.empty_like(x0)
+    while not mix.poll(rm, xm):
+        mix.step()
+    torch.cuda.synchronize()
+    mix_diff = float((xa - xm).abs().max().item())
+    del alone, mix
+
+    # ---- per-kernel times (HIP events on the engine stream), fresh engine
+    log("kernel timing")
+    eng = A.Engine(use_graphs=False, max_batch=args.batch)
+    for _ in range(args.batch):
+        eng.submit(request())
+    ktimes = A.engine_time_kernels(eng, reps=20)
+    del eng
+    N = W * H * T
+    roof = None
+    per_voxel = {k: algorithmic_bytes_per_voxel(k, wl.method, bf16) for k in ("dual", "primal")}
+    dom = max(per_voxel, key=lambda k: ktimes[k])
+    bytes_per_launch = per_voxel[dom] * N * args.batch
+    achieved = bytes_per_launch / (ktimes[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(ktimes[dom], 4)}
+
+    # ---- CPU baseline: FP64 oracle on the host cores, a few iterations
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (FP64 oracle)")
+        import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        p = O.default_params(wl.method)
+        k, d = O.adapt_lambda_defaults(wl.method)
+        lam = wl.params.get("lambda") or O.adapt_lambda_cart(inp["mask"], W, H, T, k, d)
+        p.update(maxIt=args.cpu_iters, lambda_=lam, **{kk: v for kk, v in wl.params.items()
+                                                      if kk in ("alpha0", "alpha1")})
+        t0 = time.perf_counter()
+        O.recon(p, inp["data"], inp["b1"], inp["mask"], inp["x0"], nthreads=cores)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(args.cpu_iters / dt, 4), "unit": "PD iterations/s", "cores": cores,
+               "kind": "port",
+               "sample": f"{args.cpu_iters} PD iterations of one {wl.name} reconstruction "
+                         f"(FP64 C oracle, OpenMP {cores} threads)"}
+
+    out = {
+        "metric": "PD iters/s ICTGV2 256x256x30x16c, batched vs plain",
+        "value": round(batched_its, 2),
+        "unit": "PD iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64" + ("+bf16-duals" if bf16 else ""),
+        "data": "synthetic",
+        "config": {"workload": wl.name, "method": wl.method, "matrix": [H, W], "frames": T,
+                   "coils": C, "accel": wl.accel, "batch": args.batch,
+                   "dual_storage": args.dual, "parallelism": "single-gpu"},
+        "plain_iters_per_s": round(plain_its, 2),
+        "batched_over_plain": round(batched_its / plain_its, 3),
+        "alone_vs_mix_max_abs_diff": mix_diff,
+        "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

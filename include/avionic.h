@@ -270,6 +270,8 @@ typedef struct avionic_request_desc {
   const float *data;             /* device, [frames][coils][height][width] */
   const float *b1;               /* device, [coils][height][width] */
   const float *x0;               /* device, [frames][height][width] or NULL */
+  void *stream;                  /* stream that produced the inputs; submit
+                                    returns once they are copied */
 } avionic_request_desc;
 
 int avionic_engine_create(const avionic_engine_config *cfg,
@@ -292,6 +294,17 @@ int avionic_engine_cancel(avionic_engine *engine, long long id);
 /* Requests resident / bytes in use. */
 int avionic_engine_stats(const avionic_engine *engine, int *resident,
                          size_t *bytes_in_use);
+/* Average duration in ms of each kernel of one iteration (dual,
+ * col_forward, row_dual_update, col_adjoint, primal, step_reg,
+ * col_forward_delta, row_norm2, tail) over the first resident batch, each
+ * launched `reps` times alone between HIP events.  Advances the resident
+ * state: use a dedicated engine.  n: capacity in, count out. */
+int avionic_engine_time_kernels(avionic_engine *engine, int reps, double *ms,
+                                const char **names, size_t *n);
+/* sigma/tau after each iteration run so far (n: capacity in, count out);
+ * available until the request is polled or cancelled. */
+int avionic_engine_get_step_log(const avionic_engine *engine, long long id,
+                                double *sigma, double *tau, size_t *n);
 
 #ifdef __cplusplus
 }

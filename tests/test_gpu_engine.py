@@ -1,0 +1,129 @@
+"""Batched / served execution: every request's result equals the oracle and
+equals its own result alone, for mixed methods, shapes, dual storage and
+iteration counts resident together; admission, cancel and polling semantics."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(T, H, W, C, seed):
+    r = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = np.zeros((T, H, W), complex)
+    for t in range(T):
+        img[t] += 120.0 * (((yy - H / 2 - t) ** 2 + (xx - W / 2) ** 2) < (H / 4) ** 2)
+    b1 = 1.0 + 0.2 * (r.standard_normal((C, H, W)) + 1j * r.standard_normal((C, H, W)))
+    b1 /= np.sqrt((np.abs(b1) ** 2).sum(0))
+    mask = (r.random((T, H, 1)) < 0.4) * np.ones((1, 1, W))
+    mask[:, 0] = 1.0
+    data = O.cart_K(img, b1, mask)
+    data += 0.5 * (r.standard_normal(data.shape) + 1j * r.standard_normal(data.shape)) * mask[:, None]
+    return data, b1, mask
+
+
+def _dev(torch, a, dt=None):
+    import numpy as np
+    return torch.from_numpy(np.ascontiguousarray(a, dt or np.complex64)).to("cuda")
+
+
+CASES = [  # method, T, H, W, C, maxIt, dualStorage
+    (2, 4, 32, 32, 4, 60, 0),
+    (0, 3, 24, 40, 3, 45, 0),
+    (1, 5, 16, 16, 2, 70, 0),
+    (2, 3, 20, 28, 3, 30, 1),
+    (2, 2, 32, 32, 5, 55, 0),
+]
+
+
+def _params(method, maxit, dual, mask, W, H, T):
+    k, d = O.adapt_lambda_defaults(method)
+    return dict(maxIt=maxit, dualStorage=dual, adaptLambda=0,
+                **{"lambda": O.adapt_lambda_cart(mask, W, H, T, k, d)})
+
+
+def test_engine_mixed_batch_matches_alone_and_oracle(gpu):
+    import torch
+    import avionic_amd as A
+    reqs, refs, alone = [], [], []
+    for i, (m, T, H, W, C, it, dual) in enumerate(CASES):
+        data, b1, mask = _inputs(T, H, W, C, seed=i)
+        p = _params(m, it, dual, mask, W, H, T)
+        reqs.append(A.Request(m, _dev(torch, data), _dev(torch, b1),
+                              mask=_dev(torch, mask, np.float32), params=p))
+        op = O.default_params(m)
+        op.update(maxIt=it, lambda_=p["lambda"])
+        refs.append((O.recon(op, data, b1, mask), dual))
+    # alone, eager
+    for r in reqs:
+        e = A.Engine(use_graphs=False, max_batch=1)
+        rid = e.submit(r)
+        out = torch.empty(r.shape[3], r.shape[1], r.shape[0], dtype=torch.complex64, device="cuda")
+        while not e.poll(rid, out):
+            e.step()
+        alone.append(out.cpu().numpy())
+    # all together, graphs, several iterations per tick
+    e = A.Engine(use_graphs=True, iterations_per_tick=7)
+    ids = [e.submit(r) for r in reqs]
+    outs = {}
+    while len(outs) < len(ids):
+        e.step()
+        for i, rid in enumerate(ids):
+            if i in outs:
+                continue
+            r = reqs[i]
+            buf = torch.empty(r.shape[3], r.shape[1], r.shape[0], dtype=torch.complex64, device="cuda")
+            if e.poll(rid, buf):
+                outs[i] = buf.cpu().numpy()
+    for i in range(len(reqs)):
+        assert np.array_equal(outs[i], alone[i]), f"request {i} differs alone vs in the mix"
+        ref, dual = refs[i]
+        err = np.linalg.norm(outs[i] - ref) / np.linalg.norm(ref)
+        print(f"[engine] case {i} NRMSE vs oracle {err:.2e}", flush=True)
+        assert err < (2e-2 if dual else 1e-4)
+
+
+def test_engine_admission_cancel_and_errors(gpu):
+    import torch
+    import avionic_amd as A
+    data, b1, mask = _inputs(3, 16, 16, 2, seed=9)
+    r = A.Request(0, _dev(torch, data), _dev(torch, b1), mask=_dev(torch, mask, np.float32),
+                  params=dict(maxIt=20))
+    fp = A.Engine.footprint(r)
+    assert fp > 0
+    e = A.Engine(hbm_budget_bytes=int(fp * 1.5))
+    a = e.submit(r)
+    assert a is not None
+    assert e.submit(r) is None  # over budget now
+    assert e.stats()[0] == 1
+    e.cancel(a)
+    b = e.submit(r)  # fits again after cancel
+    assert b is not None
+    with pytest.raises(A.AvionicError) as ex:
+        e.poll(12345, torch.empty(1, dtype=torch.complex64, device="cuda"))
+    assert ex.value.code == A.AVIONIC_E_NOTFOUND
+    out = torch.empty(3, 16, 16, dtype=torch.complex64, device="cuda")
+    while not e.poll(b, out):
+        e.step()
+    assert torch.isfinite(torch.view_as_real(out)).all()
+    assert e.stats() == (0, 0)
+
+
+def test_engine_step_log_matches_oracle(gpu):
+    import torch
+    import avionic_amd as A
+    data, b1, mask = _inputs(4, 32, 32, 4, seed=5)
+    p = _params(2, 120, 0, mask, 32, 32, 4)
+    e = A.Engine(use_graphs=True, iterations_per_tick=13)
+    rid = e.submit(A.Request(2, _dev(torch, data), _dev(torch, b1), mask=_dev(torch, mask, np.float32),
+                             params=p))
+    for _ in range(10):
+        e.step()
+    s, t = e.step_log(rid)
+    op = O.default_params("ictgv2")
+    op.update(maxIt=120, lambda_=p["lambda"])
+    ref = O.recon(op, data, b1, mask, logs=True)
+    np.testing.assert_allclose(s, ref["sigma"], rtol=1e-5)
+    np.testing.assert_allclose(t, ref["tau"], rtol=1e-5)
