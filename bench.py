@@ -94,6 +94,8 @@ def main():
     total_its = args.warmup + args.steps + 10
     params = dict(wl.params)
     params.update(maxIt=total_its, dualStorage=1 if bf16 else 0)
+    if "lambda" in wl.params:
+        params["adaptLambda"] = 0
 
     def request():
         return A.Request(method, data, b1, mask=mask, x0=x0, centered=False, params=params)
