@@ -31,6 +31,7 @@ struct CartGeom {
   const int *rows;     // [nrows] ky of each sampled row, frame-major
   const float *mrows;  // [nrows][W] mask values of the sampled rows
   const int4 *rowinfo; // [nrows_c] (t, c, r = row_off[t] + k, ky) per compact row
+  const int *rowmap;   // [T][H] compact row index k of (t, ky), -1 if not sampled
   float scale;         // 1 / sqrt(H W)
   FftPlan pw, ph;
 };
@@ -51,7 +52,8 @@ class FftPlanCache;
 struct CartPlan {
   CartGeom g{};
   std::vector<int> h_row_off, h_rows;
-  void *d_row_off = nullptr, *d_rows = nullptr, *d_mrows = nullptr, *d_rowinfo = nullptr;
+  void *d_row_off = nullptr, *d_rows = nullptr, *d_mrows = nullptr, *d_rowinfo = nullptr,
+       *d_rowmap = nullptr;
   void *d_twx = nullptr, *d_twy = nullptr;
   double mask_norm2 = 0.0;  // ||mask||^2 (AdaptLambda)
   bool has_mask = false;

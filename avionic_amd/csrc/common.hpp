@@ -84,8 +84,10 @@ struct DualIO<bf16x2> {
 // ------------------------------------------------------------ FFT tiling
 // Elements (complex) held in LDS by one FFT tile, and threads per tile.
 constexpr int kFftThreads = 256;
-constexpr int kFftTileElems = 4096;  // 32 KiB of float2
+constexpr int kFftTileElems = 4096;  // 32 KiB of float2 (column passes)
+constexpr int kRowTileElems = 2048;  // 16 KiB of float2 (row passes)
 constexpr int kMaxStages = 12;
+constexpr int kMaxFftLen = 512;  // longest 1D transform (twiddles staged in LDS)
 
 // A 1D FFT plan for one length: Stockham stages (radix, Ns) and twiddles.
 struct FftPlan {

@@ -211,4 +211,150 @@ __device__ inline void fft_lds(float2 *buf, const FftPlan &p, int B, int bstride
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused transforms: the first stage may take its inputs from a load functor
+// (global memory) instead of LDS, and the last stage may hand its outputs to
+// a store functor instead of writing LDS.  Only the exchanges between stages
+// touch LDS.  `tw` may point to LDS (a per-block copy of the twiddles).
+//
+//   ld(b, i, slot) -> float2  input element i of transform b   (natural order)
+//   st(b, i, v, slot)        output element i of transform b   (natural order)
+// The lane -> (b, j) mapping of a stage is the same for every call with the
+// same plan and B, so a store functor may keep per-lane state across calls:
+// `slot` (a compile-time constant once the stage is unrolled) numbers the
+// outputs a lane produces, at most kMaxSlots of them.
+
+template <int R, int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, class LD, class ST>
+__device__ inline void fft_stage_io(float2 *buf, int L, int B, int bstride, int istride, int Ns,
+                                    float sign, const float2 *__restrict__ tw, int tid, LD &ld,
+                                    ST &st) {
+  constexpr int MAXB = (EMAX / R + NT - 1) / NT;
+  asm volatile("" : "+v"(tid));
+  const int nbf = L / R;
+  const int total = nbf * B;
+  const int twstep = L / (Ns * R);
+  const int rstride = nbf * istride;
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int c = 0; c < MAXB; ++c) {
+    const int idx = tid + c * NT;
+    if (idx < total) {
+      int b, j;
+      if (BFAST) { b = idx % B; j = idx / B; } else { j = idx % nbf; b = idx / nbf; }
+      const int k = j % Ns;
+      if (FROMG) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[c][r] = ld(b, j + r * nbf, c * R + r);
+      } else {
+        const int base = b * bstride + j * istride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[c][r] = buf[base + r * rstride];
+      }
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * twstep];
+          w.y = -sign * w.y;
+          v[c][r] = cmul(v[c][r], w);
+        }
+      }
+      Dft<R>::run(v[c], sign);
+    }
+  }
+  if (!FROMG && !TOG) __syncthreads();  // in place: all reads before any write
+#pragma unroll
+  for (int c = 0; c < MAXB; ++c) {
+    const int idx = tid + c * NT;
+    if (idx < total) {
+      int b, j;
+      if (BFAST) { b = idx % B; j = idx / B; } else { j = idx % nbf; b = idx / nbf; }
+      const int k = j % Ns;
+      const int o = (j / Ns) * Ns * R + k;
+      if (TOG) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) st(b, o + r * Ns, v[c][r], c * R + r);
+      } else {
+        const int base = b * bstride + o * istride;
+        const int ostride = Ns * istride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[base + r * ostride] = v[c][r];
+      }
+    }
+  }
+  if (!TOG) __syncthreads();
+}
+
+template <int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, class LD, class ST>
+__device__ inline void fft_stage_io_dispatch(int R, float2 *buf, int L, int B, int bstride,
+                                             int istride, int Ns, float sign, const float2 *tw,
+                                             int tid, LD &ld, ST &st) {
+  switch (R) {
+    case 16: fft_stage_io<16, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    case 8: fft_stage_io<8, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    case 4: fft_stage_io<4, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    case 2: fft_stage_io<2, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    case 3: fft_stage_io<3, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    case 5: fft_stage_io<5, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    case 7: fft_stage_io<7, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    default: break;
+  }
+}
+
+struct NoLoad {
+  __device__ float2 operator()(int, int, int) const { return make_float2(0.f, 0.f); }
+};
+struct NoStore {
+  __device__ void operator()(int, int, float2, int) {}
+};
+
+// Outputs per lane of one stage: max over radices of ceil(EMAX/R/NT) * R.
+constexpr int max_slots(int emax, int nt) {
+  int m = 0;
+  const int rs[] = {16, 8, 4, 2, 3, 5, 7};
+  for (int r : rs) {
+    const int v = ((emax / r + nt - 1) / nt) * r;
+    m = v > m ? v : m;
+  }
+  return m;
+}
+
+// Transform with optional global input / output.  With FROMG the LDS tile's
+// previous contents may still be being read by other lanes: callers sync
+// before re-using a tile.  With TOG the LDS tile is free once this returns
+// (the last stage read it before the function's final store loop; lanes may
+// still be reading it, so callers sync before writing the tile again).
+template <int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, class LD, class ST>
+__device__ inline void fft_io(float2 *buf, const FftPlan &p, int B, int bstride, int istride,
+                              float sign, int tid, const float2 *tw, LD &ld, ST &st) {
+  const int n = p.nst;
+  if (n == 0) {  // length 1: identity
+    for (int b = tid; b < B; b += NT) {
+      const float2 v = FROMG ? ld(b, 0, 0) : buf[b * bstride];
+      if (TOG) st(b, 0, v, 0); else buf[b * bstride] = v;
+    }
+    if (!TOG) __syncthreads();
+    return;
+  }
+  if (n == 1) {
+    fft_stage_io_dispatch<NT, EMAX, BFAST, FROMG, TOG>(p.radix[0], buf, p.L, B, bstride, istride,
+                                                       p.ns[0], sign, tw, tid, ld, st);
+    return;
+  }
+  NoStore ns;
+  NoLoad nl;
+  fft_stage_io_dispatch<NT, EMAX, BFAST, FROMG, false>(p.radix[0], buf, p.L, B, bstride, istride,
+                                                       p.ns[0], sign, tw, tid, ld, ns);
+  for (int s = 1; s < n - 1; ++s)
+    fft_stage_io_dispatch<NT, EMAX, BFAST, false, false>(p.radix[s], buf, p.L, B, bstride, istride,
+                                                         p.ns[s], sign, tw, tid, nl, ns);
+  fft_stage_io_dispatch<NT, EMAX, BFAST, false, TOG>(p.radix[n - 1], buf, p.L, B, bstride, istride,
+                                                     p.ns[n - 1], sign, tw, tid, nl, st);
+}
+
+// Copy a plan's twiddles to LDS (L <= kMaxFftLen) and return the LDS view.
+__device__ inline const float2 *stage_twiddles(const FftPlan &p, float2 *lds, int tid, int nt) {
+  for (int i = tid; i < p.L; i += nt) lds[i] = p.tw[i];
+  return lds;
+}
+
 }  // namespace avionic

@@ -1,5 +1,6 @@
 // Cartesian operator kernels for the per-operator API (K, K^H, norms); the
 // block bodies live in cart_device.hpp.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -112,7 +113,7 @@ static int pick_xb(int H, int W) {
 }
 
 void make_fft_plan(int L, FftPlan &p, void **d_tw) {
-  AV_CHECK(L >= 1 && L <= EMAX, "FFT length out of range");
+  AV_CHECK(L >= 1 && L <= kMaxFftLen, "FFT length out of range (1..512)");
   p = FftPlan{};
   p.L = L;
   int m = L, ns = 1;
@@ -139,18 +140,18 @@ void make_fft_plan(int L, FftPlan &p, void **d_tw) {
 }
 
 CartPlan::~CartPlan() {
-  for (void *p : {d_row_off, d_rows, d_mrows, d_rowinfo, d_twx, d_twy})
+  for (void *p : {d_row_off, d_rows, d_mrows, d_rowinfo, d_rowmap, d_twx, d_twy})
     if (p) (void)hipFree(p);
 }
 
 void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered, hipStream_t s) {
   AV_CHECK(W >= 1 && H >= 1 && C >= 1 && T >= 1, "invalid operator dimensions");
-  AV_CHECK(H <= EMAX && W <= EMAX, "image dimensions exceed the FFT tile");
+  AV_CHECK(H <= kMaxFftLen && W <= kMaxFftLen, "image dimensions exceed the longest FFT (512)");
   g = CartGeom{};
   g.W = W; g.H = H; g.C = C; g.T = T;
   g.centered = centered ? 1 : 0;
   g.XB = pick_xb(H, W);
-  g.RB = EMAX / W;
+  g.RB = std::max(1, kRowTileElems / W);
   g.nxb = (W + g.XB - 1) / g.XB;
   g.scale = (float)(1.0 / std::sqrt((double)H * W));
   const long HW = (long)H * W;
@@ -183,6 +184,9 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   }
   g.nrows = (long)h_rows.size();
   g.nrows_c = g.nrows * C;
+  std::vector<int> rowmap((size_t)T * H, -1);
+  for (int t = 0; t < T; ++t)
+    for (int k = h_row_off[t]; k < h_row_off[t + 1]; ++k) rowmap[(size_t)t * H + h_rows[k]] = k - h_row_off[t];
   std::vector<int4> info;
   info.reserve(g.nrows_c);
   for (int t = 0; t < T; ++t) {
@@ -198,10 +202,12 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   up(&d_rows, h_rows.data(), sizeof(int) * h_rows.size());
   up(&d_mrows, mrows.data(), sizeof(float) * mrows.size());
   up(&d_rowinfo, info.data(), sizeof(int4) * info.size());
+  up(&d_rowmap, rowmap.data(), sizeof(int) * rowmap.size());
   g.row_off = (const int *)d_row_off;
   g.rows = (const int *)d_rows;
   g.mrows = (const float *)d_mrows;
   g.rowinfo = (const int4 *)d_rowinfo;
+  g.rowmap = (const int *)d_rowmap;
   make_fft_plan(W, g.pw, &d_twx);
   make_fft_plan(H, g.ph, &d_twy);
 }

@@ -1,7 +1,7 @@
 // Task-list kernels of the PD iteration (see tasks.hpp).  Each block finds
 // its task, checks the task's device-side predicates (uniform per block, so a
 // whole block returns together) and runs the shared block / voxel bodies.
-#include "cart_device.hpp"
+#include "cart_fused.hpp"
 #include "reg_device.hpp"
 #include "tasks.hpp"
 
@@ -67,36 +67,49 @@ __global__ __launch_bounds__(RT) void kt_step_reg(const Task *__restrict__ tasks
 // DELTA = false: K of the extragradient image (dual update);
 // DELTA = true : K of ext - x (step rule, adapting tasks only).
 template <bool DELTA>
-__global__ __launch_bounds__(NT) void kt_col_forward(const Task *__restrict__ tasks, int n) {
+__global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n) {
   __shared__ float2 tile[EMAX];
+  __shared__ float2 twl[kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
   if (!task_active(T.sc)) return;
   if (DELTA && !task_adapt(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
-  col_forward_block(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.ext, DELTA ? T.x : nullptr, T.b1,
-                    T.kcol, tile);
+  col_forward_fused<DELTA>(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.ext, DELTA ? T.x : nullptr,
+                           T.b1, T.kcol, tile, twl);
 }
 
 template <int MODE>
 __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int n) {
-  __shared__ float2 tile[EMAX];
-  __shared__ float2 tile2[(MODE == kRowDualUpdate) ? EMAX : 1];
+  __shared__ float2 tile[RTILE];
+  __shared__ float2 tile2[(MODE == kRowDualUpdate) ? RTILE : 1];
+  __shared__ float2 twl[kMaxFftLen];
   __shared__ double red[NT];
   const Task &T = tasks[find_task<kRow>(tasks, n, blockIdx.x)];
   if (!task_active(T.sc)) return;
   if (MODE == kRowNorm2 && !task_adapt(T.sc)) return;
   const int lb = blockIdx.x - T.b_row;
-  row_pass_block<MODE>(T.cg, lb, T.kcol, T.z, T.d, T.zhat, nullptr, nullptr, T.sc, T.part_k, tile,
-                       tile2, red);
+  if (MODE == kRowDualUpdate) {
+    row_dual_fused(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, tile2, twl);
+  } else {
+    const double a = row_norm2_fused(T.cg, lb, T.kcol, tile, twl);
+    const int tid = threadIdx.x;
+    red[tid] = a;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) T.part_k[lb] = red[0];
+  }
 }
 
-__global__ __launch_bounds__(NT) void kt_col_adjoint(const Task *__restrict__ tasks, int n) {
+__global__ __launch_bounds__(NTC) void kt_col_adjoint(const Task *__restrict__ tasks, int n) {
   __shared__ float2 tile[EMAX];
-  __shared__ float2 acc[EMAX];
+  __shared__ float2 twl[kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
   if (!task_active(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
-  col_adjoint_block(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.zhat, T.b1, T.khz, tile, acc);
+  col_adjoint_fused(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.zhat, T.b1, T.khz, tile, twl);
 }
 
 // One block per task: step rule for adapting tasks, then the iteration
@@ -144,7 +157,7 @@ const char *kernel_name(int id) {
 void launch_kernel(int id, int method, bool bf, const Task *d_tasks, int n, const TaskGrid &gr,
                    hipStream_t s) {
   if (n <= 0) return;
-  const dim3 bRT(RT), bNT(NT);
+  const dim3 bRT(RT), bNT(NT), bNC(NTC);
 #define AV_BYMETHOD2(KER, ...)                                                       \
   do {                                                                               \
     if (!bf) {                                                                       \
@@ -159,9 +172,9 @@ void launch_kernel(int id, int method, bool bf, const Task *d_tasks, int n, cons
   } while (0)
   switch (id) {
     case kKDual: if (gr.vox) AV_BYMETHOD2(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
-    case kKColFwd: if (gr.col) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNT, 0, s, d_tasks, n); break;
+    case kKColFwd: if (gr.col) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNC, 0, s, d_tasks, n); break;
     case kKRowDual: if (gr.row) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
-    case kKColAdj: if (gr.col) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNT, 0, s, d_tasks, n); break;
+    case kKColAdj: if (gr.col) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNC, 0, s, d_tasks, n); break;
     case kKPrimal: if (gr.vox) AV_BYMETHOD2(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
     case kKStepReg:
       if (gr.vox) {
@@ -170,7 +183,7 @@ void launch_kernel(int id, int method, bool bf, const Task *d_tasks, int n, cons
         else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
       }
       break;
-    case kKColFwdDelta: if (gr.col) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNT, 0, s, d_tasks, n); break;
+    case kKColFwdDelta: if (gr.col) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNC, 0, s, d_tasks, n); break;
     case kKRowNorm2: if (gr.row) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
     case kKTail: hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n); break;
     default: break;
@@ -182,7 +195,7 @@ void launch_kernel(int id, int method, bool bf, const Task *d_tasks, int n, cons
 void launch_iteration(int method, bool bf, const Task *d_tasks, int n, const TaskGrid &gr,
                       bool with_step, hipStream_t s) {
   if (n <= 0) return;
-  const dim3 bRT(RT), bNT(NT);
+  const dim3 bRT(RT), bNT(NT), bNC(NTC);
 #define AV_BYMETHOD(KER, ...)                                                        \
   do {                                                                               \
     if (!bf) {                                                                       \
@@ -197,9 +210,9 @@ void launch_iteration(int method, bool bf, const Task *d_tasks, int n, const Tas
   } while (0)
   if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();
-  if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNT, 0, s, d_tasks, n);
+  if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNC, 0, s, d_tasks, n);
   if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n);
-  if (gr.col > 0) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNT, 0, s, d_tasks, n);
+  if (gr.col > 0) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNC, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();
   if (gr.vox > 0) AV_BYMETHOD(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();
@@ -209,7 +222,7 @@ void launch_iteration(int method, bool bf, const Task *d_tasks, int n, const Tas
       else if (method == kTGV2) hipLaunchKernelGGL(kt_step_reg<kTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
       else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
     }
-    if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNT, 0, s, d_tasks, n);
+    if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNC, 0, s, d_tasks, n);
     if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n);
     AV_LAUNCH_CHECK();
   }

@@ -148,6 +148,14 @@ def main():
     mix_diff = float((xa - xm).abs().max().item())
     del alone, mix
 
+    # ---- a plain device copy of known size, the reference point that scales
+    #      rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes
+    calib = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
+    calib_out = torch.empty_like(calib)
+    calib_out.copy_(calib)
+    torch.cuda.synchronize()
+    del calib, calib_out
+
     # ---- per-kernel times (HIP events on the engine stream), fresh engine
     log("kernel timing")
     eng = A.Engine(use_graphs=False, max_batch=args.batch)

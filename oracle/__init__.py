@@ -325,3 +325,93 @@ def primal_components(method):
 
 def dual_components(method):
     return lib().orc_dual_components(int(method))
+
+
+# -------------------------------------------------------------- non-Cartesian
+def kb_beta(J, osf):
+    return lib().orc_kb_beta(int(J), float(osf))
+
+
+def _nc_dims(traj):
+    T, two, M = traj.shape
+    assert two == 2
+    return T, M
+
+
+def nufft_K(x, b1, traj, dcf=None, J=3, osf=1.5):
+    x, b1 = _c(x), _c(b1)
+    traj = _c(traj, np.float64)
+    T, H, W = x.shape
+    C = b1.shape[0]
+    _, M = _nc_dims(traj)
+    w = None if dcf is None else _c(dcf, np.float64)
+    out = np.zeros((T, C, M), np.complex128)
+    lib().orc_nufft_K(_ptr(x), _ptr(b1), _ptr(traj), _ptr(w), W, H, C, T, M, int(J), float(osf),
+                      _ptr(out))
+    return out
+
+
+def nufft_KH(k, b1, traj, dcf=None, J=3, osf=1.5, shape=None):
+    k, b1 = _c(k), _c(b1)
+    traj = _c(traj, np.float64)
+    T, C, M = k.shape
+    H, W = b1.shape[1:]
+    w = None if dcf is None else _c(dcf, np.float64)
+    out = np.zeros((T, H, W), np.complex128)
+    lib().orc_nufft_KH(_ptr(k), _ptr(b1), _ptr(traj), _ptr(w), W, H, C, T, M, int(J), float(osf),
+                       _ptr(out))
+    return out
+
+
+def nudft_K(x, b1, traj, dcf=None):
+    x, b1 = _c(x), _c(b1)
+    traj = _c(traj, np.float64)
+    T, H, W = x.shape
+    C = b1.shape[0]
+    _, M = _nc_dims(traj)
+    w = None if dcf is None else _c(dcf, np.float64)
+    out = np.zeros((T, C, M), np.complex128)
+    lib().orc_nudft_K(_ptr(x), _ptr(b1), _ptr(traj), _ptr(w), W, H, C, T, M, _ptr(out))
+    return out
+
+
+def recon_nc(params: dict, data, b1, traj, dcf=None, x0=None, J=3, osf=1.5, nthreads=0,
+             logs=False):
+    L = lib()
+    if not hasattr(L, "_nc_declared"):
+        L.orc_recon_nc.argtypes = [ctypes.POINTER(OrcParams), _I, _I, _I, _I, _I, _I, _D, _P, _P,
+                                   _P, _P, _P, _P, _P, _P, _I]
+        L.orc_recon_nc.restype = _I
+        L._nc_declared = True
+    pr = OrcParams(**params)
+    data, b1 = _c(data), _c(b1)
+    traj = _c(traj, np.float64)
+    T, C, M = data.shape
+    H, W = b1.shape[1:]
+    w = None if dcf is None else _c(dcf, np.float64)
+    P = L.orc_primal_components(pr.method)
+    x = np.zeros((T, H, W), np.complex128) if x0 is None else _c(x0).copy()
+    xall = np.zeros((P, T, H, W), np.complex128)
+    sig, tau = np.zeros(max(pr.maxIt, 1)), np.zeros(max(pr.maxIt, 1))
+    rc = L.orc_recon_nc(ctypes.byref(pr), W, H, C, T, M, int(J), float(osf), _ptr(traj), _ptr(w),
+                        _ptr(data), _ptr(b1), _ptr(x), _ptr(xall), _ptr(sig), _ptr(tau),
+                        int(nthreads))
+    if rc < 0:
+        raise ValueError("invalid parameters")
+    if not logs:
+        return x
+    return dict(x=x, primal=xall, sigma=sig[: pr.maxIt], tau=tau[: pr.maxIt])
+
+
+def radial_trajectory(n_fe, spokes_per_frame, frames, golden=True):
+    """Golden-angle radial spokes: [frames, 2, spokes*n_fe] in cycles/pixel,
+    k in [-0.5, 0.5), angle increment 111.246 degrees (golden angle)."""
+    ga = np.pi * (np.sqrt(5.0) - 1.0) / 2.0 if golden else np.pi / spokes_per_frame
+    r = (np.arange(n_fe) - n_fe / 2.0) / n_fe
+    out = np.zeros((frames, 2, spokes_per_frame * n_fe))
+    for t in range(frames):
+        for s in range(spokes_per_frame):
+            a = (t * spokes_per_frame + s) * ga
+            out[t, 0, s * n_fe:(s + 1) * n_fe] = r * np.cos(a)
+            out[t, 1, s * n_fe:(s + 1) * n_fe] = r * np.sin(a)
+    return out

@@ -496,31 +496,63 @@ static void prox_all(int method, dc *y, long N, double alpha0, double alpha1,
   orc_prox6((double *)(y + 12 * N), N, alpha0 * ((1.0 - alpha) / m));
 }
 
-void orc_solver_adapt_step(int method, const double *deltar, const double *b1,
-                           const double *mask, int W, int H, int C, int T,
-                           int centered, const double *d6, double ratio,
-                           double *sigma, double *tau, double *nKx_out,
-                           double *nx_out) {
-  const dc *delta = (const dc *)deltar;
+/* The data operator of a reconstruction: Cartesian or non-Cartesian. */
+typedef struct {
+  int W, H, C, T;
+  long kn;  /* k-space length (complex entries) */
+  int centered;
+  const double *mask;
+  /* non-Cartesian */
+  int nc, M, J;
+  double osf;
+  const double *traj, *dcf;
+} orc_op;
+
+static void op_K(const orc_op *op, const double *x, const double *b1, double *z) {
+  if (op->nc)
+    orc_nufft_K(x, b1, op->traj, op->dcf, op->W, op->H, op->C, op->T, op->M, op->J, op->osf, z);
+  else
+    orc_cart_K(x, b1, op->mask, op->W, op->H, op->C, op->T, op->centered, z);
+}
+static void op_KH(const orc_op *op, const double *z, const double *b1, double *x) {
+  if (op->nc)
+    orc_nufft_KH(z, b1, op->traj, op->dcf, op->W, op->H, op->C, op->T, op->M, op->J, op->osf, x);
+  else
+    orc_cart_KH(z, b1, op->mask, op->W, op->H, op->C, op->T, op->centered, x);
+}
+
+static void adapt_step_op(int method, const dc *delta, const double *b1, const orc_op *op,
+                          const double *d6, double ratio, double *sigma, double *tau,
+                          double *nKx_out, double *nx_out) {
+  int W = op->W, H = op->H, T = op->T;
   long N = (long)W * H * T;
   int P = orc_primal_components(method), D = orc_dual_components(method);
   dc *ky = malloc(sizeof(dc) * D * N), *tmp = malloc(sizeof(dc) * N);
-  dc *kz = malloc(sizeof(dc) * (size_t)C * N);
+  dc *kz = malloc(sizeof(dc) * (size_t)op->kn);
   reg_K(method, delta, W, H, T, d6, ky, tmp);
-  orc_cart_K((const double *)delta, b1, mask, W, H, C, T, centered, (double *)kz);
+  op_K(op, (const double *)delta, b1, (double *)kz);
   double s = 0.0, sx = 0.0;
   for (int k = 0; k < D; k++) {
     double wk = dual_weight(method, k), acc = 0.0;
     for (long i = 0; i < N; i++) acc += abs2(ky[k * N + i]);
     s += wk * acc;
   }
-  for (long i = 0; i < (long)C * N; i++) s += abs2(kz[i]);
+  for (long i = 0; i < op->kn; i++) s += abs2(kz[i]);
   for (long i = 0; i < (long)P * N; i++) sx += abs2(delta[i]);
   double nKx = sqrt(s), nx = sqrt(sx);
   if (nKx_out) *nKx_out = nKx;
   if (nx_out) *nx_out = nx;
   orc_adapt_step(nKx, nx, ratio, sigma, tau);
   free(ky); free(tmp); free(kz);
+}
+
+void orc_solver_adapt_step(int method, const double *deltar, const double *b1,
+                           const double *mask, int W, int H, int C, int T,
+                           int centered, const double *d6, double ratio,
+                           double *sigma, double *tau, double *nKx_out,
+                           double *nx_out) {
+  orc_op op = {W, H, C, T, (long)C * W * H * T, centered, mask, 0, 0, 0, 0.0, NULL, NULL};
+  adapt_step_op(method, (const dc *)deltar, b1, &op, d6, ratio, sigma, tau, nKx_out, nx_out);
 }
 
 /* ======================================================================== */
@@ -667,17 +699,20 @@ double orc_pdgap(int method, const double *xr, const double *yr,
 /* tv_dmri.m:161-227, tgv2_dmri.m:176-255, ictgv2_dmri.m:187-282.             */
 /* ======================================================================== */
 
-int orc_recon(const orc_params *p, int W, int H, int C, int T,
-              const double *datar, const double *b1, const double *mask,
-              double *x_inout, double *x_out_all, double *sig_log,
-              double *tau_log, double *gap_log, int *n_gap, int nthreads) {
+static int recon_core(const orc_params *p, const orc_op *op, const double *datar,
+                      const double *b1, double *x_inout, double *x_out_all,
+                      double *sig_log, double *tau_log, double *gap_log, int *n_gap,
+                      int nthreads) {
+  const int W = op->W, H = op->H, C = op->C, T = op->T;
+  const double *mask = op->mask;
+  (void)C;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
   (void)nthreads;
 #endif
   int method = p->method;
-  long N = (long)W * H * T, CN = (long)C * N;
+  long N = (long)W * H * T, CN = op->kn;
   int P = orc_primal_components(method), D = orc_dual_components(method);
   double d6[6] = {1, 1, 1, 1, 1, 1};
   if (orc_time_space_weights(p->tsw, &d6[0], &d6[2]) != 0) return -1;
@@ -701,7 +736,7 @@ int orc_recon(const orc_params *p, int W, int H, int C, int T,
     /* dual ascent: y += sigma K_reg ext ; z += sigma K ext_u */
     reg_K(method, ext, W, H, T, d6, ky, tmp);
     for (long i = 0; i < (long)D * N; i++) y[i] += sigma * ky[i];
-    orc_cart_K((double *)ext, b1, mask, W, H, C, T, p->centered, (double *)kz);
+    op_K(op, (double *)ext, b1, (double *)kz);
     for (long i = 0; i < CN; i++) z[i] += sigma * kz[i];
     /* proximal maps */
     prox_all(method, y, N, p->alpha0, p->alpha1, p->alpha);
@@ -709,7 +744,7 @@ int orc_recon(const orc_params *p, int W, int H, int C, int T,
     for (long i = 0; i < CN; i++) z[i] = (z[i] - sigma * data[i]) * zs;
     /* primal descent: x+ = x - tau (K_reg^T y + [K^H z, 0...]) */
     reg_KT(method, y, W, H, T, d6, xn);
-    orc_cart_KH((double *)z, b1, mask, W, H, C, T, p->centered, (double *)img);
+    op_KH(op, (double *)z, b1, (double *)img);
     for (long i = 0; i < N; i++) xn[i] += img[i];
     for (long i = 0; i < (long)P * N; i++) xn[i] = x[i] - tau * xn[i];
     /* extragradient: ext = 2 x+ - x ; x = x+ */
@@ -720,13 +755,11 @@ int orc_recon(const orc_params *p, int W, int H, int C, int T,
     int adapt = (p->schedule == 1) || (k < 10) || (k % 50 == 0);
     if (adapt) {
       for (long i = 0; i < (long)P * N; i++) delta[i] = ext[i] - x[i];
-      orc_solver_adapt_step(method, (double *)delta, b1, mask, W, H, C, T,
-                            p->centered, d6, p->sigmaTauRatio, &sigma, &tau,
-                            NULL, NULL);
+      adapt_step_op(method, delta, b1, op, d6, p->sigmaTauRatio, &sigma, &tau, NULL, NULL);
     }
     if (sig_log) sig_log[k] = sigma;
     if (tau_log) tau_log[k] = tau;
-    if (p->gapEvery > 0 && k % p->gapEvery == 0 && gap_log) {
+    if (p->gapEvery > 0 && k % p->gapEvery == 0 && gap_log && !op->nc) {
       double g = orc_pdgap(method, (double *)x, (double *)y, (double *)z, datar,
                            b1, mask, W, H, C, T, p->centered, lambda, p->alpha0,
                            p->alpha1, p->alpha, d6);
@@ -739,4 +772,23 @@ int orc_recon(const orc_params *p, int W, int H, int C, int T,
   free(x); free(ext); free(xn); free(y); free(ky); free(z); free(kz);
   free(img); free(tmp); free(delta);
   return k;
+}
+
+int orc_recon(const orc_params *p, int W, int H, int C, int T,
+              const double *datar, const double *b1, const double *mask,
+              double *x_inout, double *x_out_all, double *sig_log,
+              double *tau_log, double *gap_log, int *n_gap, int nthreads) {
+  orc_op op = {W, H, C, T, (long)C * W * H * T, p->centered, mask, 0, 0, 0, 0.0, NULL, NULL};
+  return recon_core(p, &op, datar, b1, x_inout, x_out_all, sig_log, tau_log, gap_log, n_gap,
+                    nthreads);
+}
+
+int orc_recon_nc(const orc_params *p, int W, int H, int C, int T, int M, int J,
+                 double osf, const double *traj, const double *dcf,
+                 const double *datar, const double *b1, double *x_inout,
+                 double *x_out_all, double *sig_log, double *tau_log,
+                 int nthreads) {
+  orc_op op = {W, H, C, T, (long)C * M * T, 0, NULL, 1, M, J, osf, traj, dcf};
+  return recon_core(p, &op, datar, b1, x_inout, x_out_all, sig_log, tau_log, NULL, NULL,
+                    nthreads);
 }

@@ -127,6 +127,12 @@ void orc_nufft_KH(const double *kdata, const double *b1, const double *traj,
 void orc_nudft_K(const double *x, const double *b1, const double *traj,
                  const double *dcf, int W, int H, int C, int T, int M,
                  double *kdata);
+/* The PD loop with the non-Cartesian operator (k-space [T][C][M]). */
+int orc_recon_nc(const orc_params *p, int W, int H, int C, int T, int M, int J,
+                 double osf, const double *traj, const double *dcf,
+                 const double *data, const double *b1, double *x_inout,
+                 double *x_out_all, double *sig_log, double *tau_log,
+                 int nthreads);
 
 #ifdef __cplusplus
 }
