@@ -14,9 +14,22 @@
 
 namespace avionic {
 
+#ifndef AVIONIC_COL_THREADS
+#define AVIONIC_COL_THREADS 512
+#endif
+#ifndef AVIONIC_COL_UCACHE
+#define AVIONIC_COL_UCACHE 1
+#endif
+
 constexpr int RTILE = kRowTileElems;
-constexpr int NTC = 512;  // threads of a column-pass block (8 tile elements per lane)
+constexpr int NTC = AVIONIC_COL_THREADS;  // threads of a column-pass block
 constexpr int kColSlots = max_slots(EMAX, NTC);
+constexpr bool kColUcache = AVIONIC_COL_UCACHE != 0;  // keep the image tile in registers
+#ifndef AVIONIC_COL_ACC
+#define AVIONIC_COL_ACC 1
+#endif
+// adjoint coil sum in registers (else read-modify-write of the output per coil)
+constexpr bool kColAcc = AVIONIC_COL_ACC != 0;
 
 // ------------------------------------------------------------ column forward
 // 32-bit offsets from per-(t, c) base pointers (uniform, in SGPRs) keep one
@@ -24,7 +37,7 @@ constexpr int kColSlots = max_slots(EMAX, NTC);
 // kept in registers (a lane loads the same elements for every coil).
 template <bool DELTA>
 struct ColFwdLoad {
-  float2 ucache[kColSlots];
+  float2 ucache[kColUcache ? kColSlots : 1];
   const float2 *ut, *st, *bc;
   int H, W, xb0, centered;
   bool first;
@@ -34,6 +47,7 @@ struct ColFwdLoad {
     const int sy = centered ? (i + H / 2) % H : i;
     const int sx = centered ? (x + W / 2) % W : x;
     const unsigned si = (unsigned)(sy * W + sx);
+    if (!kColUcache) return cmul(bc[si], DELTA ? csub(ut[si], st[si]) : ut[si]);
     if (first) ucache[slot] = DELTA ? csub(ut[si], st[si]) : ut[si];
     return cmul(bc[si], ucache[slot]);
   }
@@ -91,17 +105,22 @@ struct ColAdjLoad {
   }
 };
 struct ColAdjStore {
-  float2 acc[kColSlots];
+  float2 acc[kColAcc ? kColSlots : 1];
   const float2 *bc;
   float2 *out;  // image frame t
   int H, W, xb0, centered;
-  bool last;
+  bool first, last;
   float scale;
   __device__ void operator()(int b, int i, float2 v, int slot) {
     const int x = xb0 + b;
     if (x >= W) return;
     const int y = centered ? (i + H / 2) % H : i;
     const unsigned o = (unsigned)(y * W + x);
+    if (!kColAcc) {
+      const float2 a = cadd(first ? czero() : out[o], cmulc(bc[o], v));
+      out[o] = last ? cscale(a, scale) : a;
+      return;
+    }
     const float2 a = cadd(acc[slot], cmulc(bc[o], v));
     if (last)
       out[o] = cscale(a, scale);
@@ -121,7 +140,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   ColAdjLoad ld{nullptr, g.rowmap + (long)t * g.H, g.H, g.W, xbi * g.XB, g.centered};
   ColAdjStore st;
 #pragma unroll
-  for (int k = 0; k < kColSlots; ++k) st.acc[k] = czero();
+  for (int k = 0; k < (kColAcc ? kColSlots : 1); ++k) st.acc[k] = czero();
   st.out = out + t * HW;
   st.H = g.H;
   st.W = g.W;
@@ -131,6 +150,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   for (int c = 0; c < g.C; ++c) {
     ld.zh = zhat + ((long)g.C * r0 + (long)c * S) * g.W;
     st.bc = b1 + c * HW;
+    st.first = (c == 0);
     st.last = (c == g.C - 1);
     fft_io<NTC, EMAX, true, true, true>(tile, g.ph, g.XB, 1, g.XB, -1.f, tid, tw, ld, st);
     __syncthreads();

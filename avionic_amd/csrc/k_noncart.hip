@@ -1,0 +1,323 @@
+// Kaiser-Bessel gridding passes of the non-Cartesian operator (noncart.hpp).
+#include <algorithm>
+
+#include "fft_device.hpp"
+#include "noncart.hpp"
+
+namespace avionic {
+
+namespace {
+
+constexpr int NCC = 512;   // threads of a column-pass block
+constexpr int NCR = 256;   // threads of a row-pass / sample-pass block
+constexpr int NCE = 4096;  // complex elements of an LDS transform tile
+constexpr int kColSlotsNc = max_slots(NCE, NCC);
+constexpr int kRowSlotsNc = max_slots(NCE, NCR);
+
+__device__ inline int wrap_up(int v, int G) { return v >= G ? v - G : v; }
+
+// ------------------------------------------------------------ K, column pass
+struct NcColFwdLoad {
+  float2 ucache[kColSlotsNc];
+  const float2 *ut, *us, *bc;
+  const float *dxi, *dyi;
+  int W, H, Gy, xb0;
+  bool first, delta;
+  __device__ float2 operator()(int b, int i, int slot) {
+    const int x = xb0 + b;
+    if (x >= W) return czero();
+    const int y = wrap_up(i + H / 2, Gy);
+    if (y >= H) return czero();
+    const unsigned si = (unsigned)(y * W + x);
+    if (first) {
+      const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
+      ucache[slot] = cscale(v, dxi[x] * dyi[y]);
+    }
+    return cmul(bc[si], ucache[slot]);
+  }
+};
+struct NcColFwdStore {
+  float2 *tc;  // tmp rows of (t, c): [Gy][W]
+  int W, xb0;
+  __device__ void operator()(int b, int i, float2 v, int) {
+    const int x = xb0 + b;
+    if (x < W) tc[(unsigned)(i * W + x)] = v;
+  }
+};
+
+__global__ __launch_bounds__(NCC) void k_nc_col_fwd(NcGeom g, const float2 *__restrict__ u,
+                                                    const float2 *__restrict__ usub,
+                                                    const float2 *__restrict__ b1,
+                                                    float2 *__restrict__ tmp) {
+  __shared__ float2 tile[NCE];
+  __shared__ float2 twl[kMaxFftLen];
+  const int tid = threadIdx.x;
+  const int t = blockIdx.y;
+  const long HW = (long)g.H * g.W;
+  const float2 *tw = stage_twiddles(g.py, twl, tid, NCC);
+  __syncthreads();
+  NcColFwdLoad ld;
+  ld.ut = u + t * HW;
+  ld.us = usub ? usub + t * HW : nullptr;
+  ld.delta = usub != nullptr;
+  ld.dxi = g.dxi;
+  ld.dyi = g.dyi;
+  ld.W = g.W;
+  ld.H = g.H;
+  ld.Gy = g.Gy;
+  ld.xb0 = blockIdx.x * g.XBf;
+  NcColFwdStore st{nullptr, g.W, (int)blockIdx.x * g.XBf};
+  for (int c = 0; c < g.C; ++c) {
+    ld.bc = b1 + c * HW;
+    ld.first = (c == 0);
+    st.tc = tmp + ((long)t * g.C + c) * g.Gy * g.W;
+    fft_io<NCC, NCE, true, true, true>(tile, g.py, g.XBf, 1, g.XBf, -1.f, tid, tw, ld, st);
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------- K, row pass
+struct NcRowFwdLoad {
+  const float2 *tr;  // tmp rows gy0.. of the plane: [.][W]
+  int W, Gx;
+  __device__ float2 operator()(int b, int i, int) const {
+    const int x = wrap_up(i + W / 2, Gx);
+    return x < W ? tr[(unsigned)(b * W + x)] : czero();
+  }
+};
+
+__global__ __launch_bounds__(NCR) void k_nc_row_fwd(NcGeom g, const float2 *__restrict__ tmp,
+                                                    float2 *__restrict__ part) {
+  __shared__ float2 tile[NCE];
+  __shared__ float2 twl[kMaxFftLen];
+  const int tid = threadIdx.x;
+  const int plane = blockIdx.y;
+  const int t = plane / g.C;
+  const int gy0 = blockIdx.x * g.RBf;
+  const int nr = min(g.RBf, g.Gy - gy0);
+  const int Gx = g.Gx;
+  const float2 *tw = stage_twiddles(g.px, twl, tid, NCR);
+  __syncthreads();
+  NcRowFwdLoad ld{tmp + ((long)plane * g.Gy + gy0) * g.W, g.W, Gx};
+  NoStore ns;
+  fft_io<NCR, NCE, false, true, false>(tile, g.px, nr, Gx, 1, -1.f, tid, tw, ld, ns);
+  // tile[b * Gx + i] holds grid row gy0 + b in natural order.
+  const int *rp = g.rptr + (long)t * (g.Gy + 1);
+  const int e0 = rp[gy0], e1 = rp[gy0 + nr];
+  const long pm = (long)t * g.M;
+  float2 *pp = part + (long)plane * g.M * kNcTaps;
+  for (int e = e0 + tid; e < e1; e += NCR) {
+    const int2 md = g.rlist[e];
+    const int m = md.x, dy = md.y;
+    const int2 q = g.q0[pm + m];
+    const int b = wrap_up(q.y + dy, g.Gy) - gy0;
+    const float4 wx = g.wx[pm + m];
+    const float4 wy = g.wy[pm + m];
+    const float2 *row = tile + b * Gx;
+    float2 acc = cscale(row[q.x], wx.x);
+    acc = cadd(acc, cscale(row[wrap_up(q.x + 1, Gx)], wx.y));
+    acc = cadd(acc, cscale(row[wrap_up(q.x + 2, Gx)], wx.z));
+    acc = cadd(acc, cscale(row[wrap_up(q.x + 3, Gx)], wx.w));
+    const float wyd = dy == 0 ? wy.x : dy == 1 ? wy.y : dy == 2 ? wy.z : wy.w;
+    pp[(unsigned)(m * kNcTaps + dy)] = cscale(acc, wyd);
+  }
+}
+
+// ------------------------------------------------------------- sample pass
+__device__ inline double block_sum(double v, double *red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = NCR / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(NCR) void k_nc_samples(NcGeom g, const float2 *__restrict__ part,
+                                                    float2 *__restrict__ out, float2 *z,
+                                                    const float2 *__restrict__ d,
+                                                    const DevScalars *__restrict__ sc,
+                                                    double *__restrict__ partial) {
+  __shared__ double red[NCR];
+  const long n = (long)g.T * g.C * g.M;
+  const long CM = (long)g.C * g.M;
+  float sigma = 0.f, inv = 0.f;
+  if (MODE == kNcDualUpdate) {
+    sigma = (float)sc->sigma;
+    inv = (float)(1.0 / (1.0 + sc->sigma / sc->lambda));
+  }
+  double acc = 0.0;
+  for (long e = (long)blockIdx.x * NCR + threadIdx.x; e < n; e += (long)gridDim.x * NCR) {
+    const long t = e / CM;
+    const long m = e % g.M;
+    const float4 *p4 = reinterpret_cast<const float4 *>(part + e * kNcTaps);
+    const float4 a = p4[0], b = p4[1];
+    float2 v = make_float2(a.x + a.z + b.x + b.z, a.y + a.w + b.y + b.w);
+    v = cscale(v, g.sw[t * g.M + m]);
+    if (MODE == kNcApplyK) {
+      out[e] = v;
+    } else if (MODE == kNcDualUpdate) {
+      float2 zz = cadd(z[e], cscale(v, sigma));
+      zz = cscale(csub(zz, cscale(d[e], sigma)), inv);
+      z[e] = zz;
+    } else if (MODE == kNcNorm2) {
+      acc += (double)cabs2(v);
+    } else {
+      acc += (double)cabs2(csub(v, d[e]));
+    }
+  }
+  if (MODE == kNcNorm2 || MODE == kNcResidual) {
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+// --------------------------------------------------------- K^H, column pass
+struct NcColAdjLoad {
+  const int *cp;     // cell offsets of frame t
+  const int *cm;
+  const float *cw;
+  const float2 *zp;  // z of the plane
+  int Gx, gx0;
+  __device__ float2 operator()(int b, int i, int) const {
+    const int gx = gx0 + b;
+    if (gx >= Gx) return czero();
+    const unsigned cell = (unsigned)(i * Gx + gx);
+    const int e0 = cp[cell], e1 = cp[cell + 1];
+    float2 acc = czero();
+    for (int e = e0; e < e1; ++e) acc = cadd(acc, cscale(zp[cm[e]], cw[e]));
+    return acc;
+  }
+};
+struct NcColAdjStore {
+  float2 *tp;  // tmp of the plane: [H][Gx]
+  int H, Gx, Gy, gx0;
+  __device__ void operator()(int b, int i, float2 v, int) {
+    const int gx = gx0 + b;
+    if (gx >= Gx) return;
+    const int y = wrap_up(i + H / 2, Gy);
+    if (y < H) tp[(unsigned)(y * Gx + gx)] = v;
+  }
+};
+
+__global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__restrict__ z,
+                                                    float2 *__restrict__ tmp) {
+  __shared__ float2 tile[NCE];
+  __shared__ float2 twl[kMaxFftLen];
+  const int tid = threadIdx.x;
+  const int plane = blockIdx.y;
+  const int t = plane / g.C;
+  const float2 *tw = stage_twiddles(g.py, twl, tid, NCC);
+  __syncthreads();
+  const int gx0 = blockIdx.x * g.XBa;
+  NcColAdjLoad ld{g.cptr + (long)t * ((long)g.Gx * g.Gy + 1), g.cm, g.cw, z + (long)plane * g.M,
+                  g.Gx, gx0};
+  NcColAdjStore st{tmp + (long)plane * g.H * g.Gx, g.H, g.Gx, g.Gy, gx0};
+  fft_io<NCC, NCE, true, true, true>(tile, g.py, g.XBa, 1, g.XBa, +1.f, tid, tw, ld, st);
+}
+
+// ------------------------------------------------------------ K^H, row pass
+struct NcRowAdjLoad {
+  const float2 *tr;  // tmp rows y0.. of plane (t, c)
+  int Gx;
+  __device__ float2 operator()(int b, int i, int) const { return tr[(unsigned)(b * Gx + i)]; }
+};
+struct NcRowAdjStore {
+  float2 acc[kRowSlotsNc];
+  const float2 *bc;
+  float2 *img;  // frame t
+  const float *dxi, *dyi;
+  int W, Gx, y0;
+  bool last;
+  __device__ void operator()(int b, int i, float2 v, int slot) {
+    const int x = wrap_up(i + W / 2, Gx);
+    if (x >= W) return;
+    const int y = y0 + b;
+    const unsigned o = (unsigned)(y * W + x);
+    const float2 a = cadd(acc[slot], cmulc(bc[o], v));
+    if (last)
+      img[o] = cscale(a, dxi[x] * dyi[y]);
+    else
+      acc[slot] = a;
+  }
+};
+
+__global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__restrict__ tmp,
+                                                    const float2 *__restrict__ b1,
+                                                    float2 *__restrict__ img) {
+  __shared__ float2 tile[NCE];
+  __shared__ float2 twl[kMaxFftLen];
+  const int tid = threadIdx.x;
+  const int t = blockIdx.y;
+  const int y0 = blockIdx.x * g.RBa;
+  const int nr = min(g.RBa, g.H - y0);
+  const long HW = (long)g.H * g.W;
+  const float2 *tw = stage_twiddles(g.px, twl, tid, NCR);
+  __syncthreads();
+  NcRowAdjLoad ld{nullptr, g.Gx};
+  NcRowAdjStore st;
+#pragma unroll
+  for (int k = 0; k < kRowSlotsNc; ++k) st.acc[k] = czero();
+  st.img = img + t * HW;
+  st.dxi = g.dxi;
+  st.dyi = g.dyi;
+  st.W = g.W;
+  st.Gx = g.Gx;
+  st.y0 = y0;
+  for (int c = 0; c < g.C; ++c) {
+    ld.tr = tmp + (((long)t * g.C + c) * g.H + y0) * g.Gx;
+    st.bc = b1 + c * HW;
+    st.last = (c == g.C - 1);
+    fft_io<NCR, NCE, false, true, true>(tile, g.px, nr, g.Gx, 1, +1.f, tid, tw, ld, st);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+
+void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub,
+                            const float2 *b1, float2 *tmp, float2 *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_nc_col_fwd, dim3((g.W + g.XBf - 1) / g.XBf, g.T), dim3(NCC), 0, s, g, u,
+                     usub, b1, tmp);
+  AV_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_nc_row_fwd, dim3((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C), dim3(NCR), 0, s, g,
+                     (const float2 *)tmp, part);
+  AV_LAUNCH_CHECK();
+}
+
+int nc_sample_blocks(const NcGeom &g) {
+  const long n = (long)g.T * g.C * g.M;
+  return (int)std::max<long>(1, std::min<long>(1024, (n + NCR - 1) / NCR));
+}
+
+int launch_nc_samples(int mode, const NcGeom &g, const float2 *part, float2 *out, float2 *z,
+                      const float2 *d, const DevScalars *sc, double *partial, hipStream_t s) {
+  const int nb = nc_sample_blocks(g);
+  switch (mode) {
+    case kNcApplyK: hipLaunchKernelGGL(k_nc_samples<kNcApplyK>, dim3(nb), dim3(NCR), 0, s, g, part, out, z, d, sc, partial); break;
+    case kNcDualUpdate: hipLaunchKernelGGL(k_nc_samples<kNcDualUpdate>, dim3(nb), dim3(NCR), 0, s, g, part, out, z, d, sc, partial); break;
+    case kNcNorm2: hipLaunchKernelGGL(k_nc_samples<kNcNorm2>, dim3(nb), dim3(NCR), 0, s, g, part, out, z, d, sc, partial); break;
+    case kNcResidual: hipLaunchKernelGGL(k_nc_samples<kNcResidual>, dim3(nb), dim3(NCR), 0, s, g, part, out, z, d, sc, partial); break;
+    default: AV_CHECK(false, "bad sample-pass mode");
+  }
+  AV_LAUNCH_CHECK();
+  return nb;
+}
+
+void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float2 *tmp,
+                       float2 *img, hipStream_t s) {
+  hipLaunchKernelGGL(k_nc_col_adj, dim3((g.Gx + g.XBa - 1) / g.XBa, g.T * g.C), dim3(NCC), 0, s, g,
+                     z, tmp);
+  AV_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_nc_row_adj, dim3((g.H + g.RBa - 1) / g.RBa, g.T), dim3(NCR), 0, s, g,
+                     (const float2 *)tmp, b1, img);
+  AV_LAUNCH_CHECK();
+}
+
+}  // namespace avionic

@@ -1,9 +1,260 @@
-// Placeholder until the Kaiser-Bessel gridding operator lands.
+// NoncartesianOperator (CUDA/src/noncartesian_operator.cpp:34-185) with the
+// operator definition of matlab/backward_opt_noncart.m:2-21 and
+// forward_opt_noncart.m:2-18: K applies the coil maps and a type-2 NUFFT with
+// sqrt(density) weighting, K^H is its exact adjoint.  The NUFFT is
+// Kaiser-Bessel gridding (kernel width J <= 3, grid oversampling osf,
+// beta = pi sqrt((J/osf)^2 (osf - 1/2)^2 - 0.8) of Beatty et al., IEEE TMI
+// 2005), deapodised by the kernel's continuous Fourier transform; the GPU
+// passes are described in noncart.hpp.
+//
+// The trajectory is static per operator, so the tables the passes use (the
+// separable tap weights of every point, the row -> point list of the forward
+// row pass and the cell -> point list of the adjoint's spreading) are built
+// once on the host here and kept on the device.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cartesian.hpp"
+#include "noncart.hpp"
 #include "solver.hpp"
 
 namespace avionic {
-Operator *make_noncartesian(int, int, int, int, int, int, const float *, const float *, int,
-                            double) {
-  throw Error(AVIONIC_E_UNSUPPORTED, "non-Cartesian operator not built yet");
+
+namespace {
+
+double bessel_i0(double x) {
+  double s = 1.0, t = 1.0;
+  const double q = 0.25 * x * x;
+  for (int k = 1; k < 200; ++k) {
+    t *= q / ((double)k * k);
+    s += t;
+    if (t < 1e-17 * s) break;
+  }
+  return s;
 }
+
+int pick_cols(int L, int n) {
+  int xb = 1;
+  while (xb * 2 * L <= 4096 && xb * 2 <= 64 && xb < n) xb *= 2;
+  return xb;
+}
+
+class NoncartesianOperator : public Operator {
+ public:
+  NoncartesianOperator(int W_, int H_, int C_, int T_, int nFE, int spf, const float *traj,
+                       const float *dens, int J, double osf)
+      : spf_(spf) {
+    W = W_; H = H_; C = C_; T = T_;
+    AV_CHECK(W > 0 && H > 0 && C > 0 && T > 0 && nFE > 0 && spf > 0,
+             "invalid operator dimensions");
+    AV_CHECK(J >= 1 && J <= 3, "kernel width must be 1, 2 or 3");
+    AV_CHECK(osf >= 1.0 && osf <= 4.0, "grid oversampling must be in [1, 4]");
+    const long M = (long)nFE * spf;
+    AV_CHECK(M < (1L << 26), "too many trajectory points per frame");
+    const int Gx = (int)std::lround(osf * W), Gy = (int)std::lround(osf * H);
+    AV_CHECK(Gx >= 4 && Gy >= 4, "oversampled grid too small (at least 4 x 4)");
+    AV_CHECK(Gx <= kMaxFftLen && Gy <= kMaxFftLen,
+             "oversampled grid exceeds the longest FFT (512)");
+    const long G2 = (long)Gx * Gy;
+    AV_CHECK((long)T * M * kNcTaps * kNcTaps < (1L << 31), "trajectory tables exceed 2^31 entries");
+
+    const double beta = M_PI * std::sqrt(std::pow((double)J / osf * (osf - 0.5), 2) - 0.8);
+    auto kb = [&](double u) {
+      const double r = 2.0 * u / J;
+      return std::fabs(r) > 1.0 ? 0.0 : bessel_i0(beta * std::sqrt(1.0 - r * r));
+    };
+    auto kb_ft = [&](double nu) {
+      const double a = M_PI * J * nu, d = beta * beta - a * a;
+      if (d > 1e-12) return J * std::sinh(std::sqrt(d)) / std::sqrt(d);
+      if (d < -1e-12) return J * std::sin(std::sqrt(-d)) / std::sqrt(-d);
+      return (double)J;
+    };
+    auto wrap = [](long q, long G) {
+      long r = q % G;
+      return r < 0 ? r + G : r;
+    };
+
+    std::vector<float> dxi(W), dyi(H);
+    for (int i = 0; i < W; ++i) dxi[i] = (float)(1.0 / kb_ft((double)(i - W / 2) / Gx));
+    for (int i = 0; i < H; ++i) dyi[i] = (float)(1.0 / kb_ft((double)(i - H / 2) / Gy));
+
+    std::vector<float> tr((size_t)T * 2 * M), dn((size_t)T * M, 1.f);
+    AV_HIP(hipMemcpy(tr.data(), traj, sizeof(float) * tr.size(), hipMemcpyDeviceToHost));
+    if (dens) AV_HIP(hipMemcpy(dn.data(), dens, sizeof(float) * dn.size(), hipMemcpyDeviceToHost));
+    const double sc = 1.0 / std::sqrt((double)W * H);
+
+    std::vector<float> sw((size_t)T * M);
+    std::vector<int2> q0((size_t)T * M);
+    std::vector<float4> wx4((size_t)T * M), wy4((size_t)T * M);
+    std::vector<int> rptr((size_t)T * (Gy + 1));
+    std::vector<int2> rlist;
+    std::vector<int> cptr((size_t)T * (G2 + 1));
+    std::vector<int> cm;
+    std::vector<float> cw;
+    rlist.reserve((size_t)T * M * kNcTaps);
+    for (int t = 0; t < T; ++t) {
+      std::vector<double> wxd((size_t)M * kNcTaps), wyd((size_t)M * kNcTaps);
+      std::vector<long> qx((size_t)M), qy((size_t)M);
+      for (long m = 0; m < M; ++m) {
+        const size_t p = (size_t)t * M + m;
+        const double d = dn[p];
+        AV_CHECK(std::isfinite(d) && d >= 0.0, "density weights must be finite and >= 0");
+        sw[p] = (float)(std::sqrt(d) * sc);
+        const double sx = (double)tr[((size_t)t * 2 + 0) * M + m] * Gx;
+        const double sy = (double)tr[((size_t)t * 2 + 1) * M + m] * Gy;
+        AV_CHECK(std::isfinite(sx) && std::isfinite(sy), "trajectory must be finite");
+        qx[m] = (long)std::ceil(sx - J / 2.0);
+        qy[m] = (long)std::ceil(sy - J / 2.0);
+        float w4x[4], w4y[4];
+        for (int j = 0; j < kNcTaps; ++j) {
+          wxd[m * kNcTaps + j] = kb(sx - (double)(qx[m] + j));
+          wyd[m * kNcTaps + j] = kb(sy - (double)(qy[m] + j));
+          w4x[j] = (float)wxd[m * kNcTaps + j];
+          w4y[j] = (float)wyd[m * kNcTaps + j];
+        }
+        q0[p] = make_int2((int)wrap(qx[m], Gx), (int)wrap(qy[m], Gy));
+        wx4[p] = make_float4(w4x[0], w4x[1], w4x[2], w4x[3]);
+        wy4[p] = make_float4(w4y[0], w4y[1], w4y[2], w4y[3]);
+      }
+      // row -> (point, tap): every point lists all kNcTaps rows (a zero
+      // weight row still writes its partial, so none is left stale)
+      std::vector<std::vector<int2>> rows(Gy);
+      for (long m = 0; m < M; ++m)
+        for (int dy = 0; dy < kNcTaps; ++dy)
+          rows[wrap(qy[m] + dy, Gy)].push_back(make_int2((int)m, dy));
+      int *rp = &rptr[(size_t)t * (Gy + 1)];
+      for (int r = 0; r < Gy; ++r) {
+        rp[r] = (int)rlist.size();
+        rlist.insert(rlist.end(), rows[r].begin(), rows[r].end());
+      }
+      rp[Gy] = (int)rlist.size();
+      // cell -> (point, weight) for the spreading, points in increasing order
+      std::vector<int> count(G2, 0);
+      for (long m = 0; m < M; ++m)
+        for (int dy = 0; dy < kNcTaps; ++dy)
+          for (int dx = 0; dx < kNcTaps; ++dx)
+            if (wyd[m * kNcTaps + dy] * wxd[m * kNcTaps + dx] != 0.0)
+              count[wrap(qy[m] + dy, Gy) * Gx + wrap(qx[m] + dx, Gx)]++;
+      int *cp = &cptr[(size_t)t * (G2 + 1)];
+      cp[0] = (int)cm.size();
+      for (long g = 0; g < G2; ++g) cp[g + 1] = cp[g] + count[g];
+      cm.resize(cp[G2]);
+      cw.resize(cp[G2]);
+      std::vector<int> fill(G2, 0);
+      for (long m = 0; m < M; ++m)
+        for (int dy = 0; dy < kNcTaps; ++dy)
+          for (int dx = 0; dx < kNcTaps; ++dx) {
+            const double w = wyd[m * kNcTaps + dy] * wxd[m * kNcTaps + dx];
+            if (w == 0.0) continue;
+            const long cell = wrap(qy[m] + dy, Gy) * Gx + wrap(qx[m] + dx, Gx);
+            const int pos = cp[cell] + fill[cell]++;
+            cm[pos] = (int)m;
+            cw[pos] = (float)(w * std::sqrt((double)dn[(size_t)t * M + m]) * sc);
+          }
+    }
+
+    auto up = [](DevBuf &b, const void *src, size_t bytes) {
+      b.ensure(std::max<size_t>(bytes, 16));
+      if (bytes) AV_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    up(dxi_, dxi.data(), sizeof(float) * dxi.size());
+    up(dyi_, dyi.data(), sizeof(float) * dyi.size());
+    up(sw_, sw.data(), sizeof(float) * sw.size());
+    up(q0_, q0.data(), sizeof(int2) * q0.size());
+    up(wx_, wx4.data(), sizeof(float4) * wx4.size());
+    up(wy_, wy4.data(), sizeof(float4) * wy4.size());
+    up(rptr_, rptr.data(), sizeof(int) * rptr.size());
+    up(rlist_, rlist.data(), sizeof(int2) * rlist.size());
+    up(cptr_, cptr.data(), sizeof(int) * cptr.size());
+    up(cm_, cm.data(), sizeof(int) * cm.size());
+    up(cw_, cw.data(), sizeof(float) * cw.size());
+    make_fft_plan(Gx, g_.px, &twx_.p);
+    make_fft_plan(Gy, g_.py, &twy_.p);
+
+    g_.W = W; g_.H = H; g_.C = C; g_.T = T; g_.M = (int)M;
+    g_.Gx = Gx; g_.Gy = Gy;
+    g_.XBf = pick_cols(Gy, W);
+    g_.XBa = pick_cols(Gy, Gx);
+    g_.RBf = std::max(1, 4096 / Gx);
+    g_.RBa = std::max(1, 4096 / Gx);
+    g_.scale = (float)sc;
+    g_.dxi = dxi_.as<float>();
+    g_.dyi = dyi_.as<float>();
+    g_.sw = sw_.as<float>();
+    g_.q0 = q0_.as<int2>();
+    g_.wx = wx_.as<float4>();
+    g_.wy = wy_.as<float4>();
+    g_.rptr = rptr_.as<int>();
+    g_.rlist = rlist_.as<int2>();
+    g_.cptr = cptr_.as<int>();
+    g_.cm = cm_.as<int>();
+    g_.cw = cw_.as<float>();
+  }
+
+  long dense_elems() const override { return (long)T * C * g_.M; }
+  long compact_elems() const override { return dense_elems(); }
+
+  // noncartesian_operator.cpp:88-95: lambda = k W / (spokesPerFrame pi / 2) + d
+  double adapt_lambda(double k, double d) const override {
+    return (double)W / (spf_ * M_PI / 2.0) * k + d;
+  }
+
+  void backward(const float2 *img, float2 *kd, const float2 *b1, hipStream_t s) override {
+    grid_pass(img, nullptr, b1, s);
+    launch_nc_samples(kNcApplyK, g_, part_.as<float2>(), kd, nullptr, nullptr, nullptr, nullptr, s);
+  }
+  void forward(const float2 *kd, float2 *img, const float2 *b1, hipStream_t s) override {
+    adjoint_compact(kd, b1, img, s);
+  }
+  void gather(const float2 *kd, float2 *compact, hipStream_t s) override {
+    AV_HIP(hipMemcpyAsync(compact, kd, sizeof(float2) * dense_elems(), hipMemcpyDeviceToDevice, s));
+  }
+  void dual_update(const float2 *u, const float2 *b1, float2 *z, const float2 *d, float2 *khz,
+                   const DevScalars *sc, hipStream_t s) override {
+    grid_pass(u, nullptr, b1, s);
+    launch_nc_samples(kNcDualUpdate, g_, part_.as<float2>(), nullptr, z, d, sc, nullptr, s);
+    adjoint_compact(z, b1, khz, s);
+  }
+  void adjoint_compact(const float2 *z, const float2 *b1, float2 *img, hipStream_t s) override {
+    scratch();
+    launch_nc_adjoint(g_, z, b1, tmp_.as<float2>(), img, s);
+  }
+  int norm2(const float2 *u, const float2 *usub, const float2 *b1, double *partial,
+            hipStream_t s) override {
+    grid_pass(u, usub, b1, s);
+    return launch_nc_samples(kNcNorm2, g_, part_.as<float2>(), nullptr, nullptr, nullptr, nullptr,
+                             partial, s);
+  }
+  int residual2(const float2 *u, const float2 *b1, const float2 *d, double *partial,
+                hipStream_t s) override {
+    grid_pass(u, nullptr, b1, s);
+    return launch_nc_samples(kNcResidual, g_, part_.as<float2>(), nullptr, nullptr, d, nullptr,
+                             partial, s);
+  }
+  int max_partials() const override { return nc_sample_blocks(g_); }
+
+ private:
+  int spf_;
+  NcGeom g_{};
+  DevBuf dxi_, dyi_, sw_, q0_, wx_, wy_, rptr_, rlist_, cptr_, cm_, cw_, twx_, twy_, tmp_, part_;
+
+  void scratch() {
+    const size_t planes = (size_t)T * C;
+    tmp_.ensure(sizeof(float2) * planes * std::max((size_t)g_.Gy * W, (size_t)H * g_.Gx));
+    part_.ensure(sizeof(float2) * planes * g_.M * kNcTaps);
+  }
+  void grid_pass(const float2 *u, const float2 *usub, const float2 *b1, hipStream_t s) {
+    scratch();
+    launch_nc_forward_grid(g_, u, usub, b1, tmp_.as<float2>(), part_.as<float2>(), s);
+  }
+};
+
+}  // namespace
+
+Operator *make_noncartesian(int W, int H, int C, int T, int nFE, int spf, const float *traj,
+                            const float *dens, int kw, double osf) {
+  return new NoncartesianOperator(W, H, C, T, nFE, spf, traj, dens, kw, osf);
+}
+
 }  // namespace avionic

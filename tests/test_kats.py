@@ -19,6 +19,4 @@ def test_kat_oracle(cpu, name):
 @pytest.mark.parametrize("name", K.NAMES)
 def test_kat_gpu(gpu, name):
     print(f"[gpu kat] {name}", flush=True)
-    if name == "noncart_adapt_lambda":
-        pytest.skip("non-Cartesian operator not built yet")
     _check(getattr(K, "kat_" + name)(gpu, K.KATS[name]))
