@@ -15,10 +15,10 @@
 namespace avionic {
 
 #ifndef AVIONIC_COL_THREADS
-#define AVIONIC_COL_THREADS 512
+#define AVIONIC_COL_THREADS 256
 #endif
 #ifndef AVIONIC_COL_UCACHE
-#define AVIONIC_COL_UCACHE 1
+#define AVIONIC_COL_UCACHE 0
 #endif
 
 constexpr int RTILE = kRowTileElems;
@@ -30,15 +30,25 @@ constexpr bool kColUcache = AVIONIC_COL_UCACHE != 0;  // keep the image tile in 
 #endif
 // adjoint coil sum in registers (else read-modify-write of the output per coil)
 constexpr bool kColAcc = AVIONIC_COL_ACC != 0;
+#ifndef AVIONIC_COL_PREFETCH
+#define AVIONIC_COL_PREFETCH 0
+#endif
+// load the next coil's b1 (forward) / zhat and b1 (adjoint) while this coil
+// is transformed
+constexpr bool kColPrefetch = AVIONIC_COL_PREFETCH != 0;
+constexpr int kPre = kColPrefetch ? kColSlots : 1;
 
 // ------------------------------------------------------------ column forward
 // 32-bit offsets from per-(t, c) base pointers (uniform, in SGPRs) keep one
-// VGPR per in-flight address.  The image tile is read on the first coil and
-// kept in registers (a lane loads the same elements for every coil).
+// VGPR per in-flight address.  The image tile is re-read per coil (an L2 hit)
+// by default: keeping it in registers costs occupancy, and on MI355X 256-lane
+// blocks without that cache were fastest (0.91 ms against 1.94 ms for
+// 512-lane blocks with it, ICTGV2 256x256x30x16, 8 reconstructions).
 template <bool DELTA>
 struct ColFwdLoad {
   float2 ucache[kColUcache ? kColSlots : 1];
-  const float2 *ut, *st, *bc;
+  float2 bpre[kPre];
+  const float2 *ut, *st, *bc, *bn;  // bn: next coil's map (prefetch), or null
   int H, W, xb0, centered;
   bool first;
   __device__ float2 operator()(int b, int i, int slot) {
@@ -47,9 +57,16 @@ struct ColFwdLoad {
     const int sy = centered ? (i + H / 2) % H : i;
     const int sx = centered ? (x + W / 2) % W : x;
     const unsigned si = (unsigned)(sy * W + sx);
-    if (!kColUcache) return cmul(bc[si], DELTA ? csub(ut[si], st[si]) : ut[si]);
+    float2 bv;
+    if (kColPrefetch) {
+      bv = first ? bc[si] : bpre[slot];
+      if (bn) bpre[slot] = bn[si];
+    } else {
+      bv = bc[si];
+    }
+    if (!kColUcache) return cmul(bv, DELTA ? csub(ut[si], st[si]) : ut[si]);
     if (first) ucache[slot] = DELTA ? csub(ut[si], st[si]) : ut[si];
-    return cmul(bc[si], ucache[slot]);
+    return cmul(bv, ucache[slot]);
   }
 };
 struct ColFwdStore {
@@ -84,6 +101,7 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
   ColFwdStore st{nullptr, g.rowmap + (long)t * g.H, g.H, g.W, xbi * g.XB, g.centered};
   for (int c = 0; c < g.C; ++c) {
     ld.bc = b1 + c * HW;
+    ld.bn = (c + 1 < g.C) ? b1 + (c + 1) * HW : nullptr;
     ld.first = (c == 0);
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
     fft_io<NTC, EMAX, true, true, true>(tile, g.ph, g.XB, 1, g.XB, +1.f, tid, tw, ld, st);
@@ -93,20 +111,28 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
 
 // ------------------------------------------------------------ column adjoint
 struct ColAdjLoad {
-  const float2 *zh;  // zhat rows of (t, c)
+  float2 zpre[kPre];
+  const float2 *zh, *zn;  // zhat rows of (t, c); of (t, c + 1) or null
   const int *rmap;
   int H, W, xb0, centered;
-  __device__ float2 operator()(int b, int i, int) const {
+  bool first;
+  __device__ float2 operator()(int b, int i, int slot) {
     const int x = xb0 + b;
     if (x >= W) return czero();
     const int ky = centered ? (i + H / 2) % H : i;
     const int k = rmap[ky];
-    return k >= 0 ? zh[(unsigned)(k * W + x)] : czero();
+    if (k < 0) return czero();
+    const unsigned o = (unsigned)(k * W + x);
+    if (!kColPrefetch) return zh[o];
+    const float2 v = first ? zh[o] : zpre[slot];
+    if (zn) zpre[slot] = zn[o];
+    return v;
   }
 };
 struct ColAdjStore {
   float2 acc[kColAcc ? kColSlots : 1];
-  const float2 *bc;
+  float2 bpre[kPre];
+  const float2 *bc, *bn;
   float2 *out;  // image frame t
   int H, W, xb0, centered;
   bool first, last;
@@ -116,12 +142,19 @@ struct ColAdjStore {
     if (x >= W) return;
     const int y = centered ? (i + H / 2) % H : i;
     const unsigned o = (unsigned)(y * W + x);
+    float2 bv;
+    if (kColPrefetch) {
+      bv = first ? bc[o] : bpre[slot];
+      if (bn) bpre[slot] = bn[o];
+    } else {
+      bv = bc[o];
+    }
     if (!kColAcc) {
-      const float2 a = cadd(first ? czero() : out[o], cmulc(bc[o], v));
+      const float2 a = cadd(first ? czero() : out[o], cmulc(bv, v));
       out[o] = last ? cscale(a, scale) : a;
       return;
     }
-    const float2 a = cadd(acc[slot], cmulc(bc[o], v));
+    const float2 a = cadd(acc[slot], cmulc(bv, v));
     if (last)
       out[o] = cscale(a, scale);
     else
@@ -137,7 +170,12 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   const float2 *tw = stage_twiddles(g.ph, twl, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColAdjLoad ld{nullptr, g.rowmap + (long)t * g.H, g.H, g.W, xbi * g.XB, g.centered};
+  ColAdjLoad ld;
+  ld.rmap = g.rowmap + (long)t * g.H;
+  ld.H = g.H;
+  ld.W = g.W;
+  ld.xb0 = xbi * g.XB;
+  ld.centered = g.centered;
   ColAdjStore st;
 #pragma unroll
   for (int k = 0; k < (kColAcc ? kColSlots : 1); ++k) st.acc[k] = czero();
@@ -149,7 +187,10 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   st.scale = g.scale;
   for (int c = 0; c < g.C; ++c) {
     ld.zh = zhat + ((long)g.C * r0 + (long)c * S) * g.W;
+    ld.zn = (c + 1 < g.C) ? ld.zh + (long)S * g.W : nullptr;
+    ld.first = (c == 0);
     st.bc = b1 + c * HW;
+    st.bn = (c + 1 < g.C) ? b1 + (c + 1) * HW : nullptr;
     st.first = (c == 0);
     st.last = (c == g.C - 1);
     fft_io<NTC, EMAX, true, true, true>(tile, g.ph, g.XB, 1, g.XB, -1.f, tid, tw, ld, st);

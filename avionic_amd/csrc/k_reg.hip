@@ -13,9 +13,9 @@ namespace avionic {
 template <int M, typename DT>
 __global__ __launch_bounds__(RT) void k_dual(Grid3 g, RegParams rp, const float2 *__restrict__ ext,
                                              DT *__restrict__ y, const DevScalars *__restrict__ sc) {
-  const long i = (long)blockIdx.x * RT + threadIdx.x;
-  if (i >= g.N) return;
-  dual_voxel<M, DT>(g, rp, ext, y, (float)sc->sigma, vox_of(g, i));
+  Vox v;
+  if (!vox_of_block<false>(g, blockIdx.x, threadIdx.x, v)) return;
+  dual_voxel<M, DT>(g, rp, ext, y, (float)sc->sigma, v);
 }
 
 template <int M, typename DT>
@@ -23,18 +23,18 @@ __global__ __launch_bounds__(RT) void k_primal(Grid3 g, RegParams rp, float2 *__
                                                float2 *__restrict__ ext, const DT *__restrict__ y,
                                                const float2 *__restrict__ khz,
                                                const DevScalars *__restrict__ sc) {
-  const long i = (long)blockIdx.x * RT + threadIdx.x;
-  if (i >= g.N) return;
-  primal_voxel<M, DT>(g, rp, x, ext, y, khz, (float)sc->tau, vox_of(g, i));
+  Vox v;
+  if (!vox_of_block<false>(g, blockIdx.x, threadIdx.x, v)) return;
+  primal_voxel<M, DT>(g, rp, x, ext, y, khz, (float)sc->tau, v);
 }
 
 template <int M>
 __global__ __launch_bounds__(RT) void k_step_reg(Grid3 g, RegParams rp, const float2 *__restrict__ ext,
                                                  const float2 *__restrict__ x,
                                                  double *__restrict__ partial) {
-  const long i = (long)blockIdx.x * RT + threadIdx.x;
+  Vox v;
   double ak = 0.0, ax = 0.0;
-  if (i < g.N) step_voxel<M>(g, rp, ext, x, vox_of(g, i), ak, ax);
+  if (vox_of_block<true>(g, blockIdx.x, threadIdx.x, v)) step_voxel<M>(g, rp, ext, x, v, ak, ax);
   block_reduce2(ak, ax, partial + 2 * blockIdx.x);
 }
 
@@ -238,7 +238,7 @@ static inline unsigned nblk(long n) { return (unsigned)((n + RT - 1) / RT); }
 
 void launch_dual(int method, bool bf, const Grid3 &g, const RegParams &rp, const float2 *ext, void *y,
                  const DevScalars *sc, hipStream_t s) {
-  dim3 grid(nblk(g.N)), block(RT);
+  dim3 grid((unsigned)step_blocks(g)), block(RT);
 #define AV_DUAL(M, DT) hipLaunchKernelGGL((k_dual<M, DT>), grid, block, 0, s, g, rp, ext, (DT *)y, sc)
   if (!bf) {
     if (method == kTV) AV_DUAL(kTV, float2);
@@ -255,7 +255,7 @@ void launch_dual(int method, bool bf, const Grid3 &g, const RegParams &rp, const
 
 void launch_primal(int method, bool bf, const Grid3 &g, const RegParams &rp, float2 *x, float2 *ext,
                    const void *y, const float2 *khz, const DevScalars *sc, hipStream_t s) {
-  dim3 grid(nblk(g.N)), block(RT);
+  dim3 grid((unsigned)step_blocks(g)), block(RT);
 #define AV_PRIMAL(M, DT) \
   hipLaunchKernelGGL((k_primal<M, DT>), grid, block, 0, s, g, rp, x, ext, (const DT *)y, khz, sc)
   if (!bf) {
@@ -271,11 +271,11 @@ void launch_primal(int method, bool bf, const Grid3 &g, const RegParams &rp, flo
   AV_LAUNCH_CHECK();
 }
 
-int step_blocks(const Grid3 &g) { return (int)nblk(g.N); }
+int step_blocks(const Grid3 &g) { return (int)(tiles_per_frame(g) * g.T); }
 
 void launch_step_reg(int method, const Grid3 &g, const RegParams &rp, const float2 *ext,
                      const float2 *x, double *partial, hipStream_t s) {
-  dim3 grid(nblk(g.N)), block(RT);
+  dim3 grid((unsigned)step_blocks(g)), block(RT);
   if (method == kTV) hipLaunchKernelGGL(k_step_reg<kTV>, grid, block, 0, s, g, rp, ext, x, partial);
   else if (method == kTGV2) hipLaunchKernelGGL(k_step_reg<kTGV2>, grid, block, 0, s, g, rp, ext, x, partial);
   else hipLaunchKernelGGL(k_step_reg<kICTGV2>, grid, block, 0, s, g, rp, ext, x, partial);

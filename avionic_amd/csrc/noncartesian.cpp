@@ -59,7 +59,9 @@ class NoncartesianOperator : public Operator {
     const long G2 = (long)Gx * Gy;
     AV_CHECK((long)T * M * kNcTaps * kNcTaps < (1L << 31), "trajectory tables exceed 2^31 entries");
 
-    const double beta = M_PI * std::sqrt(std::pow((double)J / osf * (osf - 0.5), 2) - 0.8);
+    const double b2 = std::pow((double)J / osf * (osf - 0.5), 2) - 0.8;
+    AV_CHECK(b2 > 0.0, "kernel width too small for this grid oversampling (beta undefined)");
+    const double beta = M_PI * std::sqrt(b2);
     auto kb = [&](double u) {
       const double r = 2.0 * u / J;
       return std::fabs(r) > 1.0 ? 0.0 : bessel_i0(beta * std::sqrt(1.0 - r * r));

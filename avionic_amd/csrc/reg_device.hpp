@@ -28,6 +28,31 @@ __device__ inline Vox vox_of(const Grid3 &g, long i) {
   return v;
 }
 
+// Voxel blocks: each frame is cut into tiles of RT consecutive voxels.
+// Frame-major order (tile lb % tiles of frame lb / tiles) streams memory
+// linearly; the interleaved order (tile lb / T of frame lb % T) has
+// consecutive blocks work on the same tile of consecutive frames, so the
+// t-neighbours one block reads are the centre voxels the next block reads.
+// Measured on MI355X: the dual and primal updates are faster frame-major, the
+// step-rule reduction (which reads two primal states) interleaved.
+// Returns false past the end of the frame.
+__host__ __device__ inline long tiles_per_frame(const Grid3 &g) {
+  return ((long)g.W * g.H + RT - 1) / RT;
+}
+template <bool INTERLEAVE>
+__device__ inline bool vox_of_block(const Grid3 &g, long lb, int tid, Vox &v) {
+  const long HW = (long)g.W * g.H;
+  const long tiles = (HW + RT - 1) / RT;
+  const int t = (int)(INTERLEAVE ? lb % g.T : lb / tiles);
+  const long p = (INTERLEAVE ? lb / g.T : lb % tiles) * RT + tid;
+  if (p >= HW) return false;
+  v.t = t;
+  v.y = (int)(p / g.W);
+  v.x = (int)(p % g.W);
+  v.i = (long)t * HW + p;
+  return true;
+}
+
 struct PlainAcc {
   const float2 *p;
   __device__ float2 operator()(long j) const { return p[j]; }

@@ -181,3 +181,38 @@ def make(wl: Workload | str, with_truth=False):
     if with_truth:
         out["truth"] = rho
     return out
+
+
+# ------------------------------------------------------------- serving stream
+SERVE_SIZES = (128, 160, 192, 224, 256)
+# acceleration / centre lines / shifted pattern per method, as in the three
+# single-reconstruction workloads above
+SERVE_SAMPLING = {"tv": (8, 12, False), "tgv2": (10, 16, False), "ictgv2": (12, 16, True)}
+
+
+def serving_stream(rate=50.0, seconds=2.0, seed=4):
+    """Poisson arrivals at `rate` per second over `seconds`: a list of
+    (arrival_s, method, n, frames, coils), method uniform over TV / TGV2 /
+    ICTGV2, n uniform over SERVE_SIZES, frames uniform 8..30, coils 8..20."""
+    rng = np.random.default_rng(seed + 5000)
+    out, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return out
+        out.append((t, str(rng.choice(["tv", "tgv2", "ictgv2"])), int(rng.choice(SERVE_SIZES)),
+                    int(rng.integers(8, 31)), int(rng.integers(8, 21))))
+
+
+def serving_pool(seed=4, frames=30, coils=20):
+    """Per matrix size: phantom [frames, n, n] (10 ellipses), coil maps
+    [coils, n, n] and, per method, the line mask [frames, n, n] (uncentred).
+    Requests slice the first T frames and C coils."""
+    pool = {}
+    for n in SERVE_SIZES:
+        rho = phantom(n, frames, 10, seed + n)
+        b1 = coil_maps(n, coils, seed + n)
+        masks = {m: np.fft.ifftshift(vd_line_mask(n, frames, a, c, seed + n, s), axes=(1, 2))
+                 for m, (a, c, s) in SERVE_SAMPLING.items()}
+        pool[n] = dict(rho=rho, b1=b1, masks=masks)
+    return pool

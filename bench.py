@@ -54,6 +54,75 @@ def algorithmic_bytes_per_voxel(kernel, method, bf16):
     return None
 
 
+def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500):
+    """Continuous batching under Poisson arrivals: mixed TV / TGV2 / ICTGV2
+    Cartesian requests (n in 128..256, T 8..30, C 8..20, 500 iterations each).
+    Request inputs are built on the device before the clock starts; a
+    request's latency runs from its scheduled arrival to the moment its
+    result has been copied out."""
+    stream = synth.serving_stream(rate, seconds)
+    pool = synth.serving_pool()
+    reqs = []
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11)
+    for (t_arr, method, n, T, C) in stream:
+        P = pool[n]
+        rho = torch.from_numpy(np.ascontiguousarray(P["rho"][:T], np.complex64)).to(dev)
+        b1 = torch.from_numpy(np.ascontiguousarray(P["b1"][:C], np.complex64)).to(dev)
+        mask = torch.from_numpy(np.ascontiguousarray(P["masks"][method][:T], np.float32)).to(dev)
+        op = A.CartesianOperator(n, n, C, T, mask=mask, centered=False)
+        data = op.BackwardOperation(rho, b1)
+        sig = 0.01 * float(data.abs().max().item())
+        noise = torch.randn(data.shape, dtype=torch.complex64, device=dev, generator=gen)
+        data = data + sig * noise * mask[:, None]
+        x0 = op.ForwardOperation(data, b1)
+        m = {"tv": 0, "tgv2": 1, "ictgv2": 2}[method]
+        reqs.append(dict(t=t_arr, method=method, n=n, T=T, C=C,
+                         req=A.Request(m, data, b1, mask=mask, x0=x0,
+                                       params={"maxIt": iterations})))
+        del op
+    torch.cuda.synchronize()
+    eng = A.Engine(use_graphs=True)
+    outs = {}
+    lat, waiting, running = [], list(range(len(reqs))), {}
+    t0 = time.perf_counter()
+    steps = 0
+    while waiting or running:
+        now = time.perf_counter() - t0
+        while waiting and reqs[waiting[0]]["t"] <= now:
+            i = waiting[0]
+            rid = eng.submit(reqs[i]["req"])
+            if rid is None:
+                break  # HBM budget full: retry after completions
+            running[rid] = i
+            waiting.pop(0)
+        if not running:
+            time.sleep(max(0.0, reqs[waiting[0]]["t"] - (time.perf_counter() - t0)))
+            continue
+        eng.step()
+        steps += 1
+        for rid, i in list(running.items()):
+            r = reqs[i]
+            if i not in outs:
+                outs[i] = torch.empty((r["T"], r["n"], r["n"]), dtype=torch.complex64, device=dev)
+            if eng.poll(rid, outs[i]):
+                lat.append(time.perf_counter() - t0 - r["t"])
+                del running[rid]
+    makespan = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+    total_its = iterations * len(reqs)
+    vox_its = sum(r["n"] * r["n"] * r["T"] for r in reqs) * iterations
+    finite = all(bool(torch.isfinite(torch.view_as_real(o)).all()) for o in outs.values())
+    return {"requests": len(reqs), "rate_per_s": rate, "window_s": seconds,
+            "iterations_each": iterations, "engine_steps": steps,
+            "makespan_s": round(makespan, 3),
+            "p50_latency_ms": round(float(np.percentile(lat_ms, 50)), 1),
+            "p99_latency_ms": round(float(np.percentile(lat_ms, 99)), 1),
+            "iters_per_s": round(total_its / makespan, 1),
+            "voxel_iters_per_s": round(vox_its / makespan, 1),
+            "all_finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +133,9 @@ def main():
     ap.add_argument("--workload", default="ictgv2_256x30x16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="arrival window of the serving stream (0: skip the serving leg)")
+    ap.add_argument("--serve-rate", type=float, default=50.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -173,6 +245,11 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(ktimes[dom], 4)}
 
+    serving = None
+    if args.serve_seconds > 0:
+        log(f"serving stream ({args.serve_rate}/s for {args.serve_seconds}s)")
+        serving = serving_leg(A, synth, torch, dev, args.serve_rate, args.serve_seconds)
+
     # ---- CPU baseline: FP64 oracle on the host cores, a few iterations
     cpu = None
     if not args.no_cpu:
@@ -214,6 +291,7 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
 

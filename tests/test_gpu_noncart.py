@@ -38,7 +38,7 @@ CASES = [
     (32, 32, 3, 2, 48, 7, 3, 1.5),
     (40, 24, 2, 3, 64, 5, 3, 1.5),
     (30, 30, 2, 2, 40, 6, 2, 2.0),
-    (16, 16, 1, 1, 16, 3, 1, 2.0),
+    (16, 16, 1, 1, 16, 3, 2, 1.25),
     (64, 64, 4, 2, 128, 9, 3, 1.25),
     (256, 256, 2, 2, 512, 21, 3, 1.5),
 ]
@@ -90,6 +90,8 @@ def test_invalid_geometry_rejected(gpu):
     tr, dcf = radial(32, 5, 1)
     with pytest.raises(Exception):
         make_op(gpu, 32, 32, 1, 1, 32, 5, tr, dcf, 4, 1.5)  # kernel width above 3
+    with pytest.raises(Exception):
+        make_op(gpu, 32, 32, 1, 1, 32, 5, tr, dcf, 1, 2.0)  # beta^2 < 0 for J = 1
     with pytest.raises(Exception):
         make_op(gpu, 400, 400, 1, 1, 32, 5, tr, dcf, 3, 1.5)  # grid 600 > 512
 
