@@ -171,7 +171,8 @@ int avionic_recon_get_params(const avionic_recon *recon, avionic_params *params)
 int avionic_recon_set_params(avionic_recon *recon, const avionic_params *params) {
   return guard([&] {
     NEED(recon); NEED(params);
-    AV_CHECK(params->dualStorage == 0 || params->dualStorage == 1, "dualStorage must be 0 or 1");
+    AV_CHECK(params->dualStorage >= 0 && params->dualStorage <= 2,
+             "dualStorage must be 0, 1 or 2");
     recon->r->params = *params;
   });
 }
@@ -317,9 +318,9 @@ static double norm_via(int method, const float2 *x, unsigned W, unsigned H, unsi
   const Grid3 g{(int)W, (int)H, (int)T, N};
   const RegParams rp = make_reg_params(method, d6, a0, a1, alpha);
   DevBuf part;
-  part.ensure(sizeof(double) * 2 * (size_t)step_blocks(g));
+  part.ensure(sizeof(double) * 2 * (size_t)norm_blocks(g));
   launch_reg_norm(method, g, rp, a0, a1, alpha, x, part.as<double>(), s);
-  return reduce_host(part.as<double>(), step_blocks(g), 2, 0, s);
+  return reduce_host(part.as<double>(), norm_blocks(g), 2, 0, s);
 }
 
 int avionic_tv_norm(const float *u, unsigned width, unsigned height, unsigned frames, float dx,

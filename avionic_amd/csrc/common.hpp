@@ -62,21 +62,56 @@ __host__ __device__ inline uint16_t f32_to_bf16(float f) {
 }
 __host__ __device__ inline float bf16_to_f32(uint16_t h) { return u2f(((uint32_t)h) << 16); }
 
+// 16-bit fixed-point complex pair scaled by a bound r: v = q * r / 32767.
+// After the norm_3 / norm_6 projections every dual component satisfies
+// |re|, |im| <= r (the projection radius), so the dual variables fit this
+// format with an absolute resolution of r / 32767.
+struct q16x2 {
+  int16_t re, im;
+};
+constexpr float kQ16Max = 32767.f;
+
+__host__ __device__ inline int16_t f32_to_q16(float v, float inv_scale) {
+  float q = rintf(v * inv_scale);
+  q = fminf(fmaxf(q, -kQ16Max), kQ16Max);
+  return (int16_t)q;
+}
+
+// Dual storage kinds (avionic_params::dualStorage).
+enum DualStorage { kDualF32 = 0, kDualBF16 = 1, kDualQ16 = 2 };
+inline size_t dual_bytes(int storage) { return storage == kDualF32 ? 8 : 4; }
+
+// Loads and stores of one complex dual value; r is the projection radius of
+// its component group (used by the fixed-point format only).
 template <typename DT>
 struct DualIO;
 template <>
 struct DualIO<float2> {
-  __device__ static inline float2 load(const float2 *p, long i) { return p[i]; }
-  __device__ static inline void store(float2 *p, long i, float2 v) { p[i] = v; }
+  __device__ static inline float2 load(const float2 *p, long i, float) { return p[i]; }
+  __device__ static inline void store(float2 *p, long i, float2 v, float) { p[i] = v; }
 };
 template <>
 struct DualIO<bf16x2> {
-  __device__ static inline float2 load(const bf16x2 *p, long i) {
+  __device__ static inline float2 load(const bf16x2 *p, long i, float) {
     uint32_t w = *reinterpret_cast<const uint32_t *>(p + i);
     return make_float2(u2f(w << 16), u2f(w & 0xffff0000u));
   }
-  __device__ static inline void store(bf16x2 *p, long i, float2 v) {
+  __device__ static inline void store(bf16x2 *p, long i, float2 v, float) {
     uint32_t w = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
+    *reinterpret_cast<uint32_t *>(p + i) = w;
+  }
+};
+template <>
+struct DualIO<q16x2> {
+  __device__ static inline float2 load(const q16x2 *p, long i, float r) {
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(p + i);
+    const float s = r * (1.f / kQ16Max);
+    return make_float2((float)(int16_t)(w & 0xffffu) * s, (float)(int16_t)(w >> 16) * s);
+  }
+  __device__ static inline void store(q16x2 *p, long i, float2 v, float r) {
+    const float inv = kQ16Max / r;
+    const uint32_t w = (uint32_t)(uint16_t)f32_to_q16(v.x, inv) |
+                       ((uint32_t)(uint16_t)f32_to_q16(v.y, inv) << 16);
     *reinterpret_cast<uint32_t *>(p + i) = w;
   }
 };

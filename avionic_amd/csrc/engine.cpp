@@ -28,7 +28,7 @@ namespace {
 struct Slot {
   long long id = 0;
   int method = 0;
-  bool bf = false;
+  int ds = 0;  // dual storage
   int W = 0, H = 0, C = 0, T = 0;
   long N = 0, K = 0;
   std::unique_ptr<CartesianOperator> op;
@@ -44,11 +44,11 @@ struct Slot {
   }
 };
 
-size_t slot_bytes(int method, bool bf, long N, long K, long CHW, int maxit) {
+size_t slot_bytes(int method, int ds, long N, long K, long CHW, int maxit) {
   const int P = primal_components(method), D = dual_components(method);
   size_t b = 0;
   b += 2 * sizeof(float2) * (size_t)P * N;                        // x, ext
-  b += (bf ? sizeof(bf16x2) : sizeof(float2)) * (size_t)D * N;    // y
+  b += dual_bytes(ds) * (size_t)D * N;                            // y
   b += 4 * sizeof(float2) * (size_t)std::max(1L, K);              // z, d, kcol, zhat
   b += sizeof(float2) * (size_t)N;                                // khz
   b += sizeof(float2) * (size_t)CHW;                              // b1 copy
@@ -59,7 +59,7 @@ size_t slot_bytes(int method, bool bf, long N, long K, long CHW, int maxit) {
 
 struct Group {
   int method = 0;
-  bool bf = false;
+  int ds = 0;  // dual storage
   std::vector<Slot *> slots;  // active (not done)
   bool dirty = true;
   DevBuf table;
@@ -82,7 +82,7 @@ struct avionic_engine {
   long long next_id = 1;
   size_t used = 0;
   std::map<long long, std::unique_ptr<Slot>> slots;
-  std::map<int, Group> groups;  // key = method * 2 + bf
+  std::map<int, Group> groups;  // key = method * 3 + dual storage
 
   ~avionic_engine() {
     for (auto &kv : groups) kv.second.drop_graphs();
@@ -122,14 +122,14 @@ struct avionic_engine {
   void run_iteration(Group &g, bool with_step) {
     const int n = (int)g.slots.size();
     if (!cfg.use_graphs) {
-      launch_iteration(g.method, g.bf, g.table.as<Task>(), n, g.grid, with_step, stream);
+      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, stream);
       return;
     }
     hipGraphExec_t &ex = with_step ? g.g_step : g.g_plain;
     if (!ex) {
       hipGraph_t graph = nullptr;
       AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      launch_iteration(g.method, g.bf, g.table.as<Task>(), n, g.grid, with_step, stream);
+      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, stream);
       AV_HIP(hipStreamEndCapture(stream, &graph));
       AV_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);
@@ -178,7 +178,7 @@ avionic_params request_params(const avionic_request_desc *d, const Operator &op)
     p.dt2 = dt;
   }
   AV_CHECK(p.lambda > 0.0, "lambda must be positive");
-  AV_CHECK(p.dualStorage == 0 || p.dualStorage == 1, "dualStorage must be 0 or 1");
+  AV_CHECK(p.dualStorage >= 0 && p.dualStorage <= 2, "dualStorage must be 0, 1 or 2");
   return p;
 }
 
@@ -226,8 +226,8 @@ size_t avionic_request_footprint(const avionic_request_desc *d) {
               nullptr);
     const long N = (long)d->width * d->height * d->frames;
     const int maxit = d->params ? (int)d->params->maxIt : 500;
-    const bool bf = d->params && d->params->dualStorage == 1;
-    return slot_bytes((int)d->method, bf, N, plan.kspace_elems(),
+    const int ds = d->params ? d->params->dualStorage : 0;
+    return slot_bytes((int)d->method, ds, N, plan.kspace_elems(),
                       (long)d->coils * d->height * d->width, maxit);
   } catch (...) {
     return 0;
@@ -245,9 +245,9 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->op = std::make_unique<CartesianOperator>(s->W, s->H, s->C, s->T, d->mask, d->centered);
     s->K = s->op->compact_elems();
     s->params = request_params(d, *s->op);
-    s->bf = s->params.dualStorage == 1;
+    s->ds = s->params.dualStorage;
     s->maxit = (int)s->params.maxIt;
-    s->bytes = slot_bytes(s->method, s->bf, s->N, s->K, (long)s->C * s->H * s->W, s->maxit);
+    s->bytes = slot_bytes(s->method, s->ds, s->N, s->K, (long)s->C * s->H * s->W, s->maxit);
     if (e->cfg.hbm_budget_bytes && e->used + s->bytes > e->cfg.hbm_budget_bytes)
       throw Error(AVIONIC_E_CAPACITY, "request does not fit the HBM budget now");
     int resident = 0;
@@ -263,7 +263,7 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     const long N = s->N, K = std::max(1L, s->K);
     s->x.ensure(sizeof(float2) * P * N);
     s->ext.ensure(sizeof(float2) * P * N);
-    s->y.ensure((s->bf ? sizeof(bf16x2) : sizeof(float2)) * D * N);
+    s->y.ensure(dual_bytes(s->ds) * D * N);
     s->z.ensure(sizeof(float2) * K);
     s->d.ensure(sizeof(float2) * K);
     s->kcol.ensure(sizeof(float2) * K);
@@ -325,9 +325,9 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->id = e->next_id++;
     *id = s->id;
     e->used += s->bytes;
-    Group &grp = e->groups[s->method * 2 + (s->bf ? 1 : 0)];
+    Group &grp = e->groups[s->method * 3 + s->ds];
     grp.method = s->method;
-    grp.bf = s->bf;
+    grp.ds = s->ds;
     grp.slots.push_back(s.get());
     grp.dirty = true;
     if (s->maxit == 0) s->done = true;
@@ -458,11 +458,11 @@ int avionic_engine_time_kernels(avionic_engine *e, int reps, double *ms, const c
     AV_HIP(hipEventCreate(&a));
     AV_HIP(hipEventCreate(&b));
     for (size_t k = 0; k < m; ++k) {
-      launch_kernel((int)k, g->method, g->bf, g->table.as<Task>(), (int)g->slots.size(), g->grid,
+      launch_kernel((int)k, g->method, g->ds, g->table.as<Task>(), (int)g->slots.size(), g->grid,
                     e->stream);  // warm
       AV_HIP(hipEventRecord(a, e->stream));
       for (int r = 0; r < reps; ++r)
-        launch_kernel((int)k, g->method, g->bf, g->table.as<Task>(), (int)g->slots.size(), g->grid,
+        launch_kernel((int)k, g->method, g->ds, g->table.as<Task>(), (int)g->slots.size(), g->grid,
                       e->stream);
       AV_HIP(hipEventRecord(b, e->stream));
       AV_HIP(hipEventSynchronize(b));

@@ -236,42 +236,45 @@ RegParams make_reg_params(int method, const double d6[6], double alpha0, double 
 
 static inline unsigned nblk(long n) { return (unsigned)((n + RT - 1) / RT); }
 
-void launch_dual(int method, bool bf, const Grid3 &g, const RegParams &rp, const float2 *ext, void *y,
+void launch_dual(int method, int ds, const Grid3 &g, const RegParams &rp, const float2 *ext, void *y,
                  const DevScalars *sc, hipStream_t s) {
   dim3 grid((unsigned)step_blocks(g)), block(RT);
 #define AV_DUAL(M, DT) hipLaunchKernelGGL((k_dual<M, DT>), grid, block, 0, s, g, rp, ext, (DT *)y, sc)
-  if (!bf) {
-    if (method == kTV) AV_DUAL(kTV, float2);
-    else if (method == kTGV2) AV_DUAL(kTGV2, float2);
-    else AV_DUAL(kICTGV2, float2);
-  } else {
-    if (method == kTV) AV_DUAL(kTV, bf16x2);
-    else if (method == kTGV2) AV_DUAL(kTGV2, bf16x2);
-    else AV_DUAL(kICTGV2, bf16x2);
-  }
+#define AV_DUAL_M(DT)                        \
+  do {                                       \
+    if (method == kTV) AV_DUAL(kTV, DT);     \
+    else if (method == kTGV2) AV_DUAL(kTGV2, DT); \
+    else AV_DUAL(kICTGV2, DT);               \
+  } while (0)
+  if (ds == kDualF32) AV_DUAL_M(float2);
+  else if (ds == kDualBF16) AV_DUAL_M(bf16x2);
+  else AV_DUAL_M(q16x2);
+#undef AV_DUAL_M
 #undef AV_DUAL
   AV_LAUNCH_CHECK();
 }
 
-void launch_primal(int method, bool bf, const Grid3 &g, const RegParams &rp, float2 *x, float2 *ext,
+void launch_primal(int method, int ds, const Grid3 &g, const RegParams &rp, float2 *x, float2 *ext,
                    const void *y, const float2 *khz, const DevScalars *sc, hipStream_t s) {
   dim3 grid((unsigned)step_blocks(g)), block(RT);
 #define AV_PRIMAL(M, DT) \
   hipLaunchKernelGGL((k_primal<M, DT>), grid, block, 0, s, g, rp, x, ext, (const DT *)y, khz, sc)
-  if (!bf) {
-    if (method == kTV) AV_PRIMAL(kTV, float2);
-    else if (method == kTGV2) AV_PRIMAL(kTGV2, float2);
-    else AV_PRIMAL(kICTGV2, float2);
-  } else {
-    if (method == kTV) AV_PRIMAL(kTV, bf16x2);
-    else if (method == kTGV2) AV_PRIMAL(kTGV2, bf16x2);
-    else AV_PRIMAL(kICTGV2, bf16x2);
-  }
+#define AV_PRIMAL_M(DT)                          \
+  do {                                           \
+    if (method == kTV) AV_PRIMAL(kTV, DT);       \
+    else if (method == kTGV2) AV_PRIMAL(kTGV2, DT); \
+    else AV_PRIMAL(kICTGV2, DT);                 \
+  } while (0)
+  if (ds == kDualF32) AV_PRIMAL_M(float2);
+  else if (ds == kDualBF16) AV_PRIMAL_M(bf16x2);
+  else AV_PRIMAL_M(q16x2);
+#undef AV_PRIMAL_M
 #undef AV_PRIMAL
   AV_LAUNCH_CHECK();
 }
 
 int step_blocks(const Grid3 &g) { return (int)(tiles_per_frame(g) * g.T); }
+int norm_blocks(const Grid3 &g) { return (int)nblk(g.N); }
 
 void launch_step_reg(int method, const Grid3 &g, const RegParams &rp, const float2 *ext,
                      const float2 *x, double *partial, hipStream_t s) {

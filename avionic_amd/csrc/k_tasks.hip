@@ -168,20 +168,24 @@ const char *kernel_name(int id) {
   return (id >= 0 && id < kNumKernels) ? names[id] : "?";
 }
 
-void launch_kernel(int id, int method, bool bf, const Task *d_tasks, int n, const TaskGrid &gr,
+void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
                    hipStream_t s) {
   if (n <= 0) return;
   const dim3 bRT(RT), bNT(NT), bNC(NTC);
 #define AV_BYMETHOD2(KER, ...)                                                       \
   do {                                                                               \
-    if (!bf) {                                                                       \
+    if (ds == kDualF32) {                                                            \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
       else hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);                   \
-    } else {                                                                         \
+    } else if (ds == kDualBF16) {                                                    \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
       else hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);                   \
+    } else {                                                                         \
+      if (method == kTV) hipLaunchKernelGGL((KER<kTV, q16x2>), __VA_ARGS__);          \
+      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, q16x2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);                    \
     }                                                                                \
   } while (0)
   switch (id) {
@@ -206,20 +210,24 @@ void launch_kernel(int id, int method, bool bf, const Task *d_tasks, int n, cons
   AV_LAUNCH_CHECK();
 }
 
-void launch_iteration(int method, bool bf, const Task *d_tasks, int n, const TaskGrid &gr,
+void launch_iteration(int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
                       bool with_step, hipStream_t s) {
   if (n <= 0) return;
   const dim3 bRT(RT), bNT(NT), bNC(NTC);
 #define AV_BYMETHOD(KER, ...)                                                        \
   do {                                                                               \
-    if (!bf) {                                                                       \
+    if (ds == kDualF32) {                                                            \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
       else hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);                   \
-    } else {                                                                         \
+    } else if (ds == kDualBF16) {                                                    \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
       else hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);                   \
+    } else {                                                                         \
+      if (method == kTV) hipLaunchKernelGGL((KER<kTV, q16x2>), __VA_ARGS__);          \
+      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, q16x2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);                    \
     }                                                                                \
   } while (0)
   if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n);

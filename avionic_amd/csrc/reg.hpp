@@ -29,16 +29,19 @@ struct RegParams {
 RegParams make_reg_params(int method, const double d6[6], double alpha0, double alpha1,
                           double alpha);
 
-// y += sigma K_reg(ext); projections.  dual_bf16: duals stored as bf16 pairs.
-void launch_dual(int method, bool dual_bf16, const Grid3 &g, const RegParams &rp,
+// y += sigma K_reg(ext); projections.  ds: dual storage (DualStorage).
+void launch_dual(int method, int ds, const Grid3 &g, const RegParams &rp,
                  const float2 *ext, void *y, const DevScalars *sc, hipStream_t s);
 // x+ = x - tau (K_reg^T y + [khz, 0..]); ext = 2 x+ - x; x = x+.
-void launch_primal(int method, bool dual_bf16, const Grid3 &g, const RegParams &rp, float2 *x,
+void launch_primal(int method, int ds, const Grid3 &g, const RegParams &rp, float2 *x,
                    float2 *ext, const void *y, const float2 *khz, const DevScalars *sc,
                    hipStream_t s);
 // Step-rule partial sums over the primal difference ext - x:
 // partial[2b] = weighted sum |K_reg delta|^2, partial[2b+1] = sum |delta|^2.
 int step_blocks(const Grid3 &g);
+// Partial sums written by launch_reg_norm / launch_gstar_terms (one per
+// RT consecutive voxels).
+int norm_blocks(const Grid3 &g);
 void launch_step_reg(int method, const Grid3 &g, const RegParams &rp, const float2 *ext,
                      const float2 *x, double *partial, hipStream_t s);
 // Final deterministic reduction + sigma/tau rule (pd_recon.cpp:40-61).

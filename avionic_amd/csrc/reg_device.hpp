@@ -60,7 +60,8 @@ struct PlainAcc {
 template <typename DT>
 struct DualAcc {
   const DT *p;
-  __device__ float2 operator()(long j) const { return DualIO<DT>::load(p, j); }
+  float r;  // projection radius of the component group read through it
+  __device__ float2 operator()(long j) const { return DualIO<DT>::load(p, j, r); }
 };
 // a[j] - b[j]
 struct DiffAcc {
@@ -203,10 +204,10 @@ __device__ inline void dual_voxel(const Grid3 &g, const RegParams &rp, const flo
   if (M == kTV) {
     grad3(E, 0, v, g, d1, gr);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[c] = cadd(DualIO<DT>::load(y, c * N + i), cscale(gr[c], sig));
+    for (int c = 0; c < 3; ++c) p[c] = cadd(DualIO<DT>::load(y, c * N + i, rp.r1), cscale(gr[c], sig));
     proj3(p, rp.r1);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, c * N + i, p[c]);
+    for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, c * N + i, p[c], rp.r1);
     return;
   }
   if (M == kTGV2) {
@@ -218,33 +219,33 @@ __device__ inline void dual_voxel(const Grid3 &g, const RegParams &rp, const flo
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const float2 t = csub(gr[c], ext[(1 + c) * N + i]);
-    p[c] = cadd(DualIO<DT>::load(y, c * N + i), cscale(t, sig));
+    p[c] = cadd(DualIO<DT>::load(y, c * N + i, rp.r1), cscale(t, sig));
   }
   proj3(p, rp.r1);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, c * N + i, p[c]);
+  for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, c * N + i, p[c], rp.r1);
   symgrad6(E, N, 2 * N, 3 * N, v, g, d1, e);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) p[k] = cadd(DualIO<DT>::load(y, (3 + k) * N + i), cscale(e[k], sig));
+  for (int k = 0; k < 6; ++k) p[k] = cadd(DualIO<DT>::load(y, (3 + k) * N + i, rp.r2), cscale(e[k], sig));
   proj6(p, rp.r2);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) DualIO<DT>::store(y, (3 + k) * N + i, p[k]);
+  for (int k = 0; k < 6; ++k) DualIO<DT>::store(y, (3 + k) * N + i, p[k], rp.r2);
   if (M == kICTGV2) {
     grad3(E, 4 * N, v, g, d2, gr);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float2 t = csub(gr[c], ext[(5 + c) * N + i]);
-      p[c] = cadd(DualIO<DT>::load(y, (9 + c) * N + i), cscale(t, sig));
+      p[c] = cadd(DualIO<DT>::load(y, (9 + c) * N + i, rp.r3), cscale(t, sig));
     }
     proj3(p, rp.r3);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, (9 + c) * N + i, p[c]);
+    for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, (9 + c) * N + i, p[c], rp.r3);
     symgrad6(E, 5 * N, 6 * N, 7 * N, v, g, d2, e);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) p[k] = cadd(DualIO<DT>::load(y, (12 + k) * N + i), cscale(e[k], sig));
+    for (int k = 0; k < 6; ++k) p[k] = cadd(DualIO<DT>::load(y, (12 + k) * N + i, rp.r4), cscale(e[k], sig));
     proj6(p, rp.r4);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) DualIO<DT>::store(y, (12 + k) * N + i, p[k]);
+    for (int k = 0; k < 6; ++k) DualIO<DT>::store(y, (12 + k) * N + i, p[k], rp.r4);
   }
 }
 
@@ -256,25 +257,25 @@ __device__ inline void primal_voxel(const Grid3 &g, const RegParams &rp, float2 
                                     const float2 *__restrict__ khz, float tau, const Vox &v) {
   const long N = g.N, i = v.i;
   const Inv3 d1{rp.idx, rp.idy, rp.idt}, d2{rp.idx2, rp.idy2, rp.idt2};
-  DualAcc<DT> Y{y};
+  const DualAcc<DT> Y1{y, rp.r1}, Y2{y, rp.r2}, Y3{y, rp.r3}, Y4{y, rp.r4};
   float2 upd[8];
-  const float2 dv1 = div3(Y, 0, N, 2 * N, v, g, d1);
+  const float2 dv1 = div3(Y1, 0, N, 2 * N, v, g, d1);
   upd[0] = csub(x[i], cscale(csub(khz[i], dv1), tau));
   int P = 1;
   if (M != kTV) {
     float2 sd[3];
-    symdiv3(Y, 3 * N, N, v, g, d1, sd);
+    symdiv3(Y2, 3 * N, N, v, g, d1, sd);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      upd[1 + c] = cadd(x[(1 + c) * N + i], cscale(cadd(Y(c * N + i), sd[c]), tau));
+      upd[1 + c] = cadd(x[(1 + c) * N + i], cscale(cadd(Y1(c * N + i), sd[c]), tau));
     P = 4;
     if (M == kICTGV2) {
-      const float2 dv3 = div3(Y, 9 * N, 10 * N, 11 * N, v, g, d2);
+      const float2 dv3 = div3(Y3, 9 * N, 10 * N, 11 * N, v, g, d2);
       upd[4] = csub(x[4 * N + i], cscale(csub(dv1, dv3), tau));
-      symdiv3(Y, 12 * N, N, v, g, d2, sd);
+      symdiv3(Y4, 12 * N, N, v, g, d2, sd);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        upd[5 + c] = cadd(x[(5 + c) * N + i], cscale(cadd(Y((9 + c) * N + i), sd[c]), tau));
+        upd[5 + c] = cadd(x[(5 + c) * N + i], cscale(cadd(Y3((9 + c) * N + i), sd[c]), tau));
       P = 8;
     }
   }

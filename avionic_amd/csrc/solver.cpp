@@ -229,7 +229,7 @@ double Recon::regularizer_norm(const float2 *x, hipStream_t s) {
   const Grid3 g{W, H, T, N};
   launch_reg_norm(method, g, rp, params.alpha0, params.alpha1, params.alpha, x,
                   part_a_.as<double>(), s);
-  return reduce_pair(part_a_.as<double>(), step_blocks(g), 0, s);
+  return reduce_pair(part_a_.as<double>(), norm_blocks(g), 0, s);
 }
 
 // G* on caller-provided dense arrays (tv.cpp:202-225, tgv2.cpp:130-165,
@@ -266,7 +266,7 @@ double Recon::gstar_dense(const float2 *x1, const float2 *y, const float2 *z, co
   const RegParams rp = make_reg_params(method, dd, params.alpha0, params.alpha1, params.alpha);
   const Grid3 gr{W, H, T, N};
   launch_gstar_terms(method, gr, rp, img.as<float2>(), y, part_b_.as<double>(), s);
-  g += reduce_pair(part_b_.as<double>(), step_blocks(gr), 0, s);
+  g += reduce_pair(part_b_.as<double>(), norm_blocks(gr), 0, s);
   return g;
 }
 
@@ -294,12 +294,12 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
   AV_CHECK(params.lambda > 0.0, "lambda must be positive");
   ensure_partials();
   const int P = primal_components(method), D = dual_components(method);
-  const bool bf = params.dualStorage == 1;
+  const int dstore = params.dualStorage;
   const long K = op->compact_elems();
   DevBuf x, ext, y, z, d, khz, slog, tlog;
   x.ensure(sizeof(float2) * (size_t)P * N);
   ext.ensure(sizeof(float2) * (size_t)P * N);
-  y.ensure((bf ? sizeof(bf16x2) : sizeof(float2)) * (size_t)D * N);
+  y.ensure(dual_bytes(dstore) * (size_t)D * N);
   z.ensure(sizeof(float2) * (size_t)std::max(1L, K));
   d.ensure(sizeof(float2) * (size_t)std::max(1L, K));
   khz.ensure(sizeof(float2) * (size_t)N);
@@ -338,11 +338,11 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
   unsigned k = 0;
   for (; k < maxIt; ++k) {
     // dual ascent + projections (regulariser)
-    launch_dual(method, bf, g, rp, es, y.p, sc, s);
+    launch_dual(method, dstore, g, rp, es, y.p, sc, s);
     // data term: z update, prox and K^H z
     op->dual_update(es, b1, z.as<float2>(), d.as<float2>(), khz.as<float2>(), sc, s);
     // primal descent + extragradient
-    launch_primal(method, bf, g, rp, xs, es, y.p, khz.as<float2>(), sc, s);
+    launch_primal(method, dstore, g, rp, xs, es, y.p, khz.as<float2>(), sc, s);
     const bool adapt = params.adaptSchedule == 1 || k < 10 || (k % 50 == 0);
     if (adapt) {
       adapted[k] = 1;
@@ -365,19 +365,30 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
       const double zz = reduce_pair(part_a_.as<double>(), nb, 1, s);
       const float2 *yf = y.as<float2>();
       DevBuf yconv;
-      if (bf) {
+      if (dstore != kDualF32) {
+        // decode the 4-byte duals on the host (diagnostic path only)
         std::vector<uint16_t> hb((size_t)2 * D * N);
-        AV_HIP(hipMemcpyAsync(hb.data(), y.p, y.bytes, hipMemcpyDeviceToHost, s));
+        AV_HIP(hipMemcpyAsync(hb.data(), y.p, (size_t)4 * D * N, hipMemcpyDeviceToHost, s));
         AV_HIP(hipStreamSynchronize(s));
         std::vector<float2> hf((size_t)D * N);
-        for (size_t i = 0; i < hf.size(); ++i)
-          hf[i] = make_float2(bf16_to_f32(hb[2 * i]), bf16_to_f32(hb[2 * i + 1]));
+        const float radii[4] = {rp.r1, rp.r2, rp.r3, rp.r4};
+        for (size_t i = 0; i < hf.size(); ++i) {
+          if (dstore == kDualBF16) {
+            hf[i] = make_float2(bf16_to_f32(hb[2 * i]), bf16_to_f32(hb[2 * i + 1]));
+          } else {
+            // component groups: 3 (r1), 6 (r2), 3 (r3), 6 (r4)
+            const int c = (int)(i / (size_t)N);
+            const int grp = c < 3 ? 0 : c < 9 ? 1 : c < 12 ? 2 : 3;
+            const float sc16 = radii[grp] / kQ16Max;
+            hf[i] = make_float2((float)(int16_t)hb[2 * i] * sc16, (float)(int16_t)hb[2 * i + 1] * sc16);
+          }
+        }
         yconv.ensure(sizeof(float2) * hf.size());
         AV_HIP(hipMemcpyAsync(yconv.p, hf.data(), yconv.bytes, hipMemcpyHostToDevice, s));
         yf = yconv.as<float2>();
       }
       launch_gstar_terms(method, g, rp, khz.as<float2>(), yf, part_a_.as<double>(), s);
-      const double l1 = reduce_pair(part_a_.as<double>(), step_blocks(g), 0, s);
+      const double l1 = reduce_pair(part_a_.as<double>(), norm_blocks(g), 0, s);
       const double lam = h.lambda;
       const double gstar = 0.5 * lam * r2 + re_dz + zz / (2.0 * lam) + l1;
       const double r = regularizer_norm(xs, s);

@@ -25,7 +25,7 @@ struct Task {
   RegParams rp;
   CartGeom cg;
   float2 *x, *ext;     // primal, P components each
-  void *y;             // dual, D components (float2 or bf16x2)
+  void *y;             // dual, D components (float2, bf16x2 or q16x2)
   float2 *khz;         // K^H z (image)
   float2 *z;           // compact k-space dual
   const float2 *d;     // compact data
@@ -46,7 +46,7 @@ struct TaskGrid {
   int vox = 0, col = 0, row = 0;
 };
 
-void launch_iteration(int method, bool bf16, const Task *d_tasks, int n, const TaskGrid &grid,
+void launch_iteration(int method, int dual_storage, const Task *d_tasks, int n, const TaskGrid &grid,
                       bool with_step, hipStream_t s);
 
 // Kernel families of one iteration, in launch order.
@@ -56,7 +56,7 @@ enum KernelId {
 };
 const char *kernel_name(int id);
 // Launch one kernel family alone (for timing).
-void launch_kernel(int id, int method, bool bf16, const Task *d_tasks, int n, const TaskGrid &grid,
+void launch_kernel(int id, int method, int dual_storage, const Task *d_tasks, int n, const TaskGrid &grid,
                    hipStream_t s);
 
 }  // namespace avionic

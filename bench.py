@@ -18,7 +18,7 @@ FP64 oracle (C, OpenMP) on the host cores for a few iterations of the same
 reconstruction.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-                       [--dual fp32|bf16] [--no-cpu]
+                       [--dual fp32|bf16|q16] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -123,19 +123,59 @@ def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500):
             "all_finite": finite}
 
 
+def small_leg(A, synth, torch, dev, B, steps, warmup):
+    """Batching where it matters: the TV 128x128x12 frames x 8 coils workload
+    is too small to fill the GPU alone.  One reconstruction with eager
+    launches against B of them advanced together from a replayed graph."""
+    wl = synth.CONFIGS["tv_128x12x8"]
+    inp = synth.make(wl)
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.complex64)).to(dev)
+    data, b1, x0 = c64(inp["data"]), c64(inp["b1"]), c64(inp["x0"])
+    mask = torch.from_numpy(np.ascontiguousarray(inp["mask"], np.float32)).to(dev)
+    params = dict(wl.params, maxIt=warmup + steps + 10, adaptLambda=0)
+
+    def run(use_graphs, n):
+        eng = A.Engine(use_graphs=use_graphs, max_batch=n)
+        for _ in range(n):
+            eng.submit(A.Request(0, data, b1, mask=mask, x0=x0, params=params))
+        for _ in range(warmup):
+            eng.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.step()
+        torch.cuda.synchronize()
+        return n * steps / (time.perf_counter() - t0)
+
+    plain = run(False, 1)
+    batched = run(True, B)
+    return {"workload": wl.name, "batch": B, "plain_iters_per_s": round(plain, 1),
+            "batched_iters_per_s": round(batched, 1), "batched_over_plain": round(batched / plain, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--dual", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--dual", choices=["fp32", "bf16", "q16"], default="q16",
+                    help="dual storage: float32, bfloat16, or 16-bit fixed point scaled by the "
+                         "projection radius (4 bytes per complex like bf16; the one of the two "
+                         "4-byte formats that meets NRMSE <= 1e-4 at full size)")
     ap.add_argument("--workload", default="ictgv2_256x30x16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window of the serving stream (0: skip the serving leg)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--small-batch", type=int, default=64,
+                    help="reconstructions of the TV 128x128x12x8 workload batched in the "
+                         "small-request leg (0: skip)")
+    ap.add_argument("--dump-x", default=None,
+                    help="directory: also run one full reconstruction (the workload's iteration "
+                         "count) with fp32 and with bf16 duals and save x as .npy, for the NRMSE "
+                         "check against the FP64 oracle (python -m oracle.nrmse)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,7 +196,8 @@ def main():
     inp = synth.make(wl)
     T, C, H, W = inp["data"].shape
     method = {"tv": 0, "tgv2": 1, "ictgv2": 2}[wl.method]
-    bf16 = args.dual == "bf16"
+    dual_kind = {"fp32": 0, "bf16": 1, "q16": 2}[args.dual]
+    bf16 = dual_kind != 0  # 4-byte duals
 
     def c64(a):
         return torch.from_numpy(np.ascontiguousarray(a, np.complex64)).to(dev)
@@ -165,7 +206,7 @@ def main():
     mask = torch.from_numpy(np.ascontiguousarray(inp["mask"], np.float32)).to(dev)
     total_its = args.warmup + args.steps + 10
     params = dict(wl.params)
-    params.update(maxIt=total_its, dualStorage=1 if bf16 else 0)
+    params.update(maxIt=total_its, dualStorage=dual_kind)
     if "lambda" in wl.params:
         params["adaptLambda"] = 0
 
@@ -245,6 +286,27 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(ktimes[dom], 4)}
 
+    small = None
+    if args.small_batch > 0:
+        log(f"small-request leg (tv_128x12x8, B={args.small_batch})")
+        small = small_leg(A, synth, torch, dev, args.small_batch, args.steps, args.warmup)
+
+    if args.dump_x:
+        os.makedirs(args.dump_x, exist_ok=True)
+        for ds in ("fp32", "bf16", "q16"):
+            p_full = dict(params, maxIt=wl.iterations,
+                          dualStorage={"fp32": 0, "bf16": 1, "q16": 2}[ds])
+            e1 = A.Engine(use_graphs=True, max_batch=1)
+            rid = e1.submit(A.Request(method, data, b1, mask=mask, x0=x0, params=p_full))
+            xo = torch.empty_like(x0)
+            while not e1.poll(rid, xo):
+                e1.step()
+            torch.cuda.synchronize()
+            path = os.path.join(args.dump_x, f"{wl.name}_{wl.iterations}_{ds}.npy")
+            np.save(path, xo.cpu().numpy())
+            log(f"saved {path}")
+            del e1
+
     serving = None
     if args.serve_seconds > 0:
         log(f"serving stream ({args.serve_rate}/s for {args.serve_seconds}s)")
@@ -280,7 +342,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64" + ("+bf16-duals" if bf16 else ""),
+        "dtype": "c64" + ({0: "", 1: "+bf16-duals", 2: "+q16-duals"}[dual_kind]),
         "data": "synthetic",
         "config": {"workload": wl.name, "method": wl.method, "matrix": [H, W], "frames": T,
                    "coils": C, "accel": wl.accel, "batch": args.batch,
@@ -291,6 +353,7 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "small_requests": small,
         "serving": serving,
     }
     print(json.dumps(out))

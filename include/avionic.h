@@ -71,8 +71,12 @@ typedef struct avionic_params {
   double dx2, dy2, dt2;
   int adaptSchedule;           /* 0: adapt when k<10 || k%50==0 (tv.cpp:172);
                                   1: every iteration (tv_dmri.m:198, debug=1) */
-  int dualStorage;             /* 0: float32 duals; 1: bfloat16 duals with
-                                  float32 arithmetic */
+  int dualStorage;             /* 0: float32 duals; 1: bfloat16 duals;
+                                  2: 16-bit fixed point scaled by each dual's
+                                  projection radius.  1 and 2 store 4 bytes
+                                  per complex value; arithmetic is float32
+                                  (no counterpart in the reference, which
+                                  keeps float32 duals) */
 } avionic_params;
 
 typedef struct avionic_operator avionic_operator;

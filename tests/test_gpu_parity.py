@@ -113,3 +113,21 @@ def test_recon_bf16_duals_runs(gpu):
     err = rel(got["x"], ref)
     print(f"[gpu recon bf16] NRMSE {err:.3e}", flush=True)
     assert err < 5e-2
+
+
+@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2"])
+def test_recon_q16_duals_match_oracle(gpu, method):
+    """16-bit fixed-point duals scaled by the projection radius (dualStorage
+    2): 4 bytes per complex dual like bf16, with an absolute resolution of
+    radius / 32767, which keeps the reconstruction within 1e-4 of the FP64
+    oracle where bf16 (8-bit mantissa) does not."""
+    data, b1, mask = _problem()
+    p = O.default_params(method)
+    T, C, H, W = data.shape
+    k, d = O.adapt_lambda_defaults(method)
+    p.update(maxIt=200, lambda_=O.adapt_lambda_cart(mask, W, H, T, k, d))
+    ref = O.recon(p, data, b1, mask)
+    got = gpu.recon_logs(dict(p, dualStorage=2), data, b1, mask, None)
+    err = rel(got["x"], ref)
+    print(f"[gpu recon q16] {method} NRMSE {err:.3e}", flush=True)
+    assert err < 1e-4
