@@ -106,9 +106,12 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2 *data, FftPlan ph, int H
 // ---------------------------------------------------------------------------
 // host side
 
+#ifndef AVIONIC_COL_MAXXB
+#define AVIONIC_COL_MAXXB 64
+#endif
 static int pick_xb(int H, int W) {
   int xb = 1;
-  while (xb * 2 * H <= EMAX && xb * 2 <= 64 && xb < W) xb *= 2;
+  while (xb * 2 * H <= EMAX && xb * 2 <= AVIONIC_COL_MAXXB && xb < W) xb *= 2;
   return xb;
 }
 

@@ -216,3 +216,47 @@ def serving_pool(seed=4, frames=30, coils=20):
                  for m, (a, c, s) in SERVE_SAMPLING.items()}
         pool[n] = dict(rho=rho, b1=b1, masks=masks)
     return pool
+
+
+# ------------------------------------------------------------ radial (cfg3)
+@dataclasses.dataclass
+class RadialWorkload:
+    name: str = "tgv2_radial_256x30x12"
+    method: str = "tgv2"
+    n: int = 256
+    frames: int = 30
+    coils: int = 12
+    n_fe: int = 512          # samples per spoke (2x readout oversampling)
+    spokes: int = 21         # spokes per frame, golden-angle increments
+    kernel_width: int = 3
+    osf: float = 1.5
+    ellipses: int = 10
+    seed: int = 1
+    iterations: int = 1000
+
+
+RADIAL = RadialWorkload()
+
+
+def radial_trajectory(n_fe, spokes_per_frame, frames):
+    """Golden-angle (111.246 deg) radial spokes, [frames, 2, spokes*n_fe] in
+    cycles per pixel, k in [-0.5, 0.5)."""
+    ga = np.pi * (np.sqrt(5.0) - 1.0) / 2.0
+    r = (np.arange(n_fe) - n_fe / 2.0) / n_fe
+    out = np.zeros((frames, 2, spokes_per_frame * n_fe))
+    for t in range(frames):
+        a = (t * spokes_per_frame + np.arange(spokes_per_frame)) * ga
+        out[t, 0] = (r[None, :] * np.cos(a)[:, None]).ravel()
+        out[t, 1] = (r[None, :] * np.sin(a)[:, None]).ravel()
+    return out
+
+
+def make_radial_host(wl: RadialWorkload = RADIAL):
+    """Host inputs of the radial workload: phantom, coil maps, trajectory and
+    density |k|.  The k-space data is K rho + noise, formed with the
+    operator itself (on the device) by the caller."""
+    rho = phantom(wl.n, wl.frames, wl.ellipses, wl.seed)
+    b1 = coil_maps(wl.n, wl.coils, wl.seed)
+    traj = radial_trajectory(wl.n_fe, wl.spokes, wl.frames)
+    dcf = np.sqrt(traj[:, 0] ** 2 + traj[:, 1] ** 2)
+    return dict(rho=rho, b1=b1, traj=traj, dcf=dcf)

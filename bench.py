@@ -123,6 +123,52 @@ def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500):
             "all_finite": finite}
 
 
+def radial_leg(A, synth, torch, dev, iters, dump_dir=None):
+    """Non-Cartesian TGV2: golden-angle radial, 512 samples x 21 spokes per
+    frame, 30 frames, 12 coils, 256x256, Kaiser-Bessel width 3 on a 1.5x grid,
+    density |k|.  One reconstruction through the PDRecon API (eager launches);
+    k-space = K rho + complex Gaussian noise (1% of max |K rho|), formed with
+    the operator itself; u0 = K^H d."""
+    wl = synth.RADIAL
+    h = synth.make_radial_host(wl)
+    f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.complex64)).to(dev)
+    n, T, C = wl.n, wl.frames, wl.coils
+    t0 = time.perf_counter()
+    op = A.NoncartesianOperator(n, n, C, T, wl.n_fe, wl.spokes, f32(h["traj"]), f32(h["dcf"]),
+                                kernel_width=wl.kernel_width, osf=wl.osf)
+    t_setup = time.perf_counter() - t0
+    rho, b1 = c64(h["rho"]), c64(h["b1"])
+    data = op.BackwardOperation(rho, b1)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(wl.seed + 3000)
+    sig = 0.01 * float(data.abs().max().item())
+    data = data + sig * torch.randn(data.shape, dtype=torch.complex64, device=dev, generator=gen)
+    x0 = op.ForwardOperation(data, b1)
+    rec = A.TGV2(n, n, C, T, op, None)
+    p = rec.GetParams()
+    p.update(alpha0=2.0, alpha1=1.0, maxIt=10)
+    rec.SetParams(p)
+    x = x0.clone()
+    rec.IterativeReconstruction(data, x, b1)  # warm-up
+    p["maxIt"] = iters
+    rec.SetParams(p)
+    x = x0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec.IterativeReconstruction(data, x, b1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    if dump_dir:
+        os.makedirs(dump_dir, exist_ok=True)
+        np.save(os.path.join(dump_dir, f"{wl.name}_{iters}_fp32.npy"), x.cpu().numpy())
+        np.save(os.path.join(dump_dir, f"{wl.name}_data.npy"), data.cpu().numpy())
+    return {"workload": wl.name, "method": "tgv2", "iterations": iters,
+            "iters_per_s": round(iters / dt, 1), "ms_per_iter": round(1e3 * dt / iters, 4),
+            "operator_setup_s": round(t_setup, 3), "lambda": round(p["lambda"], 6),
+            "points_per_frame": wl.n_fe * wl.spokes, "grid": [round(wl.osf * n)] * 2}
+
+
 def small_leg(A, synth, torch, dev, B, steps, warmup):
     """Batching where it matters: the TV 128x128x12 frames x 8 coils workload
     is too small to fill the GPU alone.  One reconstruction with eager
@@ -172,6 +218,10 @@ def main():
     ap.add_argument("--small-batch", type=int, default=64,
                     help="reconstructions of the TV 128x128x12x8 workload batched in the "
                          "small-request leg (0: skip)")
+    ap.add_argument("--radial-iters", type=int, default=200,
+                    help="iterations of the radial TGV2 leg (0: skip)")
+    ap.add_argument("--dump-radial", default=None,
+                    help="directory: save the radial leg's x and k-space data as .npy")
     ap.add_argument("--dump-x", default=None,
                     help="directory: also run one full reconstruction (the workload's iteration "
                          "count) with fp32 and with bf16 duals and save x as .npy, for the NRMSE "
@@ -307,6 +357,11 @@ def main():
             log(f"saved {path}")
             del e1
 
+    radial = None
+    if args.radial_iters > 0:
+        log(f"radial TGV2 leg ({synth.RADIAL.name}, {args.radial_iters} iterations)")
+        radial = radial_leg(A, synth, torch, dev, args.radial_iters, args.dump_radial)
+
     serving = None
     if args.serve_seconds > 0:
         log(f"serving stream ({args.serve_rate}/s for {args.serve_seconds}s)")
@@ -354,6 +409,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "small_requests": small,
+        "radial": radial,
         "serving": serving,
     }
     print(json.dumps(out))
