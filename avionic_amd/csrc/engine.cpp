@@ -77,8 +77,10 @@ struct Group {
 struct avionic_engine {
   avionic_engine_config cfg{};
   hipStream_t stream = nullptr;       // compute
+  hipStream_t side = nullptr;         // compute, overlapped regulariser kernels
   hipStream_t copy_stream = nullptr;  // admissions
   hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_req = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   long long next_id = 1;
   size_t used = 0;
   std::map<long long, std::unique_ptr<Slot>> slots;
@@ -89,6 +91,9 @@ struct avionic_engine {
     slots.clear();
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (ev_req) (void)hipEventDestroy(ev_req);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
@@ -121,15 +126,16 @@ struct avionic_engine {
 
   void run_iteration(Group &g, bool with_step) {
     const int n = (int)g.slots.size();
+    const IterStreams st{stream, side, ev_fork, ev_join};
     if (!cfg.use_graphs) {
-      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, stream);
+      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st);
       return;
     }
     hipGraphExec_t &ex = with_step ? g.g_step : g.g_plain;
     if (!ex) {
       hipGraph_t graph = nullptr;
       AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, stream);
+      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st);
       AV_HIP(hipStreamEndCapture(stream, &graph));
       AV_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);
@@ -203,6 +209,9 @@ int avionic_engine_create(const avionic_engine_config *cfg, avionic_engine **eng
     if (e->cfg.max_batch <= 0) e->cfg.max_batch = 1 << 20;
     AV_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     AV_HIP(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+    AV_HIP(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+    AV_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    AV_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
     AV_HIP(hipEventCreateWithFlags(&e->ev_req, hipEventDisableTiming));
     AV_HIP(hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming));
     AV_HIP(hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming));

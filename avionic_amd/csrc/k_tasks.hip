@@ -224,8 +224,21 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
 }
 
 void launch_iteration(int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
-                      bool with_step, hipStream_t s) {
+                      bool with_step, const IterStreams &st) {
   if (n <= 0) return;
+  hipStream_t s = st.main;
+  const bool two = st.side != nullptr;
+  hipStream_t s2 = two ? st.side : st.main;
+  auto fork = [&]() {
+    if (!two) return;
+    AV_HIP(hipEventRecord(st.fork, s));
+    AV_HIP(hipStreamWaitEvent(s2, st.fork, 0));
+  };
+  auto join = [&]() {
+    if (!two) return;
+    AV_HIP(hipEventRecord(st.join, s2));
+    AV_HIP(hipStreamWaitEvent(s, st.join, 0));
+  };
   const dim3 bRT(RT), bNT(NT), bNC(NTC);
 #define AV_BYMETHOD(KER, ...)                                                        \
   do {                                                                               \
@@ -243,23 +256,31 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
       else hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);                    \
     }                                                                                \
   } while (0)
-  if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+  // dual update (reads ext, writes y) beside the data-term chain (reads ext,
+  // writes kcol / z / zhat / khz); the primal update needs both
+  fork();
+  if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
   AV_LAUNCH_CHECK();
   if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNC, 0, s, d_tasks, n);
   if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n);
   if (gr.col > 0) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNC, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();
+  join();
   if (gr.vox > 0) AV_BYMETHOD(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();
   if (with_step) {
+    // regulariser sums (read ext, x) beside K(ext - x); the tail needs both
+    fork();
     if (gr.vox > 0) {
-      if (method == kTV) hipLaunchKernelGGL(kt_step_reg<kTV>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
-      else if (method == kTGV2) hipLaunchKernelGGL(kt_step_reg<kTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
-      else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+      if (method == kTV) hipLaunchKernelGGL(kt_step_reg<kTV>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
+      else if (method == kTGV2) hipLaunchKernelGGL(kt_step_reg<kTGV2>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
+      else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
     }
+    AV_LAUNCH_CHECK();
     if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNC, 0, s, d_tasks, n);
     if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n);
     AV_LAUNCH_CHECK();
+    join();
   }
   hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();

@@ -46,8 +46,17 @@ struct TaskGrid {
   int vox = 0, col = 0, row = 0;
 };
 
+// Streams of one iteration: the regulariser kernels that do not depend on
+// the data-term chain (the dual update; the step rule's regulariser sums)
+// run on `side`, forked from and joined back into `main` with the two
+// events, so that the HBM-bound stencil kernel overlaps the latency-bound
+// FFT passes.  side == nullptr runs everything on main.
+struct IterStreams {
+  hipStream_t main = nullptr, side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
 void launch_iteration(int method, int dual_storage, const Task *d_tasks, int n, const TaskGrid &grid,
-                      bool with_step, hipStream_t s);
+                      bool with_step, const IterStreams &st);
 
 // Kernel families of one iteration, in launch order.
 enum KernelId {
