@@ -23,7 +23,7 @@ namespace avionic {
 
 constexpr int RTILE = kRowTileElems;
 constexpr int NTC = AVIONIC_COL_THREADS;  // threads of a column-pass block
-constexpr int kColSlots = max_slots(EMAX, NTC);
+constexpr int kColSlots = max_slots(kColTileElems, NTC);
 constexpr bool kColUcache = AVIONIC_COL_UCACHE != 0;  // keep the image tile in registers
 #ifndef AVIONIC_COL_ACC
 #define AVIONIC_COL_ACC 1
@@ -82,13 +82,23 @@ struct ColFwdStore {
   }
 };
 
+// The frame's rowmap (compact row of each k-space row, -1 if not sampled)
+// staged in LDS: the store / load functors then need no dependent global
+// read before each k-space access.
+__device__ inline const int *stage_rowmap(const CartGeom &g, int t, int *lds, int tid, int nt) {
+  const int *rm = g.rowmap + (long)t * g.H;
+  for (int i = tid; i < g.H; i += nt) lds[i] = rm[i];
+  return lds;
+}
+
 template <bool DELTA>
 __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, const float2 *u,
                                          const float2 *usub, const float2 *b1, float2 *kcol,
-                                         float2 *tile, float2 *twl) {
+                                         float2 *tile, float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
   const float2 *tw = stage_twiddles(g.ph, twl, tid, NTC);
+  const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
   ColFwdLoad<DELTA> ld;
@@ -96,15 +106,15 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
   ld.st = DELTA ? usub + t * HW : nullptr;
   ld.H = g.H;
   ld.W = g.W;
-  ld.xb0 = xbi * g.XB;
+  ld.xb0 = xbi * g.XBF;
   ld.centered = g.centered;
-  ColFwdStore st{nullptr, g.rowmap + (long)t * g.H, g.H, g.W, xbi * g.XB, g.centered};
+  ColFwdStore st{nullptr, rmap, g.H, g.W, xbi * g.XBF, g.centered};
   for (int c = 0; c < g.C; ++c) {
     ld.bc = b1 + c * HW;
     ld.bn = (c + 1 < g.C) ? b1 + (c + 1) * HW : nullptr;
     ld.first = (c == 0);
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
-    fft_io<NTC, EMAX, true, true, true>(tile, g.ph, g.XB, 1, g.XB, +1.f, tid, tw, ld, st);
+    fft_io<NTC, kColTileElems, true, true, true>(tile, g.ph, g.XBF, 1, g.XBF, +1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
@@ -164,17 +174,18 @@ struct ColAdjStore {
 
 __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, const float2 *zhat,
                                          const float2 *b1, float2 *out, float2 *tile,
-                                         float2 *twl) {
+                                         float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
   const float2 *tw = stage_twiddles(g.ph, twl, tid, NTC);
+  const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
   ColAdjLoad ld;
-  ld.rmap = g.rowmap + (long)t * g.H;
+  ld.rmap = rmap;
   ld.H = g.H;
   ld.W = g.W;
-  ld.xb0 = xbi * g.XB;
+  ld.xb0 = xbi * g.XBF;
   ld.centered = g.centered;
   ColAdjStore st;
 #pragma unroll
@@ -182,7 +193,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   st.out = out + t * HW;
   st.H = g.H;
   st.W = g.W;
-  st.xb0 = xbi * g.XB;
+  st.xb0 = xbi * g.XBF;
   st.centered = g.centered;
   st.scale = g.scale;
   for (int c = 0; c < g.C; ++c) {
@@ -193,7 +204,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
     st.bn = (c + 1 < g.C) ? b1 + (c + 1) * HW : nullptr;
     st.first = (c == 0);
     st.last = (c == g.C - 1);
-    fft_io<NTC, EMAX, true, true, true>(tile, g.ph, g.XB, 1, g.XB, -1.f, tid, tw, ld, st);
+    fft_io<NTC, kColTileElems, true, true, true>(tile, g.ph, g.XBF, 1, g.XBF, -1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }

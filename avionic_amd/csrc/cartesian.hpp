@@ -22,9 +22,10 @@ namespace avionic {
 struct CartGeom {
   int W, H, C, T;
   int centered;
-  int XB;  // columns per column-pass tile
+  int XB;  // columns per column-pass tile (API kernels)
   int RB;  // rows per row-pass tile
   int nxb;
+  int XBF, nxbf;  // columns per tile / tiles of the hot loop's fused column passes
   long nrows;          // sampled rows over all frames (row_off[T])
   long nrows_c;        // C * nrows
   const int *row_off;  // [T+1]

@@ -123,6 +123,11 @@ constexpr int kFftTileElems = 4096;  // 32 KiB of float2 (column passes)
 constexpr int kRowTileElems = 2048;  // 16 KiB of float2 (row passes)
 constexpr int kMaxStages = 12;
 constexpr int kMaxFftLen = 512;  // longest 1D transform (twiddles staged in LDS)
+// LDS tile of the hot loop's fused column passes (complex elements).
+#ifndef AVIONIC_COL_EMAX
+#define AVIONIC_COL_EMAX 4096
+#endif
+constexpr int kColTileElems = AVIONIC_COL_EMAX;
 
 // A 1D FFT plan for one length: Stockham stages (radix, Ns) and twiddles.
 struct FftPlan {

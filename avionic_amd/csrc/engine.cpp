@@ -327,7 +327,7 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     t.part_reg = s->preg.as<double>();
     t.part_k = s->pk.as<double>();
     t.nvox_blocks = nvb;
-    t.ncol_blocks = t.cg.nxb * t.cg.T;
+    t.ncol_blocks = t.cg.nxbf * t.cg.T;
     t.nrow_blocks = nrb;
     t.schedule = s->params.adaptSchedule;
 

@@ -109,9 +109,9 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2 *data, FftPlan ph, int H
 #ifndef AVIONIC_COL_MAXXB
 #define AVIONIC_COL_MAXXB 64
 #endif
-static int pick_xb(int H, int W) {
+static int pick_xb(int H, int W, int emax = EMAX) {
   int xb = 1;
-  while (xb * 2 * H <= EMAX && xb * 2 <= AVIONIC_COL_MAXXB && xb < W) xb *= 2;
+  while (xb * 2 * H <= emax && xb * 2 <= AVIONIC_COL_MAXXB && xb < W) xb *= 2;
   return xb;
 }
 
@@ -156,6 +156,8 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   g.XB = pick_xb(H, W);
   g.RB = std::max(1, kRowTileElems / W);
   g.nxb = (W + g.XB - 1) / g.XB;
+  g.XBF = pick_xb(H, W, kColTileElems);
+  g.nxbf = (W + g.XBF - 1) / g.XBF;
   g.scale = (float)(1.0 / std::sqrt((double)H * W));
   const long HW = (long)H * W;
   std::vector<float> hm;

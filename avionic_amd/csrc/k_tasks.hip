@@ -82,14 +82,15 @@ __global__ __launch_bounds__(RT) void kt_step_reg(const Task *__restrict__ tasks
 // DELTA = true : K of ext - x (step rule, adapting tasks only).
 template <bool DELTA>
 __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n) {
-  __shared__ float2 tile[EMAX];
+  __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[kMaxFftLen];
+  __shared__ int rml[kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
   if (!task_active(T.sc)) return;
   if (DELTA && !task_adapt(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
-  col_forward_fused<DELTA>(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.ext, DELTA ? T.x : nullptr,
-                           T.b1, T.kcol, tile, twl);
+  col_forward_fused<DELTA>(T.cg, lb % T.cg.nxbf, lb / T.cg.nxbf, T.ext, DELTA ? T.x : nullptr,
+                           T.b1, T.kcol, tile, twl, rml);
 }
 
 template <int MODE>
@@ -131,12 +132,13 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
 #endif
 
 __global__ __launch_bounds__(NTC) AV_COL_ATTR void kt_col_adjoint(const Task *__restrict__ tasks, int n) {
-  __shared__ float2 tile[EMAX];
+  __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[kMaxFftLen];
+  __shared__ int rml[kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
   if (!task_active(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
-  col_adjoint_fused(T.cg, lb % T.cg.nxb, lb / T.cg.nxb, T.zhat, T.b1, T.khz, tile, twl);
+  col_adjoint_fused(T.cg, lb % T.cg.nxbf, lb / T.cg.nxbf, T.zhat, T.b1, T.khz, tile, twl, rml);
 }
 
 // One block per task: step rule for adapting tasks, then the iteration
