@@ -215,19 +215,26 @@ struct RowLoad {
   int W;
   __device__ float2 operator()(int b, int i, int) const { return src[(unsigned)(b * W + i)]; }
 };
+// Mask row (index into mrows) of each row of the block, staged in LDS so
+// that the mask load does not wait on a global rowinfo load.
+__device__ inline const int *stage_mask_rows(const CartGeom &g, long g0, int nr, int *lds, int tid,
+                                             int nt) {
+  for (int b = tid; b < nr; b += nt) lds[b] = g.rowinfo[g0 + b].z;
+  return lds;
+}
+
 struct RowDualStore {
   float2 *z;
   const float2 *d;
   const float *mrows;
-  const int4 *rowinfo;
+  const int *mrow;  // LDS: mask row of local row b
   float2 *tile2;
   long g0;
   int W, wc, centered;
   float scale, sigma, inv;
   __device__ void operator()(int b, int i, float2 v, int) {
     const int kx = centered ? (i + W / 2) % W : i;
-    const int4 ri = rowinfo[g0 + b];
-    const float mk = mrows[(long)ri.z * W + kx];
+    const float mk = mrows[(long)mrow[b] * W + kx];
     const float2 val = cscale(v, scale * mk);
     const long gi = (g0 + b) * W + kx;
     float2 zz = z[gi];
@@ -249,34 +256,34 @@ struct RowAdjStore {
 };
 struct RowNorm2Store {
   const float *mrows;
-  const int4 *rowinfo;
+  const int *mrow;  // LDS: mask row of local row b
   long g0;
   int W, centered;
   float scale;
   double acc;
   __device__ void operator()(int b, int i, float2 v, int) {
     const int kx = centered ? (i + W / 2) % W : i;
-    const int4 ri = rowinfo[g0 + b];
-    const float2 val = cscale(v, scale * mrows[(long)ri.z * W + kx]);
+    const float2 val = cscale(v, scale * mrows[(long)mrow[b] * W + kx]);
     acc += (double)cabs2(val);
   }
 };
 
 __device__ inline void row_dual_fused(const CartGeom &g, int blk, const float2 *kcol, float2 *z,
                                       const float2 *d, float2 *zhat, const DevScalars *sc,
-                                      float2 *tile, float2 *tile2, float2 *twl) {
+                                      float2 *tile, float2 *tile2, float2 *twl, int *mrl) {
   const int tid = threadIdx.x;
   const int W = g.W;
   const long g0 = (long)blk * g.RB;
   const int nr = (int)min((long)g.RB, g.nrows_c - g0);
   const float2 *tw = stage_twiddles(g.pw, twl, tid, NT);
+  const int *mrow = stage_mask_rows(g, g0, nr, mrl, tid, NT);
   __syncthreads();
   RowLoad ld{kcol + g0 * W, W};
   RowDualStore st;
   st.z = z;
   st.d = d;
   st.mrows = g.mrows;
-  st.rowinfo = g.rowinfo;
+  st.mrow = mrow;
   st.tile2 = tile2;
   st.g0 = g0;
   st.W = W;
@@ -293,15 +300,16 @@ __device__ inline void row_dual_fused(const CartGeom &g, int blk, const float2 *
 }
 
 __device__ inline double row_norm2_fused(const CartGeom &g, int blk, const float2 *kcol,
-                                         float2 *tile, float2 *twl) {
+                                         float2 *tile, float2 *twl, int *mrl) {
   const int tid = threadIdx.x;
   const int W = g.W;
   const long g0 = (long)blk * g.RB;
   const int nr = (int)min((long)g.RB, g.nrows_c - g0);
   const float2 *tw = stage_twiddles(g.pw, twl, tid, NT);
+  const int *mrow = stage_mask_rows(g, g0, nr, mrl, tid, NT);
   __syncthreads();
   RowLoad ld{kcol + g0 * W, W};
-  RowNorm2Store st{g.mrows, g.rowinfo, g0, W, g.centered, g.scale, 0.0};
+  RowNorm2Store st{g.mrows, mrow, g0, W, g.centered, g.scale, 0.0};
   fft_io<NT, RTILE, false, true, true>(tile, g.pw, nr, W, 1, +1.f, tid, tw, ld, st);
   return st.acc;
 }

@@ -121,6 +121,7 @@ struct DualIO<q16x2> {
 constexpr int kFftThreads = 256;
 constexpr int kFftTileElems = 4096;  // 32 KiB of float2 (column passes)
 constexpr int kRowTileElems = 2048;  // 16 KiB of float2 (row passes)
+constexpr int kRowMaxRows = 256;     // rows of one row-pass tile at most
 constexpr int kMaxStages = 12;
 constexpr int kMaxFftLen = 512;  // longest 1D transform (twiddles staged in LDS)
 // LDS tile of the hot loop's fused column passes (complex elements).

@@ -154,7 +154,7 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   g.W = W; g.H = H; g.C = C; g.T = T;
   g.centered = centered ? 1 : 0;
   g.XB = pick_xb(H, W);
-  g.RB = std::max(1, kRowTileElems / W);
+  g.RB = std::max(1, std::min(kRowMaxRows, kRowTileElems / W));
   g.nxb = (W + g.XB - 1) / g.XB;
   g.XBF = pick_xb(H, W, kColTileElems);
   g.nxbf = (W + g.XBF - 1) / g.XBF;

@@ -99,14 +99,15 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
   __shared__ float2 tile2[(MODE == kRowDualUpdate) ? RTILE : 1];
   __shared__ float2 twl[kMaxFftLen];
   __shared__ double red[NT];
+  __shared__ int mrl[kRowMaxRows];
   const Task &T = tasks[find_task<kRow>(tasks, n, blockIdx.x)];
   if (!task_active(T.sc)) return;
   if (MODE == kRowNorm2 && !task_adapt(T.sc)) return;
   const int lb = blockIdx.x - T.b_row;
   if (MODE == kRowDualUpdate) {
-    row_dual_fused(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, tile2, twl);
+    row_dual_fused(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, tile2, twl, mrl);
   } else {
-    const double a = row_norm2_fused(T.cg, lb, T.kcol, tile, twl);
+    const double a = row_norm2_fused(T.cg, lb, T.kcol, tile, twl, mrl);
     const int tid = threadIdx.x;
     red[tid] = a;
     __syncthreads();
