@@ -54,6 +54,22 @@ def algorithmic_bytes_per_voxel(kernel, method, bf16):
     return None
 
 
+def measured_traffic(workload, batch, dual, kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs recorded in profiles/traffic.json (counters scaled by a
+    plain copy of known size), when that record is for this workload, batch
+    and dual storage; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (rec.get("workload"), rec.get("batch"), rec.get("dual_storage")) != (workload, batch, dual):
+        return None
+    return rec.get("hbm_bytes_per_launch", {}).get(kernel)
+
+
 def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500):
     """Continuous batching under Poisson arrivals: mixed TV / TGV2 / ICTGV2
     Cartesian requests (n in 128..256, T 8..30, C 8..20, 500 iterations each).
@@ -333,7 +349,8 @@ def main():
     bytes_per_launch = per_voxel[dom] * N * args.batch
     achieved = bytes_per_launch / (ktimes[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(wl.name, args.batch, args.dual, dom),
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(ktimes[dom], 4)}
 
     small = None

@@ -4,6 +4,8 @@ TEST INFRASTRUCTURE ONLY (the checker).  Usage:
 
     python bench.py --dump-x DIR ...            # on the GPU: saves x per dual storage
     python -m oracle.nrmse --workload ictgv2_256x30x16 DIR/*.npy [--ref REF.npy]
+    python -m oracle.nrmse --workload tgv2_radial_256x30x12 --radial-data D.npy \
+        --iterations 200 X.npy
 
 Runs the oracle PD loop for the workload's full iteration count on the same
 synthetic inputs (or loads a previously saved oracle result with --ref; with
@@ -38,6 +40,27 @@ def oracle_reference(name, iterations=None, nthreads=0):
     return O.recon(p, inp["data"], inp["b1"], inp["mask"], inp["x0"], nthreads=nthreads)
 
 
+def oracle_radial_reference(data_path, iterations, nthreads=0):
+    """FP64 gridding oracle of the radial TGV2 workload (synth.RADIAL) on the
+    k-space data the GPU run saved (bench.py --dump-radial): same phantom-free
+    inputs (coil maps, trajectory, density from synth), u0 = K^H d, TGV2
+    defaults with alpha0/alpha1 = 2/1 and lambda = k W / (spokes pi / 2) + d
+    (noncartesian_operator.cpp:88-95)."""
+    wl = synth.RADIAL
+    h = synth.make_radial_host(wl)
+    traj = h["traj"].astype(np.float32).astype(np.float64)
+    dcf = h["dcf"].astype(np.float32).astype(np.float64)
+    b1 = h["b1"].astype(np.complex64).astype(np.complex128)
+    data = np.load(data_path).astype(np.complex128)
+    x0 = O.nufft_KH(data, b1, traj, dcf, wl.kernel_width, wl.osf)
+    p = O.default_params("tgv2")
+    k, d = O.adapt_lambda_defaults("tgv2")
+    p.update(maxIt=iterations, lambda_=wl.n / (wl.spokes * np.pi / 2) * k + d, alpha0=2.0,
+             alpha1=1.0)
+    return O.recon_nc(p, data, b1, traj, dcf, x0=x0, J=wl.kernel_width, osf=wl.osf,
+                      nthreads=nthreads)
+
+
 def nrmse(x, ref):
     return float(np.linalg.norm((x - ref).ravel()) / np.linalg.norm(ref.ravel()))
 
@@ -48,13 +71,20 @@ def main():
     ap.add_argument("--ref", default=None, help="saved oracle result (.npy)")
     ap.add_argument("--save-ref", default=None)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--radial-data", default=None,
+                    help="radial workload: k-space data saved by bench.py --dump-radial")
+    ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("gpu_results", nargs="+")
     a = ap.parse_args()
     if a.ref:
         ref = np.load(a.ref)
     else:
         t0 = time.time()
-        ref = oracle_reference(a.workload, nthreads=a.threads)
+        if a.radial_data:
+            ref = oracle_radial_reference(a.radial_data, a.iterations or synth.RADIAL.iterations,
+                                          nthreads=a.threads)
+        else:
+            ref = oracle_reference(a.workload, a.iterations, nthreads=a.threads)
         print(f"oracle: {time.time() - t0:.0f} s", flush=True)
         if a.save_ref:
             np.save(a.save_ref, ref)
