@@ -8,17 +8,28 @@ namespace avionic {
 
 namespace {
 
-constexpr int NCC = 512;   // threads of a column-pass block
+#ifndef AVIONIC_NC_COLF_THREADS
+#define AVIONIC_NC_COLF_THREADS 256
+#endif
+#ifndef AVIONIC_NC_COLA_THREADS
+#define AVIONIC_NC_COLA_THREADS 512
+#endif
+#ifndef AVIONIC_NC_UCACHE
+#define AVIONIC_NC_UCACHE 0
+#endif
+constexpr int NCF = AVIONIC_NC_COLF_THREADS;  // threads of a forward column-pass block
+constexpr int NCC = AVIONIC_NC_COLA_THREADS;  // threads of an adjoint column-pass block
 constexpr int NCR = 256;   // threads of a row-pass / sample-pass block
 constexpr int NCE = 4096;  // complex elements of an LDS transform tile
-constexpr int kColSlotsNc = max_slots(NCE, NCC);
+constexpr bool kNcUcache = AVIONIC_NC_UCACHE != 0;  // image tile kept in registers across coils
+constexpr int kColSlotsNc = max_slots(NCE, NCF);
 constexpr int kRowSlotsNc = max_slots(NCE, NCR);
 
 __device__ inline int wrap_up(int v, int G) { return v >= G ? v - G : v; }
 
 // ------------------------------------------------------------ K, column pass
 struct NcColFwdLoad {
-  float2 ucache[kColSlotsNc];
+  float2 ucache[kNcUcache ? kColSlotsNc : 1];
   const float2 *ut, *us, *bc;
   const float *dxi, *dyi;
   int W, H, Gy, xb0;
@@ -29,6 +40,10 @@ struct NcColFwdLoad {
     const int y = wrap_up(i + H / 2, Gy);
     if (y >= H) return czero();
     const unsigned si = (unsigned)(y * W + x);
+    if (!kNcUcache) {
+      const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
+      return cmul(bc[si], cscale(v, dxi[x] * dyi[y]));
+    }
     if (first) {
       const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
       ucache[slot] = cscale(v, dxi[x] * dyi[y]);
@@ -45,7 +60,7 @@ struct NcColFwdStore {
   }
 };
 
-__global__ __launch_bounds__(NCC) void k_nc_col_fwd(NcGeom g, const float2 *__restrict__ u,
+__global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__restrict__ u,
                                                     const float2 *__restrict__ usub,
                                                     const float2 *__restrict__ b1,
                                                     float2 *__restrict__ tmp) {
@@ -54,7 +69,7 @@ __global__ __launch_bounds__(NCC) void k_nc_col_fwd(NcGeom g, const float2 *__re
   const int tid = threadIdx.x;
   const int t = blockIdx.y;
   const long HW = (long)g.H * g.W;
-  const float2 *tw = stage_twiddles(g.py, twl, tid, NCC);
+  const float2 *tw = stage_twiddles(g.py, twl, tid, NCF);
   __syncthreads();
   NcColFwdLoad ld;
   ld.ut = u + t * HW;
@@ -67,11 +82,14 @@ __global__ __launch_bounds__(NCC) void k_nc_col_fwd(NcGeom g, const float2 *__re
   ld.Gy = g.Gy;
   ld.xb0 = blockIdx.x * g.XBf;
   NcColFwdStore st{nullptr, g.W, (int)blockIdx.x * g.XBf};
-  for (int c = 0; c < g.C; ++c) {
+  // coils [c0, c1) of this block (blockIdx.z picks the coil group)
+  const int cpg = (g.C + gridDim.z - 1) / gridDim.z;
+  const int c0 = blockIdx.z * cpg, c1 = min(g.C, c0 + cpg);
+  for (int c = c0; c < c1; ++c) {
     ld.bc = b1 + c * HW;
-    ld.first = (c == 0);
+    ld.first = (c == c0);
     st.tc = tmp + ((long)t * g.C + c) * g.Gy * g.W;
-    fft_io<NCC, NCE, true, true, true>(tile, g.py, g.XBf, 1, g.XBf, -1.f, tid, tw, ld, st);
+    fft_io<NCF, NCE, true, true, true>(tile, g.py, g.XBf, 1, g.XBf, -1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
@@ -177,18 +195,31 @@ __global__ __launch_bounds__(NCR) void k_nc_samples(NcGeom g, const float2 *__re
 
 // --------------------------------------------------------- K^H, column pass
 struct NcColAdjLoad {
-  const int *cp;     // cell offsets of frame t
-  const int *cm;
-  const float *cw;
+  const int *cps;    // LDS: cell offsets of the tile's columns gx0..gx0+XB, row-major
+  const int2 *ce;    // (point, weight bits) of every cell entry
   const float2 *zp;  // z of the plane
-  int Gx, gx0;
+  int Gx, gx0, stride;
   __device__ float2 operator()(int b, int i, int) const {
     const int gx = gx0 + b;
     if (gx >= Gx) return czero();
-    const unsigned cell = (unsigned)(i * Gx + gx);
-    const int e0 = cp[cell], e1 = cp[cell + 1];
+    const int e0 = cps[i * stride + b], e1 = cps[i * stride + b + 1];
     float2 acc = czero();
-    for (int e = e0; e < e1; ++e) acc = cadd(acc, cscale(zp[cm[e]], cw[e]));
+    int e = e0;
+    // four entries per step: their index loads, then their z loads, are
+    // independent, so a dense cell waits for 2 latencies per 4 entries
+    // instead of 2 per entry; the sum keeps the entry order
+    for (; e + 4 <= e1; e += 4) {
+      const int2 a0 = ce[e], a1 = ce[e + 1], a2 = ce[e + 2], a3 = ce[e + 3];
+      const float2 z0 = zp[a0.x], z1 = zp[a1.x], z2 = zp[a2.x], z3 = zp[a3.x];
+      acc = cadd(acc, cscale(z0, __int_as_float(a0.y)));
+      acc = cadd(acc, cscale(z1, __int_as_float(a1.y)));
+      acc = cadd(acc, cscale(z2, __int_as_float(a2.y)));
+      acc = cadd(acc, cscale(z3, __int_as_float(a3.y)));
+    }
+    for (; e < e1; ++e) {
+      const int2 mw = ce[e];
+      acc = cadd(acc, cscale(zp[mw.x], __int_as_float(mw.y)));
+    }
     return acc;
   }
 };
@@ -207,14 +238,22 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
                                                     float2 *__restrict__ tmp) {
   __shared__ float2 tile[NCE];
   __shared__ float2 twl[kMaxFftLen];
+  extern __shared__ int cps[];  // (XBa + 1) * Gy ints, sized at launch
   const int tid = threadIdx.x;
   const int plane = blockIdx.y;
   const int t = plane / g.C;
   const float2 *tw = stage_twiddles(g.py, twl, tid, NCC);
-  __syncthreads();
   const int gx0 = blockIdx.x * g.XBa;
-  NcColAdjLoad ld{g.cptr + (long)t * ((long)g.Gx * g.Gy + 1), g.cm, g.cw, z + (long)plane * g.M,
-                  g.Gx, gx0};
+  // the tile's cell offsets (and the end offset of its last column), staged
+  // once so that each gather starts from LDS rather than a dependent load
+  const int stride = g.XBa + 1;
+  const int *cp = g.cptr + (long)t * ((long)g.Gx * g.Gy + 1);
+  for (int k = tid; k < stride * g.Gy; k += NCC) {
+    const int r = k / stride, c = k % stride;
+    if (gx0 + c <= g.Gx) cps[k] = cp[r * g.Gx + gx0 + c];
+  }
+  __syncthreads();
+  NcColAdjLoad ld{cps, g.ce, z + (long)plane * g.M, g.Gx, gx0, stride};
   NcColAdjStore st{tmp + (long)plane * g.H * g.Gx, g.H, g.Gx, g.Gy, gx0};
   fft_io<NCC, NCE, true, true, true>(tile, g.py, g.XBa, 1, g.XBa, +1.f, tid, tw, ld, st);
 }
@@ -283,8 +322,8 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
 
 void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub,
                             const float2 *b1, float2 *tmp, float2 *part, hipStream_t s) {
-  hipLaunchKernelGGL(k_nc_col_fwd, dim3((g.W + g.XBf - 1) / g.XBf, g.T), dim3(NCC), 0, s, g, u,
-                     usub, b1, tmp);
+  hipLaunchKernelGGL(k_nc_col_fwd, dim3((g.W + g.XBf - 1) / g.XBf, g.T, g.CG), dim3(NCF), 0, s, g,
+                     u, usub, b1, tmp);
   AV_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_nc_row_fwd, dim3((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C), dim3(NCR), 0, s, g,
                      (const float2 *)tmp, part);
@@ -312,8 +351,9 @@ int launch_nc_samples(int mode, const NcGeom &g, const float2 *part, float2 *out
 
 void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float2 *tmp,
                        float2 *img, hipStream_t s) {
-  hipLaunchKernelGGL(k_nc_col_adj, dim3((g.Gx + g.XBa - 1) / g.XBa, g.T * g.C), dim3(NCC), 0, s, g,
-                     z, tmp);
+  const size_t cps_bytes = sizeof(int) * (size_t)(g.XBa + 1) * g.Gy;
+  hipLaunchKernelGGL(k_nc_col_adj, dim3((g.Gx + g.XBa - 1) / g.XBa, g.T * g.C), dim3(NCC),
+                     cps_bytes, s, g, z, tmp);
   AV_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_nc_row_adj, dim3((g.H + g.RBa - 1) / g.RBa, g.T), dim3(NCR), 0, s, g,
                      (const float2 *)tmp, b1, img);

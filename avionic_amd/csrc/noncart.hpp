@@ -41,6 +41,7 @@ struct NcGeom {
   int XBa;            // grid columns per adjoint column-pass block
   int RBf;            // grid rows per forward row-pass block
   int RBa;            // image rows per adjoint row-pass block
+  int CG;             // coil groups of the forward column pass (blocks per tile)
   float scale;        // 1 / sqrt(HW)
   const float *dxi, *dyi;  // deapodisation 1 / psi_hat per image column / row
   const float *sw;         // [T][M] sqrt(density) / sqrt(HW)
@@ -49,8 +50,8 @@ struct NcGeom {
   const int *rptr;         // [T][Gy+1] row -> (point, tap) list offsets
   const int2 *rlist;       // (m, dy) of every point touching the row
   const int *cptr;         // [T][Gx*Gy+1] cell -> point list offsets
-  const int *cm;           // point index m of each cell entry
-  const float *cw;         // weight * sqrt(density) / sqrt(HW) of each entry
+  const int2 *ce;          // per cell entry: point index m, and the bits of its
+                           // weight * sqrt(density) / sqrt(HW)
   FftPlan px, py;          // lengths Gx, Gy
 };
 

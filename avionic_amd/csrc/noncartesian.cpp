@@ -34,6 +34,16 @@ double bessel_i0(double x) {
   return s;
 }
 
+// Launch shapes of the gridding passes (MI355X measurements pick them).
+#ifndef AVIONIC_NC_ADJ_ROWS
+#define AVIONIC_NC_ADJ_ROWS 4
+#endif
+#ifndef AVIONIC_NC_FWD_CG
+#define AVIONIC_NC_FWD_CG 4
+#endif
+constexpr int kNcAdjRows = AVIONIC_NC_ADJ_ROWS;      // image rows per adjoint row block, at most
+constexpr int kNcFwdCoilGroups = AVIONIC_NC_FWD_CG;  // coil groups of the forward column pass
+
 int pick_cols(int L, int n) {
   int xb = 1;
   while (xb * 2 * L <= 4096 && xb * 2 <= 64 && xb < n) xb *= 2;
@@ -92,8 +102,7 @@ class NoncartesianOperator : public Operator {
     std::vector<int> rptr((size_t)T * (Gy + 1));
     std::vector<int2> rlist;
     std::vector<int> cptr((size_t)T * (G2 + 1));
-    std::vector<int> cm;
-    std::vector<float> cw;
+    std::vector<int2> ce;
     rlist.reserve((size_t)T * M * kNcTaps);
     for (int t = 0; t < T; ++t) {
       std::vector<double> wxd((size_t)M * kNcTaps), wyd((size_t)M * kNcTaps);
@@ -139,10 +148,9 @@ class NoncartesianOperator : public Operator {
             if (wyd[m * kNcTaps + dy] * wxd[m * kNcTaps + dx] != 0.0)
               count[wrap(qy[m] + dy, Gy) * Gx + wrap(qx[m] + dx, Gx)]++;
       int *cp = &cptr[(size_t)t * (G2 + 1)];
-      cp[0] = (int)cm.size();
+      cp[0] = (int)ce.size();
       for (long g = 0; g < G2; ++g) cp[g + 1] = cp[g] + count[g];
-      cm.resize(cp[G2]);
-      cw.resize(cp[G2]);
+      ce.resize(cp[G2]);
       std::vector<int> fill(G2, 0);
       for (long m = 0; m < M; ++m)
         for (int dy = 0; dy < kNcTaps; ++dy)
@@ -151,8 +159,8 @@ class NoncartesianOperator : public Operator {
             if (w == 0.0) continue;
             const long cell = wrap(qy[m] + dy, Gy) * Gx + wrap(qx[m] + dx, Gx);
             const int pos = cp[cell] + fill[cell]++;
-            cm[pos] = (int)m;
-            cw[pos] = (float)(w * std::sqrt((double)dn[(size_t)t * M + m]) * sc);
+            const float wf = (float)(w * std::sqrt((double)dn[(size_t)t * M + m]) * sc);
+            ce[pos] = make_int2((int)m, __builtin_bit_cast(int, wf));
           }
     }
 
@@ -169,8 +177,7 @@ class NoncartesianOperator : public Operator {
     up(rptr_, rptr.data(), sizeof(int) * rptr.size());
     up(rlist_, rlist.data(), sizeof(int2) * rlist.size());
     up(cptr_, cptr.data(), sizeof(int) * cptr.size());
-    up(cm_, cm.data(), sizeof(int) * cm.size());
-    up(cw_, cw.data(), sizeof(float) * cw.size());
+    up(ce_, ce.data(), sizeof(int2) * ce.size());
     make_fft_plan(Gx, g_.px, &twx_.p);
     make_fft_plan(Gy, g_.py, &twy_.p);
 
@@ -179,7 +186,8 @@ class NoncartesianOperator : public Operator {
     g_.XBf = pick_cols(Gy, W);
     g_.XBa = pick_cols(Gy, Gx);
     g_.RBf = std::max(1, 4096 / Gx);
-    g_.RBa = std::max(1, 4096 / Gx);
+    g_.RBa = std::max(1, std::min(4096 / Gx, kNcAdjRows));
+    g_.CG = std::max(1, std::min(C, kNcFwdCoilGroups));
     g_.scale = (float)sc;
     g_.dxi = dxi_.as<float>();
     g_.dyi = dyi_.as<float>();
@@ -190,8 +198,7 @@ class NoncartesianOperator : public Operator {
     g_.rptr = rptr_.as<int>();
     g_.rlist = rlist_.as<int2>();
     g_.cptr = cptr_.as<int>();
-    g_.cm = cm_.as<int>();
-    g_.cw = cw_.as<float>();
+    g_.ce = ce_.as<int2>();
   }
 
   long dense_elems() const override { return (long)T * C * g_.M; }
@@ -239,7 +246,7 @@ class NoncartesianOperator : public Operator {
  private:
   int spf_;
   NcGeom g_{};
-  DevBuf dxi_, dyi_, sw_, q0_, wx_, wy_, rptr_, rlist_, cptr_, cm_, cw_, twx_, twy_, tmp_, part_;
+  DevBuf dxi_, dyi_, sw_, q0_, wx_, wy_, rptr_, rlist_, cptr_, ce_, twx_, twy_, tmp_, part_;
 
   void scratch() {
     const size_t planes = (size_t)T * C;
