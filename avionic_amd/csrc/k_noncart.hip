@@ -122,22 +122,19 @@ __global__ __launch_bounds__(NCR) void k_nc_row_fwd(NcGeom g, const float2 *__re
   // tile[b * Gx + i] holds grid row gy0 + b in natural order.
   const int *rp = g.rptr + (long)t * (g.Gy + 1);
   const int e0 = rp[gy0], e1 = rp[gy0 + nr];
-  const long pm = (long)t * g.M;
   float2 *pp = part + (long)plane * g.M * kNcTaps;
   for (int e = e0 + tid; e < e1; e += NCR) {
-    const int2 md = g.rlist[e];
-    const int m = md.x, dy = md.y;
-    const int2 q = g.q0[pm + m];
-    const int b = wrap_up(q.y + dy, g.Gy) - gy0;
-    const float4 wx = g.wx[pm + m];
-    const float4 wy = g.wy[pm + m];
-    const float2 *row = tile + b * Gx;
-    float2 acc = cscale(row[q.x], wx.x);
-    acc = cadd(acc, cscale(row[wrap_up(q.x + 1, Gx)], wx.y));
-    acc = cadd(acc, cscale(row[wrap_up(q.x + 2, Gx)], wx.z));
-    acc = cadd(acc, cscale(row[wrap_up(q.x + 3, Gx)], wx.w));
-    const float wyd = dy == 0 ? wy.x : dy == 1 ? wy.y : dy == 2 ? wy.z : wy.w;
-    pp[(unsigned)(m * kNcTaps + dy)] = cscale(acc, wyd);
+    // (partial index m*4 + dy, grid row, first grid column, row weight) and
+    // the column weights: no load here depends on another
+    const int4 h = g.rhdr[e];
+    const float4 wx = g.rwx[e];
+    const float2 *row = tile + (h.y - gy0) * Gx;
+    const int qx = h.z;
+    float2 acc = cscale(row[qx], wx.x);
+    acc = cadd(acc, cscale(row[wrap_up(qx + 1, Gx)], wx.y));
+    acc = cadd(acc, cscale(row[wrap_up(qx + 2, Gx)], wx.z));
+    acc = cadd(acc, cscale(row[wrap_up(qx + 3, Gx)], wx.w));
+    pp[(unsigned)h.x] = cscale(acc, __int_as_float(h.w));
   }
 }
 

@@ -45,10 +45,10 @@ struct NcGeom {
   float scale;        // 1 / sqrt(HW)
   const float *dxi, *dyi;  // deapodisation 1 / psi_hat per image column / row
   const float *sw;         // [T][M] sqrt(density) / sqrt(HW)
-  const int2 *q0;          // [T][M] first grid column / row of the point (wrapped)
-  const float4 *wx, *wy;   // [T][M] separable weights of the kNcTaps taps
   const int *rptr;         // [T][Gy+1] row -> (point, tap) list offsets
-  const int2 *rlist;       // (m, dy) of every point touching the row
+  const int4 *rhdr;        // per entry: m * kNcTaps + dy, grid row, first grid column
+                           // (wrapped), bits of the row weight wy[dy]
+  const float4 *rwx;       // per entry: the point's kNcTaps column weights
   const int *cptr;         // [T][Gx*Gy+1] cell -> point list offsets
   const int2 *ce;          // per cell entry: point index m, and the bits of its
                            // weight * sqrt(density) / sqrt(HW)

@@ -100,10 +100,12 @@ class NoncartesianOperator : public Operator {
     std::vector<int2> q0((size_t)T * M);
     std::vector<float4> wx4((size_t)T * M), wy4((size_t)T * M);
     std::vector<int> rptr((size_t)T * (Gy + 1));
-    std::vector<int2> rlist;
+    std::vector<int4> rhdr;
+    std::vector<float4> rwx;
     std::vector<int> cptr((size_t)T * (G2 + 1));
     std::vector<int2> ce;
-    rlist.reserve((size_t)T * M * kNcTaps);
+    rhdr.reserve((size_t)T * M * kNcTaps);
+    rwx.reserve((size_t)T * M * kNcTaps);
     for (int t = 0; t < T; ++t) {
       std::vector<double> wxd((size_t)M * kNcTaps), wyd((size_t)M * kNcTaps);
       std::vector<long> qx((size_t)M), qy((size_t)M);
@@ -136,10 +138,15 @@ class NoncartesianOperator : public Operator {
           rows[wrap(qy[m] + dy, Gy)].push_back(make_int2((int)m, dy));
       int *rp = &rptr[(size_t)t * (Gy + 1)];
       for (int r = 0; r < Gy; ++r) {
-        rp[r] = (int)rlist.size();
-        rlist.insert(rlist.end(), rows[r].begin(), rows[r].end());
+        rp[r] = (int)rhdr.size();
+        for (const int2 &md : rows[r]) {
+          const size_t pi = (size_t)t * M + md.x;
+          const float wyd = (&wy4[pi].x)[md.y];
+          rhdr.push_back(make_int4(md.x * kNcTaps + md.y, r, q0[pi].x, __builtin_bit_cast(int, wyd)));
+          rwx.push_back(wx4[pi]);
+        }
       }
-      rp[Gy] = (int)rlist.size();
+      rp[Gy] = (int)rhdr.size();
       // cell -> (point, weight) for the spreading, points in increasing order
       std::vector<int> count(G2, 0);
       for (long m = 0; m < M; ++m)
@@ -171,11 +178,9 @@ class NoncartesianOperator : public Operator {
     up(dxi_, dxi.data(), sizeof(float) * dxi.size());
     up(dyi_, dyi.data(), sizeof(float) * dyi.size());
     up(sw_, sw.data(), sizeof(float) * sw.size());
-    up(q0_, q0.data(), sizeof(int2) * q0.size());
-    up(wx_, wx4.data(), sizeof(float4) * wx4.size());
-    up(wy_, wy4.data(), sizeof(float4) * wy4.size());
     up(rptr_, rptr.data(), sizeof(int) * rptr.size());
-    up(rlist_, rlist.data(), sizeof(int2) * rlist.size());
+    up(rhdr_, rhdr.data(), sizeof(int4) * rhdr.size());
+    up(rwx_, rwx.data(), sizeof(float4) * rwx.size());
     up(cptr_, cptr.data(), sizeof(int) * cptr.size());
     up(ce_, ce.data(), sizeof(int2) * ce.size());
     make_fft_plan(Gx, g_.px, &twx_.p);
@@ -192,11 +197,9 @@ class NoncartesianOperator : public Operator {
     g_.dxi = dxi_.as<float>();
     g_.dyi = dyi_.as<float>();
     g_.sw = sw_.as<float>();
-    g_.q0 = q0_.as<int2>();
-    g_.wx = wx_.as<float4>();
-    g_.wy = wy_.as<float4>();
     g_.rptr = rptr_.as<int>();
-    g_.rlist = rlist_.as<int2>();
+    g_.rhdr = rhdr_.as<int4>();
+    g_.rwx = rwx_.as<float4>();
     g_.cptr = cptr_.as<int>();
     g_.ce = ce_.as<int2>();
   }
@@ -246,7 +249,7 @@ class NoncartesianOperator : public Operator {
  private:
   int spf_;
   NcGeom g_{};
-  DevBuf dxi_, dyi_, sw_, q0_, wx_, wy_, rptr_, rlist_, cptr_, ce_, twx_, twy_, tmp_, part_;
+  DevBuf dxi_, dyi_, sw_, rptr_, rhdr_, rwx_, cptr_, ce_, twx_, twy_, tmp_, part_;
 
   void scratch() {
     const size_t planes = (size_t)T * C;
