@@ -97,6 +97,9 @@ def _declare(L):
     sig = {
         "avionic_last_error": (ctypes.c_char_p, []),
         "avionic_version": (ctypes.c_char_p, []),
+        "avionic_bin_read": (_I, [ctypes.c_char_p, _P, _P, _SZ, ctypes.POINTER(_SZ),
+                                  ctypes.POINTER(_I)]),
+        "avionic_bin_write": (_I, [ctypes.c_char_p, _P, _P, _SZ]),
         "avionic_cartesian_operator_create": (_I, [_U, _U, _U, _U, _P, _I, ctypes.POINTER(_P)]),
         "avionic_noncartesian_operator_create": (_I, [_U, _U, _U, _U, _U, _U, _P, _P, _I, _D,
                                                       ctypes.POINTER(_P)]),
@@ -464,6 +467,30 @@ def ictgv2_norm(u, v, w, v2, alpha0, alpha1, alpha, dx, dy, dt, dx2, dy2, dt2):
                                      float(alpha1), float(alpha), dx, dy, dt, dx2, dy2, dt2,
                                      ctypes.byref(out), _stream()))
     return out.value
+
+
+def read_bin(path):
+    """Read a reference .bin vector file (uint8 iscomplex, uint32 n, n float64
+    real parts, [n float64 imaginary parts]; matlab/readbin_vector.m:30-40)
+    into a float64 / complex128 numpy array."""
+    L = lib()
+    n, cplx = _SZ(), ctypes.c_int()
+    _check(L.avionic_bin_read(os.fsencode(path), None, None, 0, ctypes.byref(n), ctypes.byref(cplx)))
+    re_ = np.zeros(n.value, np.float64)
+    im_ = np.zeros(n.value, np.float64)
+    _check(L.avionic_bin_read(os.fsencode(path), re_.ctypes.data_as(_P), im_.ctypes.data_as(_P),
+                              n.value, ctypes.byref(n), ctypes.byref(cplx)))
+    return re_ + 1j * im_ if cplx.value else re_
+
+
+def write_bin(path, v):
+    """Write a vector as a reference .bin file (complex if v is complex;
+    matlab/writebin_vector.m:35-44)."""
+    v = np.asarray(v).ravel()
+    re_ = np.ascontiguousarray(v.real, np.float64)
+    im_ = np.ascontiguousarray(v.imag, np.float64) if np.iscomplexobj(v) else None
+    _check(lib().avionic_bin_write(os.fsencode(path), re_.ctypes.data_as(_P),
+                                   None if im_ is None else im_.ctypes.data_as(_P), re_.size))
 
 
 def ellipke(m):

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libavionic.so")
+CLI = os.path.join(HERE, "avionic")
 ARCH = os.environ.get("AVIONIC_ARCH", "gfx950")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
@@ -61,7 +62,23 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    _build_cli()
+    if verbose:
+        print("built", CLI)
     return LIB
+
+
+def _build_cli() -> None:
+    """The `avionic` command line (csrc/cli), linked against libavionic.so
+    and found next to it at run time."""
+    src = os.path.join(CSRC, "cli", "avionic_cli.cpp")
+    tmp = CLI + ".tmp"
+    cmd = [hipcc(), src, "-o", tmp] + COMMON + ["-L" + HERE, "-lavionic",
+                                                 "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,452 @@
+// `avionic` command line: the reference CLI's reconstruction path
+// (CUDA/src/main.cpp:332-947, options_parser.cpp:79-330) over the C-ABI of
+// libavionic.
+//
+//   avionic [options] -d w:h:d:nRO:nEnc1:nEnc2:c:f <kdata.bin> <mask|traj.bin> <output.bin>
+//
+// Supported: methods TV, TGV2, ICTGV2; Cartesian and (-n) non-Cartesian data;
+// coil maps (-s) and initial image (-u); the .cfg parameter file (-p) with
+// the reference's keys; -i maxIt, -j stopPDGap, -a adaptlambda, -w density,
+// -e extra results, -v verbose, -g debugstep.  Not in this build (it stops
+// with a message): raw-data import (-r), normalisation (-o), coil-map
+// estimation when -s is missing, and the other methods.
+// --dry-run reads and checks every input and prints the resolved settings as
+// JSON without touching the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/avionic.h"
+
+namespace {
+
+struct Dims {
+  unsigned width = 0, height = 0, depth = 0, readouts = 0, enc1 = 0, enc2 = 0, coils = 0,
+           frames = 0;
+};
+
+struct Options {
+  std::string kdata, mask, output, params, sens, u0, dens, method = "ICTGV2";
+  Dims dims;
+  bool have_dims = false, nonuniform = false, extradata = false, adaptlambda = false,
+       verbose = false, dry_run = false, normalize = false, rawdata = false, method_set = false;
+  int maxIt = -1;  // -1: from the config file / defaults
+  double stopPDGap = -1.0;
+  int debugstep = 10;
+  std::map<std::string, std::string> cfg;  // "section.key" -> value
+};
+
+[[noreturn]] void die(const std::string &msg, int code = 2) {
+  std::fprintf(stderr, "avionic: %s\n", msg.c_str());
+  std::exit(code);
+}
+
+void check(int rc, const char *what) {
+  if (rc != AVIONIC_OK) die(std::string(what) + ": " + avionic_last_error(), 1);
+}
+
+std::string trim(const std::string &s) {
+  const size_t a = s.find_first_not_of(" \t\r\n");
+  if (a == std::string::npos) return "";
+  const size_t b = s.find_last_not_of(" \t\r\n");
+  return s.substr(a, b - a + 1);
+}
+
+std::string upper(std::string s) {
+  for (char &c : s) c = (char)std::toupper((unsigned char)c);
+  return s;
+}
+
+// options_parser.cpp:52-69: width:height:depth:readouts:encodings_y:encodings_z:coils:frames
+Dims parse_dims(const std::string &s) {
+  std::vector<unsigned> v;
+  std::stringstream ss(s);
+  std::string tok;
+  while (std::getline(ss, tok, ':')) {
+    char *end = nullptr;
+    const unsigned long x = std::strtoul(tok.c_str(), &end, 10);
+    if (tok.empty() || *end) die("invalid dimensions '" + s + "'");
+    v.push_back((unsigned)x);
+  }
+  if (v.size() != 8) die("invalid dimensions '" + s + "' (want w:h:d:nRO:nEnc1:nEnc2:c:f)");
+  Dims d;
+  d.width = v[0]; d.height = v[1]; d.depth = v[2]; d.readouts = v[3];
+  d.enc1 = v[4]; d.enc2 = v[5]; d.coils = v[6]; d.frames = v[7];
+  return d;
+}
+
+// Configuration file (boost program_options syntax, config/default.cfg):
+// "key = value" lines, "[section]" headers, '#' comments.
+void read_cfg(const std::string &path, std::map<std::string, std::string> &cfg) {
+  std::ifstream in(path);
+  if (!in) die("cannot open parameter file " + path);
+  std::string line, section;
+  while (std::getline(in, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line = line.substr(0, hash);
+    line = trim(line);
+    if (line.empty()) continue;
+    if (line.front() == '[') {
+      if (line.back() != ']') die("malformed section header in " + path + ": " + line);
+      section = trim(line.substr(1, line.size() - 2));
+      continue;
+    }
+    const size_t eq = line.find('=');
+    if (eq == std::string::npos) die("malformed line in " + path + ": " + line);
+    const std::string key = trim(line.substr(0, eq)), val = trim(line.substr(eq + 1));
+    cfg[section.empty() ? key : section + "." + key] = val;
+  }
+}
+
+Options parse_args(int argc, char **argv) {
+  Options o;
+  std::vector<std::string> pos;
+  auto need = [&](int &i) -> std::string {
+    if (i + 1 >= argc) die(std::string("option ") + argv[i] + " needs a value");
+    return argv[++i];
+  };
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    if (a == "-h" || a == "--help") {
+      std::printf(
+          "Usage: avionic [options] -d w:h:d:nRO:nEnc1:nEnc2:c:f <kdata> <mask/traj> <output>\n"
+          "  -m, --method TV|TGV2|ICTGV2   -i, --maxIt N     -j, --stopPDGap G\n"
+          "  -p, --params file.cfg         -s, --sens b1.bin -u, --uzero u0.bin\n"
+          "  -n, --nonuniform  -w, --dens dcf.bin  -a, --adaptlambda  -e, --extradata\n"
+          "  -v, --verbose     -g, --debugstep N   --dry-run\n");
+      std::exit(0);
+    } else if (a == "-d" || a == "--dims") {
+      o.dims = parse_dims(need(i));
+      o.have_dims = true;
+    } else if (a == "-m" || a == "--method") {
+      o.method = upper(need(i));
+      o.method_set = true;
+    } else if (a == "-i" || a == "--maxIt") {
+      o.maxIt = std::atoi(need(i).c_str());
+    } else if (a == "-j" || a == "--stopPDGap") {
+      o.stopPDGap = std::atof(need(i).c_str());
+    } else if (a == "-p" || a == "--params") {
+      o.params = need(i);
+    } else if (a == "-s" || a == "--sens") {
+      o.sens = need(i);
+    } else if (a == "-u" || a == "--uzero") {
+      o.u0 = need(i);
+    } else if (a == "-w" || a == "--dens") {
+      o.dens = need(i);
+    } else if (a == "-g" || a == "--debugstep") {
+      o.debugstep = std::atoi(need(i).c_str());
+    } else if (a == "-n" || a == "--nonuniform") {
+      o.nonuniform = true;
+    } else if (a == "-e" || a == "--extradata") {
+      o.extradata = true;
+    } else if (a == "-a" || a == "--adaptlambda") {
+      o.adaptlambda = true;
+    } else if (a == "-v" || a == "--verbose") {
+      o.verbose = true;
+    } else if (a == "-o" || a == "--normalize") {
+      o.normalize = true;
+    } else if (a == "-r" || a == "--rawdata") {
+      o.rawdata = true;
+    } else if (a == "--dry-run") {
+      o.dry_run = true;
+    } else if (!a.empty() && a[0] == '-') {
+      die("unknown or unsupported option " + a);
+    } else {
+      pos.push_back(a);
+    }
+  }
+  if (o.rawdata) die("raw-data import (-r) is not part of this build", 3);
+  if (o.normalize) die("data normalisation (-o) needs the coil-map estimation, which is not part of this build", 3);
+  if (pos.size() != 3) die("expected <kdata> <mask/traj> <output> (got " + std::to_string(pos.size()) + ")");
+  o.kdata = pos[0];
+  o.mask = pos[1];
+  o.output = pos[2];
+  if (!o.have_dims) die("-d w:h:d:nRO:nEnc1:nEnc2:c:f is required");
+  if (!o.params.empty()) read_cfg(o.params, o.cfg);
+  // the config file may also name the method / iterations (default.cfg:6-7)
+  if (!o.method_set && o.cfg.count("method")) o.method = upper(o.cfg["method"]);
+  return o;
+}
+
+avionic_method method_of(const std::string &m) {
+  if (m == "TV") return AVIONIC_TV;
+  if (m == "TGV2") return AVIONIC_TGV2;
+  if (m == "ICTGV2") return AVIONIC_ICTGV2;
+  die("method " + m + " is not part of this build (TV, TGV2, ICTGV2)", 3);
+}
+
+// Config section keys of one method onto the parameter struct
+// (options_parser.cpp:190-275).
+void apply_cfg(const Options &o, const std::string &sec, avionic_params &p) {
+  auto get = [&](const char *k, double &dst) {
+    auto it = o.cfg.find(sec + "." + k);
+    if (it != o.cfg.end()) dst = std::atof(it->second.c_str());
+  };
+  get("dx", p.dx); get("dy", p.dy); get("dt", p.dt);
+  get("sigma", p.sigma); get("tau", p.tau); get("sigmaTauRatio", p.sigmaTauRatio);
+  get("timeSpaceWeight", p.timeSpaceWeight); get("lambda", p.lambda);
+  get("alpha0", p.alpha0); get("alpha1", p.alpha1); get("alpha", p.alpha);
+  get("timeSpaceWeight2", p.timeSpaceWeight2);
+  get("dx2", p.dx2); get("dy2", p.dy2); get("dt2", p.dt2);
+  auto top = [&](const char *k, double &dst) {
+    auto it = o.cfg.find(k);
+    if (it != o.cfg.end()) dst = std::atof(it->second.c_str());
+  };
+  double maxIt = p.maxIt, stop = p.stopPDGap;
+  top("maxIt", maxIt);
+  top("stopPDGap", stop);
+  p.maxIt = (unsigned)maxIt;
+  p.stopPDGap = stop;
+  if (o.maxIt >= 0) p.maxIt = (unsigned)o.maxIt;
+  if (o.stopPDGap >= 0) p.stopPDGap = o.stopPDGap;
+  // -a: lambda from the [adaptlambda] k, d of the file (default 0, 1:
+  // options_parser.cpp:315-321); otherwise the configured lambda as is
+  p.adaptLambda = o.adaptlambda ? 1 : 0;
+  p.adaptLambdaK = 0.0;
+  p.adaptLambdaD = 1.0;
+  top("adaptlambda.k", p.adaptLambdaK);
+  top("adaptlambda.d", p.adaptLambdaD);
+}
+
+struct HostVec {
+  std::vector<double> re, im;
+  bool complex = false;
+  size_t n = 0;
+};
+
+HostVec load(const std::string &path, const char *what) {
+  HostVec v;
+  int cplx = 0;
+  check(avionic_bin_read(path.c_str(), nullptr, nullptr, 0, &v.n, &cplx), what);
+  v.complex = cplx != 0;
+  v.re.resize(v.n);
+  v.im.resize(v.n);
+  check(avionic_bin_read(path.c_str(), v.re.data(), v.im.data(), v.n, &v.n, &cplx), what);
+  return v;
+}
+
+std::vector<float> to_c64(const HostVec &v) {
+  std::vector<float> out(2 * v.n);
+  for (size_t i = 0; i < v.n; ++i) {
+    out[2 * i] = (float)v.re[i];
+    out[2 * i + 1] = (float)v.im[i];
+  }
+  return out;
+}
+
+std::string dir_of(const std::string &path) {
+  const size_t s = path.find_last_of('/');
+  return s == std::string::npos ? std::string(".") : path.substr(0, s);
+}
+
+struct Dev {
+  void *p = nullptr;
+  explicit Dev(size_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) die("device allocation failed", 1);
+  }
+  ~Dev() {
+    if (p) (void)hipFree(p);
+  }
+  float *f() const { return (float *)p; }
+};
+
+void upload(Dev &d, const std::vector<float> &h) {
+  if (hipMemcpy(d.p, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+    die("host-to-device copy failed", 1);
+}
+
+void export_cb(const char *name, const float *data, size_t count, void *user) {
+  const std::string path = *(const std::string *)user + "/" + name;
+  std::vector<double> re(count), im(count);
+  for (size_t i = 0; i < count; ++i) {
+    re[i] = data[2 * i];
+    im[i] = data[2 * i + 1];
+  }
+  if (avionic_bin_write(path.c_str(), re.data(), im.data(), count) != AVIONIC_OK)
+    std::fprintf(stderr, "avionic: could not write %s: %s\n", path.c_str(), avionic_last_error());
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  Options o = parse_args(argc, argv);
+  const Dims &d = o.dims;
+  if (d.depth > 1) die("3D data (depth > 1) is not part of this build", 3);
+  const avionic_method method = method_of(o.method);
+  const unsigned W = d.width, H = d.height, C = d.coils, T = d.frames;
+  if (!W || !H || !C || !T) die("width, height, coils and frames must be positive");
+  const size_t HW = (size_t)W * H, N = HW * T;
+  const size_t M = o.nonuniform ? (size_t)d.readouts * d.enc1 : 0;
+  const size_t K = o.nonuniform ? M * C * T : HW * C * T;
+
+  avionic_params p;
+  check(avionic_default_params(method, &p), "default parameters");
+  const char *sec = method == AVIONIC_TV ? "tv" : method == AVIONIC_TGV2 ? "tgv2" : "ictgv2";
+  apply_cfg(o, sec, p);
+  if (!p.adaptLambda && !(p.lambda > 0.0))
+    die("no lambda: set " + std::string(sec) + ".lambda in the parameter file (-p) or pass -a");
+  int kw = 3;
+  double osf = 2.0;  // options_parser.cpp:300-309 defaults
+  if (o.cfg.count("gpunufft.kernelWidth")) kw = (int)std::atof(o.cfg["gpunufft.kernelWidth"].c_str());
+  if (o.cfg.count("gpunufft.osf")) osf = std::atof(o.cfg["gpunufft.osf"].c_str());
+
+  // ---- inputs
+  HostVec kd = load(o.kdata, "k-space data");
+  if (kd.n != K) die("k-space data has " + std::to_string(kd.n) + " values, dims imply " + std::to_string(K));
+  HostVec mk = load(o.mask, o.nonuniform ? "trajectory" : "mask");
+  const size_t mask_n = o.nonuniform ? 2 * M * T : N;
+  if (mk.n != mask_n)
+    die(std::string(o.nonuniform ? "trajectory" : "mask") + " has " + std::to_string(mk.n) +
+        " values, dims imply " + std::to_string(mask_n));
+  HostVec b1, u0, w;
+  const bool have_b1 = !o.sens.empty();
+  if (!have_b1)
+    die("coil maps (-s) are required: the coil-map estimation is not part of this build", 3);
+  b1 = load(o.sens, "coil maps");
+  if (b1.n != HW * C) die("coil maps have " + std::to_string(b1.n) + " values, dims imply " + std::to_string(HW * C));
+  if (!o.u0.empty()) {
+    u0 = load(o.u0, "initial image");
+    if (u0.n != HW && u0.n != N) die("initial image must hold w*h or w*h*frames values");
+  }
+  if (o.nonuniform) {
+    if (o.dens.empty()) die("non-Cartesian data needs the density compensation (-w)");
+    w = load(o.dens, "density compensation");
+    if (w.n != M * T) die("density compensation has " + std::to_string(w.n) + " values, dims imply " + std::to_string(M * T));
+  }
+
+  if (o.dry_run) {
+    std::printf(
+        "{\"method\": \"%s\", \"width\": %u, \"height\": %u, \"coils\": %u, \"frames\": %u, "
+        "\"nonuniform\": %s, \"points_per_frame\": %zu, \"maxIt\": %u, \"stopPDGap\": %g, "
+        "\"sigma\": %g, \"tau\": %g, \"sigmaTauRatio\": %g, \"timeSpaceWeight\": %g, "
+        "\"timeSpaceWeight2\": %g, \"lambda\": %g, \"adaptLambda\": %d, \"adaptLambdaK\": %g, "
+        "\"adaptLambdaD\": %g, \"alpha0\": %g, \"alpha1\": %g, \"alpha\": %g, "
+        "\"kernelWidth\": %d, \"osf\": %g, \"initial_image\": \"%s\", \"output\": \"%s\"}\n",
+        o.method.c_str(), W, H, C, T, o.nonuniform ? "true" : "false", M, p.maxIt, p.stopPDGap,
+        p.sigma, p.tau, p.sigmaTauRatio, p.timeSpaceWeight, p.timeSpaceWeight2, p.lambda,
+        p.adaptLambda, p.adaptLambdaK, p.adaptLambdaD, p.alpha0, p.alpha1, p.alpha, kw, osf,
+        o.u0.empty() ? (o.nonuniform ? "zero" : "time-averaged") : "file", o.output.c_str());
+    return 0;
+  }
+
+  // ---- data preparation (main.cpp:509-579)
+  std::vector<float> kdh = to_c64(kd);
+  std::vector<float> maskh(mk.n);
+  for (size_t i = 0; i < mk.n; ++i) maskh[i] = (float)mk.re[i];
+  std::vector<float> wh;
+  if (!o.nonuniform) {
+    // zero the unsampled k-space of every frame and coil (main.cpp:509-519)
+    for (size_t t = 0; t < T; ++t)
+      for (size_t c = 0; c < C; ++c)
+        for (size_t i = 0; i < HW; ++i) {
+          const float m = maskh[t * HW + i];
+          float *z = &kdh[2 * ((t * C + c) * HW + i)];
+          z[0] *= m;
+          z[1] *= m;
+        }
+  } else {
+    // sqrt(density) weighting of the data (main.cpp:546-570); the operator
+    // applies sqrt(density) to its own output
+    wh.resize(w.n);
+    for (size_t i = 0; i < w.n; ++i) wh[i] = (float)w.re[i];
+    for (size_t t = 0; t < T; ++t)
+      for (size_t c = 0; c < C; ++c)
+        for (size_t m = 0; m < M; ++m) {
+          const float s = std::sqrt(std::max(0.f, wh[t * M + m]));
+          float *z = &kdh[2 * ((t * C + c) * M + m)];
+          z[0] *= s;
+          z[1] *= s;
+        }
+  }
+
+  Dev dk(sizeof(float) * kdh.size()), dmask(sizeof(float) * maskh.size()),
+      db1(sizeof(float) * 2 * HW * C), dx(sizeof(float) * 2 * N), dw(sizeof(float) * (wh.size() + 1));
+  upload(dk, kdh);
+  upload(dmask, maskh);
+  upload(db1, to_c64(b1));
+  if (!wh.empty()) upload(dw, wh);
+
+  avionic_operator *op = nullptr;
+  if (o.nonuniform)
+    check(avionic_noncartesian_operator_create(W, H, C, T, d.readouts, d.enc1, dmask.f(), dw.f(), kw,
+                                               osf, &op),
+          "non-Cartesian operator");
+  else
+    check(avionic_cartesian_operator_create(W, H, C, T, dmask.f(), 0, &op), "Cartesian operator");
+
+  // ---- initial image: the file (one frame replicated, main.cpp:772-778), or
+  // for Cartesian data the time-averaged zero-filled image
+  // u0 = sum_c conj(b1_c) F(sum_t d_t / max(1, sum_t M_t)) (get_sos_crec_opt.m:7-15)
+  std::vector<float> x0(2 * N, 0.f);
+  if (!o.u0.empty()) {
+    const std::vector<float> u = to_c64(u0);
+    for (size_t t = 0; t < T; ++t)
+      for (size_t i = 0; i < 2 * HW; ++i) x0[2 * t * HW + i] = u[(u0.n == N ? 2 * t * HW : 0) + i];
+  } else if (!o.nonuniform) {
+    std::vector<float> avg(2 * HW * C, 0.f), cnt(HW, 0.f);
+    for (size_t t = 0; t < T; ++t)
+      for (size_t i = 0; i < HW; ++i) cnt[i] += maskh[t * HW + i];
+    for (size_t t = 0; t < T; ++t)
+      for (size_t c = 0; c < C; ++c)
+        for (size_t i = 0; i < HW; ++i) {
+          avg[2 * (c * HW + i)] += kdh[2 * ((t * C + c) * HW + i)];
+          avg[2 * (c * HW + i) + 1] += kdh[2 * ((t * C + c) * HW + i) + 1];
+        }
+    for (size_t c = 0; c < C; ++c)
+      for (size_t i = 0; i < HW; ++i) {
+        const float s = 1.f / std::max(1.f, cnt[i]);
+        avg[2 * (c * HW + i)] *= s;
+        avg[2 * (c * HW + i) + 1] *= s;
+      }
+    avionic_operator *op1 = nullptr;
+    check(avionic_cartesian_operator_create(W, H, C, 1, nullptr, 0, &op1), "averaging operator");
+    Dev davg(sizeof(float) * avg.size()), du(sizeof(float) * 2 * HW);
+    upload(davg, avg);
+    check(avionic_operator_forward(op1, davg.f(), du.f(), db1.f(), nullptr), "initial image");
+    std::vector<float> u(2 * HW);
+    if (hipMemcpy(u.data(), du.p, sizeof(float) * u.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      die("device-to-host copy failed", 1);
+    avionic_operator_destroy(op1);
+    for (size_t t = 0; t < T; ++t)
+      std::memcpy(&x0[2 * t * HW], u.data(), sizeof(float) * 2 * HW);
+  }
+  upload(dx, x0);
+
+  // ---- reconstruction (main.cpp:780-789)
+  avionic_recon *rec = nullptr;
+  check(avionic_recon_create(method, W, H, C, T, &p, op, &rec), "solver");
+  check(avionic_recon_set_verbose(rec, o.verbose ? 1 : 0), "verbose");
+  check(avionic_recon_set_debug(rec, o.debugstep > 0 ? 1 : 0, o.debugstep), "debug");
+  if (o.verbose) std::printf("avionic: %s %ux%u x %u frames x %u coils, %u iterations\n",
+                             o.method.c_str(), W, H, T, C, p.maxIt);
+  check(avionic_recon_run(rec, dk.f(), dx.f(), db1.f(), nullptr), "reconstruction");
+
+  // ---- output (main.cpp:831-839, 919-921)
+  std::vector<float> xh(2 * N);
+  if (hipMemcpy(xh.data(), dx.p, sizeof(float) * xh.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    die("device-to-host copy failed", 1);
+  std::vector<double> re(N), im(N);
+  for (size_t i = 0; i < N; ++i) {
+    re[i] = xh[2 * i];
+    im[i] = xh[2 * i + 1];
+  }
+  check(avionic_bin_write(o.output.c_str(), re.data(), im.data(), N), "writing the result");
+  if (o.extradata) {
+    std::string dir = dir_of(o.output);
+    check(avionic_recon_export(rec, export_cb, &dir), "exporting additional results");
+  }
+  avionic_recon_destroy(rec);
+  avionic_operator_destroy(op);
+  return 0;
+}

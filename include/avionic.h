@@ -252,6 +252,22 @@ int avionic_fft2(float *data, unsigned height, unsigned width, unsigned batch,
                  int sign, int centered, int unitary, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* .bin vector files (host memory; no device needed)                         */
+/* ------------------------------------------------------------------------ */
+/* The reference's vector file: uint8 iscomplex, uint32 n, n float64 real
+ * parts, then (iscomplex) n float64 imaginary parts (matlab/writebin_vector.m
+ * :35-44, readbin_vector.m:30-40, python/utils.py:23-48; the CLI writes its
+ * results with it, main.cpp:831-839).
+ * avionic_bin_read: with re == NULL only *n and *iscomplex are returned;
+ * otherwise re (and im, if the file is complex and im != NULL) receive
+ * min(n, capacity) values.  A real file read with im != NULL zero-fills im. */
+int avionic_bin_read(const char *path, double *re, double *im, size_t capacity,
+                     size_t *n, int *iscomplex);
+/* im == NULL writes a real vector. */
+int avionic_bin_write(const char *path, const double *re, const double *im,
+                      size_t n);
+
+/* ------------------------------------------------------------------------ */
 /* Batched execution and serving (no reference counterpart: the reference   */
 /* runs one reconstruction per process, main.cpp:789)                        */
 /* ------------------------------------------------------------------------ */
