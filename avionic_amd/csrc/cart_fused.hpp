@@ -17,66 +17,100 @@ namespace avionic {
 #ifndef AVIONIC_COL_THREADS
 #define AVIONIC_COL_THREADS 256
 #endif
-#ifndef AVIONIC_COL_UCACHE
-#define AVIONIC_COL_UCACHE 0
-#endif
 
 constexpr int RTILE = kRowTileElems;
 constexpr int NTC = AVIONIC_COL_THREADS;  // threads of a column-pass block
 constexpr int kColSlots = max_slots(kColTileElems, NTC);
-constexpr bool kColUcache = AVIONIC_COL_UCACHE != 0;  // keep the image tile in registers
-#ifndef AVIONIC_COL_ACC
-#define AVIONIC_COL_ACC 1
-#endif
-// adjoint coil sum in registers (else read-modify-write of the output per coil)
-constexpr bool kColAcc = AVIONIC_COL_ACC != 0;
-#ifndef AVIONIC_COL_PREFETCH
-#define AVIONIC_COL_PREFETCH 0
-#endif
-// load the next coil's b1 (forward) / zhat and b1 (adjoint) while this coil
-// is transformed
-constexpr bool kColPrefetch = AVIONIC_COL_PREFETCH != 0;
-constexpr int kPre = kColPrefetch ? kColSlots : 1;
+
+// (i + n/2) mod n and (a + b) mod n for 0 <= i, a < n, 0 <= b <= n, without a
+// division.
+__device__ __forceinline__ int wrap_half(int i, int n) {
+  const int v = i + n / 2;
+  return v >= n ? v - n : v;
+}
+__device__ __forceinline__ int wrap_add(int a, int b, int n) {
+  const int v = a + b;
+  return v >= n ? v - n : v;
+}
+
+// ------------------------------------------------------------ column plans
+// A column pass runs on the geometry's runtime plan (g.ph, g.XBF) or on one
+// of a few compile-time plans.  With a compile-time plan every index of the
+// transform (butterfly, Stockham position, twiddle, centring shift) is a
+// constant expression and the integer divisions and remainders that the
+// runtime plan evaluates per element fold into shifts and masks: measured on
+// MI355X, the runtime-plan passes spend more VALU instructions on that index
+// arithmetic than on the transform.
+struct ColPlanRt {
+  static constexpr int H = 0, XB = 0;
+};
+template <int H_, int XB_, int R0_, int R1_, int R2_ = 1>
+struct ColPlanCt {
+  static constexpr int H = H_, XB = XB_, R0 = R0_, R1 = R1_, R2 = R2_;
+  static_assert(H_ * XB_ <= kColTileElems, "column tile exceeds the LDS tile");
+  static_assert(R0_ * R1_ * R2_ == H_, "radices must multiply to the length");
+};
+// The radices in make_fft_plan's order; CartGeom::cplan names one.
+using ColPlan256 = ColPlanCt<256, 16, 16, 16>;
+using ColPlan128 = ColPlanCt<128, 32, 16, 8>;
+using ColPlan160 = ColPlanCt<160, 16, 16, 2, 5>;
+using ColPlan192 = ColPlanCt<192, 16, 16, 4, 3>;
+using ColPlan224 = ColPlanCt<224, 16, 16, 2, 7>;
+
+// Column transform of one tile, first stage from `ld`, last stage to `st`.
+template <class P, class LD, class ST>
+__device__ __forceinline__ void col_fft(float2 *tile, const CartGeom &g, float sign, int tid,
+                                        const float2 *tw, LD &ld, ST &st) {
+  if constexpr (P::H == 0) {
+    fft_io<NTC, kColTileElems, true, true, true>(tile, g.ph, g.XBF, 1, g.XBF, sign, tid, tw, ld, st);
+  } else {
+    constexpr int L = P::H, B = P::XB, E = L * B;
+    NoStore ns;
+    NoLoad nl;
+    if constexpr (P::R2 == 1) {
+      fft_stage_io<P::R0, NTC, E, true, true, false>(tile, L, B, 1, B, 1, sign, tw, tid, ld, ns);
+      fft_stage_io<P::R1, NTC, E, true, false, true>(tile, L, B, 1, B, P::R0, sign, tw, tid, nl, st);
+    } else {
+      fft_stage_io<P::R0, NTC, E, true, true, false>(tile, L, B, 1, B, 1, sign, tw, tid, ld, ns);
+      fft_stage_io<P::R1, NTC, E, true, false, false>(tile, L, B, 1, B, P::R0, sign, tw, tid, nl,
+                                                      ns);
+      fft_stage_io<P::R2, NTC, E, true, false, true>(tile, L, B, 1, B, P::R0 * P::R1, sign, tw,
+                                                     tid, nl, st);
+    }
+  }
+}
 
 // ------------------------------------------------------------ column forward
 // 32-bit offsets from per-(t, c) base pointers (uniform, in SGPRs) keep one
-// VGPR per in-flight address.  The image tile is re-read per coil (an L2 hit)
-// by default: keeping it in registers costs occupancy, and on MI355X 256-lane
+// VGPR per in-flight address.  The image tile is re-read per coil (an L2
+// hit): keeping it in registers costs occupancy, and on MI355X 256-lane
 // blocks without that cache were fastest (0.91 ms against 1.94 ms for
 // 512-lane blocks with it, ICTGV2 256x256x30x16, 8 reconstructions).
-template <bool DELTA>
+// HS: the transform length when compile-time (0: runtime, H).
+template <bool DELTA, int HS>
 struct ColFwdLoad {
-  float2 ucache[kColUcache ? kColSlots : 1];
-  float2 bpre[kPre];
-  const float2 *ut, *st, *bc, *bn;  // bn: next coil's map (prefetch), or null
+  const float2 *ut, *st, *bc;
   int H, W, xb0, centered;
-  bool first;
-  __device__ float2 operator()(int b, int i, int slot) {
+  __device__ float2 operator()(int b, int i, int) const {
+    const int Hh = HS ? HS : H;
     const int x = xb0 + b;
     if (x >= W) return czero();
-    const int sy = centered ? (i + H / 2) % H : i;
-    const int sx = centered ? (x + W / 2) % W : x;
+    const int sy = centered ? wrap_half(i, Hh) : i;
+    const int sx = centered ? wrap_half(x, W) : x;
     const unsigned si = (unsigned)(sy * W + sx);
-    float2 bv;
-    if (kColPrefetch) {
-      bv = first ? bc[si] : bpre[slot];
-      if (bn) bpre[slot] = bn[si];
-    } else {
-      bv = bc[si];
-    }
-    if (!kColUcache) return cmul(bv, DELTA ? csub(ut[si], st[si]) : ut[si]);
-    if (first) ucache[slot] = DELTA ? csub(ut[si], st[si]) : ut[si];
-    return cmul(bv, ucache[slot]);
+    return cmul(bc[si], DELTA ? csub(ut[si], st[si]) : ut[si]);
   }
 };
+template <int HS>
 struct ColFwdStore {
   float2 *kc;       // kcol + (C r0 + c S) W : rows of (t, c)
-  const int *rmap;  // rowmap + t H
+  const int *rmap;  // LDS: compact row of each k-space row of frame t
   int H, W, xb0, centered;
   __device__ void operator()(int b, int i, float2 v, int) {
+    const int Hh = HS ? HS : H;
     const int x = xb0 + b;
     if (x >= W) return;
-    const int ky = centered ? (i + H / 2) % H : i;
+    const int ky = centered ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
     if (k >= 0) kc[(unsigned)(k * W + x)] = v;
   }
@@ -91,80 +125,65 @@ __device__ inline const int *stage_rowmap(const CartGeom &g, int t, int *lds, in
   return lds;
 }
 
-template <bool DELTA>
+template <bool DELTA, class P>
 __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, const float2 *u,
                                          const float2 *usub, const float2 *b1, float2 *kcol,
                                          float2 *tile, float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
+  const int XB = P::XB ? P::XB : g.XBF;
   const float2 *tw = stage_twiddles(g.ph, twl, tid, NTC);
   const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColFwdLoad<DELTA> ld;
+  ColFwdLoad<DELTA, P::H> ld;
   ld.ut = u + t * HW;
   ld.st = DELTA ? usub + t * HW : nullptr;
   ld.H = g.H;
   ld.W = g.W;
-  ld.xb0 = xbi * g.XBF;
+  ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColFwdStore st{nullptr, rmap, g.H, g.W, xbi * g.XBF, g.centered};
+  ColFwdStore<P::H> st{nullptr, rmap, g.H, g.W, xbi * XB, g.centered};
   for (int c = 0; c < g.C; ++c) {
     ld.bc = b1 + c * HW;
-    ld.bn = (c + 1 < g.C) ? b1 + (c + 1) * HW : nullptr;
-    ld.first = (c == 0);
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
-    fft_io<NTC, kColTileElems, true, true, true>(tile, g.ph, g.XBF, 1, g.XBF, +1.f, tid, tw, ld, st);
+    col_fft<P>(tile, g, +1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
 
 // ------------------------------------------------------------ column adjoint
+template <int HS>
 struct ColAdjLoad {
-  float2 zpre[kPre];
-  const float2 *zh, *zn;  // zhat rows of (t, c); of (t, c + 1) or null
+  const float2 *zh;  // zhat rows of (t, c)
   const int *rmap;
   int H, W, xb0, centered;
-  bool first;
-  __device__ float2 operator()(int b, int i, int slot) {
+  __device__ float2 operator()(int b, int i, int) const {
+    const int Hh = HS ? HS : H;
     const int x = xb0 + b;
     if (x >= W) return czero();
-    const int ky = centered ? (i + H / 2) % H : i;
+    const int ky = centered ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
     if (k < 0) return czero();
-    const unsigned o = (unsigned)(k * W + x);
-    if (!kColPrefetch) return zh[o];
-    const float2 v = first ? zh[o] : zpre[slot];
-    if (zn) zpre[slot] = zn[o];
-    return v;
+    return zh[(unsigned)(k * W + x)];
   }
 };
+// The coil sum stays in registers, one accumulator per output of the lane.
+template <int HS>
 struct ColAdjStore {
-  float2 acc[kColAcc ? kColSlots : 1];
-  float2 bpre[kPre];
-  const float2 *bc, *bn;
+  float2 acc[kColSlots];
+  const float2 *bc;
   float2 *out;  // image frame t
   int H, W, xb0, centered;
-  bool first, last;
+  bool last;
   float scale;
   __device__ void operator()(int b, int i, float2 v, int slot) {
+    const int Hh = HS ? HS : H;
     const int x = xb0 + b;
     if (x >= W) return;
-    const int y = centered ? (i + H / 2) % H : i;
+    const int y = centered ? wrap_half(i, Hh) : i;
     const unsigned o = (unsigned)(y * W + x);
-    float2 bv;
-    if (kColPrefetch) {
-      bv = first ? bc[o] : bpre[slot];
-      if (bn) bpre[slot] = bn[o];
-    } else {
-      bv = bc[o];
-    }
-    if (!kColAcc) {
-      const float2 a = cadd(first ? czero() : out[o], cmulc(bv, v));
-      out[o] = last ? cscale(a, scale) : a;
-      return;
-    }
-    const float2 a = cadd(acc[slot], cmulc(bv, v));
+    const float2 a = cadd(acc[slot], cmulc(bc[o], v));
     if (last)
       out[o] = cscale(a, scale);
     else
@@ -172,39 +191,37 @@ struct ColAdjStore {
   }
 };
 
+template <class P>
 __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, const float2 *zhat,
                                          const float2 *b1, float2 *out, float2 *tile,
                                          float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
+  const int XB = P::XB ? P::XB : g.XBF;
   const float2 *tw = stage_twiddles(g.ph, twl, tid, NTC);
   const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColAdjLoad ld;
+  ColAdjLoad<P::H> ld;
   ld.rmap = rmap;
   ld.H = g.H;
   ld.W = g.W;
-  ld.xb0 = xbi * g.XBF;
+  ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColAdjStore st;
+  ColAdjStore<P::H> st;
 #pragma unroll
-  for (int k = 0; k < (kColAcc ? kColSlots : 1); ++k) st.acc[k] = czero();
+  for (int k = 0; k < kColSlots; ++k) st.acc[k] = czero();
   st.out = out + t * HW;
   st.H = g.H;
   st.W = g.W;
-  st.xb0 = xbi * g.XBF;
+  st.xb0 = xbi * XB;
   st.centered = g.centered;
   st.scale = g.scale;
   for (int c = 0; c < g.C; ++c) {
     ld.zh = zhat + ((long)g.C * r0 + (long)c * S) * g.W;
-    ld.zn = (c + 1 < g.C) ? ld.zh + (long)S * g.W : nullptr;
-    ld.first = (c == 0);
     st.bc = b1 + c * HW;
-    st.bn = (c + 1 < g.C) ? b1 + (c + 1) * HW : nullptr;
-    st.first = (c == 0);
     st.last = (c == g.C - 1);
-    fft_io<NTC, kColTileElems, true, true, true>(tile, g.ph, g.XBF, 1, g.XBF, -1.f, tid, tw, ld, st);
+    col_fft<P>(tile, g, -1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
@@ -233,7 +250,7 @@ struct RowDualStore {
   int W, wc, centered;
   float scale, sigma, inv;
   __device__ void operator()(int b, int i, float2 v, int) {
-    const int kx = centered ? (i + W / 2) % W : i;
+    const int kx = centered ? wrap_half(i, W) : i;
     const float mk = mrows[(long)mrow[b] * W + kx];
     const float2 val = cscale(v, scale * mk);
     const long gi = (g0 + b) * W + kx;
@@ -241,7 +258,7 @@ struct RowDualStore {
     zz = cadd(zz, cscale(val, sigma));
     zz = cscale(csub(zz, cscale(d[gi], sigma)), inv);
     z[gi] = zz;
-    const int xp = centered ? (kx + wc) % W : kx;
+    const int xp = centered ? wrap_add(kx, wc, W) : kx;
     tile2[b * W + xp] = cscale(zz, mk);
   }
 };
@@ -250,7 +267,7 @@ struct RowAdjStore {
   long g0;
   int W, centered;
   __device__ void operator()(int b, int i, float2 v, int) {
-    const int kxp = centered ? (i + W / 2) % W : i;
+    const int kxp = centered ? wrap_half(i, W) : i;
     zhat[(g0 + b) * W + kxp] = v;
   }
 };
@@ -262,7 +279,7 @@ struct RowNorm2Store {
   float scale;
   double acc;
   __device__ void operator()(int b, int i, float2 v, int) {
-    const int kx = centered ? (i + W / 2) % W : i;
+    const int kx = centered ? wrap_half(i, W) : i;
     const float2 val = cscale(v, scale * mrows[(long)mrow[b] * W + kx]);
     acc += (double)cabs2(val);
   }

@@ -26,6 +26,7 @@ struct CartGeom {
   int RB;  // rows per row-pass tile
   int nxb;
   int XBF, nxbf;  // columns per tile / tiles of the hot loop's fused column passes
+  int cplan;      // compile-time plan of the fused column passes (ColPlanId)
   long nrows;          // sampled rows over all frames (row_off[T])
   long nrows_c;        // C * nrows
   const int *row_off;  // [T+1]
@@ -36,6 +37,10 @@ struct CartGeom {
   float scale;         // 1 / sqrt(H W)
   FftPlan pw, ph;
 };
+
+// Compile-time column plans (transform length H, XBF columns per tile); any
+// other geometry runs the runtime plan.
+enum ColPlanId { kColPlanRt = 0, kColPlan256, kColPlan128, kColPlan160, kColPlan192, kColPlan224 };
 
 // Row-pass modes.
 enum RowMode {

@@ -103,8 +103,10 @@ struct avionic_engine {
     g.drop_graphs();
     std::vector<Task> tasks;
     int bv = 0, bc = 0, br = 0;
+    g.grid = TaskGrid{};
     for (Slot *s : g.slots) {
       Task t = s->task;
+      (t.cg.cplan != kColPlanRt ? g.grid.col_ct : g.grid.col_rt) = true;
       t.b_vox = bv;
       t.b_col = bc;
       t.b_row = br;

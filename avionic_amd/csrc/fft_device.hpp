@@ -225,7 +225,7 @@ __device__ inline void fft_lds(float2 *buf, const FftPlan &p, int B, int bstride
 // outputs a lane produces, at most kMaxSlots of them.
 
 template <int R, int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, class LD, class ST>
-__device__ inline void fft_stage_io(float2 *buf, int L, int B, int bstride, int istride, int Ns,
+__device__ __forceinline__ void fft_stage_io(float2 *buf, int L, int B, int bstride, int istride, int Ns,
                                     float sign, const float2 *__restrict__ tw, int tid, LD &ld,
                                     ST &st) {
   constexpr int MAXB = (EMAX / R + NT - 1) / NT;

@@ -115,6 +115,15 @@ static int pick_xb(int H, int W, int emax = EMAX) {
   return xb;
 }
 
+static int pick_col_plan(int H, int XBF) {
+  if (H == 256 && XBF == 16) return kColPlan256;
+  if (H == 128 && XBF == 32) return kColPlan128;
+  if (H == 160 && XBF == 16) return kColPlan160;
+  if (H == 192 && XBF == 16) return kColPlan192;
+  if (H == 224 && XBF == 16) return kColPlan224;
+  return kColPlanRt;
+}
+
 void make_fft_plan(int L, FftPlan &p, void **d_tw) {
   AV_CHECK(L >= 1 && L <= kMaxFftLen, "FFT length out of range (1..512)");
   p = FftPlan{};
@@ -158,6 +167,7 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   g.nxb = (W + g.XB - 1) / g.XB;
   g.XBF = pick_xb(H, W, kColTileElems);
   g.nxbf = (W + g.XBF - 1) / g.XBF;
+  g.cplan = pick_col_plan(H, g.XBF);
   g.scale = (float)(1.0 / std::sqrt((double)H * W));
   const long HW = (long)H * W;
   std::vector<float> hm;

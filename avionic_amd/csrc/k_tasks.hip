@@ -80,17 +80,38 @@ __global__ __launch_bounds__(RT) void kt_step_reg(const Task *__restrict__ tasks
 
 // DELTA = false: K of the extragradient image (dual update);
 // DELTA = true : K of ext - x (step rule, adapting tasks only).
-template <bool DELTA>
+// CT = true runs the tasks whose geometry has a compile-time column plan,
+// CT = false the others (each kernel's registers are allocated for its own
+// plans only; a launch skips the blocks of the other kind).
+template <bool DELTA, bool CT>
 __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n) {
   __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[kMaxFftLen];
   __shared__ int rml[kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
+  if ((T.cg.cplan != kColPlanRt) != CT) return;
   if (!task_active(T.sc)) return;
   if (DELTA && !task_adapt(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
-  col_forward_fused<DELTA>(T.cg, lb % T.cg.nxbf, lb / T.cg.nxbf, T.ext, DELTA ? T.x : nullptr,
-                           T.b1, T.kcol, tile, twl, rml);
+  const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
+  const float2 *usub = DELTA ? T.x : nullptr;
+  if constexpr (!CT) {
+    col_forward_fused<DELTA, ColPlanRt>(T.cg, xbi, t, T.ext, usub, T.b1, T.kcol, tile, twl, rml);
+  } else {
+    switch (T.cg.cplan) {
+#define AV_COLF(ID, P)                                                                         \
+  case ID:                                                                                     \
+    col_forward_fused<DELTA, P>(T.cg, xbi, t, T.ext, usub, T.b1, T.kcol, tile, twl, rml);   \
+    break;
+      AV_COLF(kColPlan256, ColPlan256)
+      AV_COLF(kColPlan128, ColPlan128)
+      AV_COLF(kColPlan160, ColPlan160)
+      AV_COLF(kColPlan192, ColPlan192)
+      AV_COLF(kColPlan224, ColPlan224)
+#undef AV_COLF
+      default: break;
+    }
+  }
 }
 
 template <int MODE>
@@ -132,14 +153,33 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
 #define AV_COL_ATTR
 #endif
 
+template <bool CT>
 __global__ __launch_bounds__(NTC) AV_COL_ATTR void kt_col_adjoint(const Task *__restrict__ tasks, int n) {
   __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[kMaxFftLen];
   __shared__ int rml[kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
+  if ((T.cg.cplan != kColPlanRt) != CT) return;
   if (!task_active(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
-  col_adjoint_fused(T.cg, lb % T.cg.nxbf, lb / T.cg.nxbf, T.zhat, T.b1, T.khz, tile, twl, rml);
+  const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
+  if constexpr (!CT) {
+    col_adjoint_fused<ColPlanRt>(T.cg, xbi, t, T.zhat, T.b1, T.khz, tile, twl, rml);
+  } else {
+    switch (T.cg.cplan) {
+#define AV_COLA(ID, P)                                                            \
+  case ID:                                                                        \
+    col_adjoint_fused<P>(T.cg, xbi, t, T.zhat, T.b1, T.khz, tile, twl, rml);   \
+    break;
+      AV_COLA(kColPlan256, ColPlan256)
+      AV_COLA(kColPlan128, ColPlan128)
+      AV_COLA(kColPlan160, ColPlan160)
+      AV_COLA(kColPlan192, ColPlan192)
+      AV_COLA(kColPlan224, ColPlan224)
+#undef AV_COLA
+      default: break;
+    }
+  }
 }
 
 // One block per task: step rule for adapting tasks, then the iteration
@@ -184,6 +224,19 @@ const char *kernel_name(int id) {
   return (id >= 0 && id < kNumKernels) ? names[id] : "?";
 }
 
+// Column passes: one launch per kind of column plan present among the tasks.
+template <bool DELTA>
+static void launch_col_fwd(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
+  if (gr.col <= 0) return;
+  if (gr.col_ct) hipLaunchKernelGGL((kt_col_forward<DELTA, true>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+  if (gr.col_rt) hipLaunchKernelGGL((kt_col_forward<DELTA, false>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+}
+static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
+  if (gr.col <= 0) return;
+  if (gr.col_ct) hipLaunchKernelGGL(kt_col_adjoint<true>, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+  if (gr.col_rt) hipLaunchKernelGGL(kt_col_adjoint<false>, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+}
+
 void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
                    hipStream_t s) {
   if (n <= 0) return;
@@ -206,9 +259,9 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
   } while (0)
   switch (id) {
     case kKDual: if (gr.vox) AV_BYMETHOD2(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
-    case kKColFwd: if (gr.col) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNC, 0, s, d_tasks, n); break;
+    case kKColFwd: launch_col_fwd<false>(gr, d_tasks, n, s); break;
     case kKRowDual: if (gr.row) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
-    case kKColAdj: if (gr.col) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNC, 0, s, d_tasks, n); break;
+    case kKColAdj: launch_col_adj(gr, d_tasks, n, s); break;
     case kKPrimal: if (gr.vox) AV_BYMETHOD2(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
     case kKStepReg:
       if (gr.vox) {
@@ -217,7 +270,7 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
         else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
       }
       break;
-    case kKColFwdDelta: if (gr.col) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNC, 0, s, d_tasks, n); break;
+    case kKColFwdDelta: launch_col_fwd<true>(gr, d_tasks, n, s); break;
     case kKRowNorm2: if (gr.row) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
     case kKTail: hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n); break;
     default: break;
@@ -264,9 +317,9 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   fork();
   if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
   AV_LAUNCH_CHECK();
-  if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<false>, dim3(gr.col), bNC, 0, s, d_tasks, n);
+  launch_col_fwd<false>(gr, d_tasks, n, s);
   if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n);
-  if (gr.col > 0) hipLaunchKernelGGL(kt_col_adjoint, dim3(gr.col), bNC, 0, s, d_tasks, n);
+  launch_col_adj(gr, d_tasks, n, s);
   AV_LAUNCH_CHECK();
   join();
   if (gr.vox > 0) AV_BYMETHOD(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n);
@@ -280,7 +333,7 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
       else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
     }
     AV_LAUNCH_CHECK();
-    if (gr.col > 0) hipLaunchKernelGGL(kt_col_forward<true>, dim3(gr.col), bNC, 0, s, d_tasks, n);
+    launch_col_fwd<true>(gr, d_tasks, n, s);
     if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n);
     AV_LAUNCH_CHECK();
     join();

@@ -44,6 +44,7 @@ struct Task {
 // tasks: device array; the host mirror gives the grid sizes.
 struct TaskGrid {
   int vox = 0, col = 0, row = 0;
+  bool col_ct = false, col_rt = false;  // tasks with a compile-time / runtime column plan
 };
 
 // Streams of one iteration: the regulariser kernels that do not depend on

@@ -85,6 +85,56 @@ def test_engine_mixed_batch_matches_alone_and_oracle(gpu):
         assert err < (2e-2 if dual else 1e-4)
 
 
+# One case per compile-time column plan (transform length 256 / 128 / 160 /
+# 192 / 224 with its tile width; widths that leave a partial last tile, both
+# centring conventions) and one runtime-plan geometry, resident together.
+PLAN_CASES = [  # method, T, H, W, C, maxIt, centered
+    (2, 2, 256, 40, 2, 12, False),
+    (1, 2, 128, 48, 3, 12, True),
+    (0, 3, 160, 24, 2, 12, False),
+    (2, 2, 192, 20, 2, 12, True),
+    (1, 2, 224, 36, 2, 12, False),
+    (2, 2, 96, 40, 2, 12, False),
+]
+
+
+def test_engine_column_plans_match_oracle(gpu):
+    import torch
+    import avionic_amd as A
+    reqs, refs, alone = [], [], []
+    for i, (m, T, H, W, C, it, cen) in enumerate(PLAN_CASES):
+        data, b1, mask = _inputs(T, H, W, C, seed=20 + i)
+        p = _params(m, it, 0, mask, W, H, T)
+        reqs.append(A.Request(m, _dev(torch, data), _dev(torch, b1),
+                              mask=_dev(torch, mask, np.float32), centered=cen, params=p))
+        op = O.default_params(m)
+        op.update(maxIt=it, lambda_=p["lambda"], centered=int(cen))
+        refs.append(O.recon(op, data, b1, mask))
+    for r in reqs:
+        e = A.Engine(use_graphs=False, max_batch=1)
+        rid = e.submit(r)
+        out = torch.empty(r.shape[3], r.shape[1], r.shape[0], dtype=torch.complex64, device="cuda")
+        while not e.poll(rid, out):
+            e.step()
+        alone.append(out.cpu().numpy())
+    e = A.Engine(use_graphs=True)
+    ids = [e.submit(r) for r in reqs]
+    outs = [torch.empty(r.shape[3], r.shape[1], r.shape[0], dtype=torch.complex64, device="cuda")
+            for r in reqs]
+    done = set()
+    while len(done) < len(ids):
+        e.step()
+        for i, rid in enumerate(ids):
+            if i not in done and e.poll(rid, outs[i]):
+                done.add(i)
+    for i, case in enumerate(PLAN_CASES):
+        got = outs[i].cpu().numpy()
+        assert np.array_equal(got, alone[i]), f"case {case} differs alone vs in the mix"
+        err = np.linalg.norm(got - refs[i]) / np.linalg.norm(refs[i])
+        print(f"[plans] {case} NRMSE vs oracle {err:.2e}", flush=True)
+        assert err < 1e-4, case
+
+
 def test_engine_admission_cancel_and_errors(gpu):
     import torch
     import avionic_amd as A
