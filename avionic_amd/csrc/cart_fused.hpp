@@ -98,7 +98,7 @@ struct ColFwdLoad {
     const int sy = centered ? wrap_half(i, Hh) : i;
     const int sx = centered ? wrap_half(x, W) : x;
     const unsigned si = (unsigned)(sy * W + sx);
-    return cmul(bc[si], DELTA ? csub(ut[si], st[si]) : ut[si]);
+    return cmul(gld(bc, si), DELTA ? csub(gld(ut, si), gld(st, si)) : gld(ut, si));
   }
 };
 template <int HS>
@@ -112,7 +112,7 @@ struct ColFwdStore {
     if (x >= W) return;
     const int ky = centered ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
-    if (k >= 0) kc[(unsigned)(k * W + x)] = v;
+    if (k >= 0) gst(kc, (unsigned)(k * W + x), v);
   }
 };
 
@@ -165,7 +165,7 @@ struct ColAdjLoad {
     const int ky = centered ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
     if (k < 0) return czero();
-    return zh[(unsigned)(k * W + x)];
+    return gld(zh, (unsigned)(k * W + x));
   }
 };
 // The coil sum stays in registers, one accumulator per output of the lane.
@@ -183,9 +183,9 @@ struct ColAdjStore {
     if (x >= W) return;
     const int y = centered ? wrap_half(i, Hh) : i;
     const unsigned o = (unsigned)(y * W + x);
-    const float2 a = cadd(acc[slot], cmulc(bc[o], v));
+    const float2 a = cadd(acc[slot], cmulc(gld(bc, o), v));
     if (last)
-      out[o] = cscale(a, scale);
+      gst(out, o, cscale(a, scale));
     else
       acc[slot] = a;
   }
@@ -230,7 +230,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
 struct RowLoad {
   const float2 *src;  // kcol + g0 W
   int W;
-  __device__ float2 operator()(int b, int i, int) const { return src[(unsigned)(b * W + i)]; }
+  __device__ float2 operator()(int b, int i, int) const { return gld(src, (unsigned)(b * W + i)); }
 };
 // Mask row (index into mrows) of each row of the block, staged in LDS so
 // that the mask load does not wait on a global rowinfo load.
@@ -241,46 +241,43 @@ __device__ inline const int *stage_mask_rows(const CartGeom &g, long g0, int nr,
 }
 
 struct RowDualStore {
-  float2 *z;
-  const float2 *d;
+  float2 *z;        // z + g0 W: rows of the block
+  const float2 *d;  // d + g0 W
   const float *mrows;
   const int *mrow;  // LDS: mask row of local row b
   float2 *tile2;
-  long g0;
   int W, wc, centered;
   float scale, sigma, inv;
   __device__ void operator()(int b, int i, float2 v, int) {
     const int kx = centered ? wrap_half(i, W) : i;
-    const float mk = mrows[(long)mrow[b] * W + kx];
+    const float mk = gld(mrows, (unsigned)(mrow[b] * W + kx));
     const float2 val = cscale(v, scale * mk);
-    const long gi = (g0 + b) * W + kx;
-    float2 zz = z[gi];
+    const unsigned gi = (unsigned)(b * W + kx);
+    float2 zz = gld(z, gi);
     zz = cadd(zz, cscale(val, sigma));
-    zz = cscale(csub(zz, cscale(d[gi], sigma)), inv);
-    z[gi] = zz;
+    zz = cscale(csub(zz, cscale(gld(d, gi), sigma)), inv);
+    gst(z, gi, zz);
     const int xp = centered ? wrap_add(kx, wc, W) : kx;
     tile2[b * W + xp] = cscale(zz, mk);
   }
 };
 struct RowAdjStore {
-  float2 *zhat;
-  long g0;
+  float2 *zhat;  // zhat + g0 W
   int W, centered;
   __device__ void operator()(int b, int i, float2 v, int) {
     const int kxp = centered ? wrap_half(i, W) : i;
-    zhat[(g0 + b) * W + kxp] = v;
+    gst(zhat, (unsigned)(b * W + kxp), v);
   }
 };
 struct RowNorm2Store {
   const float *mrows;
   const int *mrow;  // LDS: mask row of local row b
-  long g0;
   int W, centered;
   float scale;
   double acc;
   __device__ void operator()(int b, int i, float2 v, int) {
     const int kx = centered ? wrap_half(i, W) : i;
-    const float2 val = cscale(v, scale * mrows[(long)mrow[b] * W + kx]);
+    const float2 val = cscale(v, scale * gld(mrows, (unsigned)(mrow[b] * W + kx)));
     acc += (double)cabs2(val);
   }
 };
@@ -297,12 +294,11 @@ __device__ inline void row_dual_fused(const CartGeom &g, int blk, const float2 *
   __syncthreads();
   RowLoad ld{kcol + g0 * W, W};
   RowDualStore st;
-  st.z = z;
-  st.d = d;
+  st.z = z + g0 * W;
+  st.d = d + g0 * W;
   st.mrows = g.mrows;
   st.mrow = mrow;
   st.tile2 = tile2;
-  st.g0 = g0;
   st.W = W;
   st.wc = (W + 1) / 2;
   st.centered = g.centered;
@@ -311,7 +307,7 @@ __device__ inline void row_dual_fused(const CartGeom &g, int blk, const float2 *
   st.inv = (float)(1.0 / (1.0 + sc->sigma / sc->lambda));
   fft_io<NT, RTILE, false, true, true>(tile, g.pw, nr, W, 1, +1.f, tid, tw, ld, st);
   __syncthreads();  // tile2 holds M z for every row of the block
-  RowAdjStore st2{zhat, g0, W, g.centered};
+  RowAdjStore st2{zhat + g0 * W, W, g.centered};
   NoLoad nl;
   fft_io<NT, RTILE, false, false, true>(tile2, g.pw, nr, W, 1, -1.f, tid, tw, nl, st2);
 }
@@ -326,7 +322,7 @@ __device__ inline double row_norm2_fused(const CartGeom &g, int blk, const float
   const int *mrow = stage_mask_rows(g, g0, nr, mrl, tid, NT);
   __syncthreads();
   RowLoad ld{kcol + g0 * W, W};
-  RowNorm2Store st{g.mrows, mrow, g0, W, g.centered, g.scale, 0.0};
+  RowNorm2Store st{g.mrows, mrow, W, g.centered, g.scale, 0.0};
   fft_io<NT, RTILE, false, true, true>(tile, g.pw, nr, W, 1, +1.f, tid, tw, ld, st);
   return st.acc;
 }

@@ -77,6 +77,35 @@ __host__ __device__ inline int16_t f32_to_q16(float v, float inv_scale) {
   return (int16_t)q;
 }
 
+// Global-memory access at a 32-bit element offset from a base pointer.  The
+// kernels read their pointers from a task table in memory, so the compiler
+// cannot tell their address space and would emit flat loads with 64-bit
+// per-lane addresses; through these helpers every access is a global load or
+// store with the (uniform) base in scalar registers and a 32-bit lane offset.
+// i * sizeof(T) must stay below 4 GiB (callers bound their extents).
+// (The host pass only parses these bodies; it cannot form address-space
+// qualified class objects, so it sees a plain access.)
+template <class T>
+__device__ __forceinline__ T gld(const T *base, unsigned i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const T gT;
+  return *(gT *)((gchar *)base + i * (unsigned)sizeof(T));
+#else
+  return base[i];
+#endif
+}
+template <class T>
+__device__ __forceinline__ void gst(T *base, unsigned i, const T &v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) char gchar;
+  typedef __attribute__((address_space(1))) T gT;
+  *(gT *)((gchar *)base + i * (unsigned)sizeof(T)) = v;
+#else
+  base[i] = v;
+#endif
+}
+
 // Dual storage kinds (avionic_params::dualStorage).
 enum DualStorage { kDualF32 = 0, kDualBF16 = 1, kDualQ16 = 2 };
 inline size_t dual_bytes(int storage) { return storage == kDualF32 ? 8 : 4; }
@@ -87,32 +116,35 @@ template <typename DT>
 struct DualIO;
 template <>
 struct DualIO<float2> {
-  __device__ static inline float2 load(const float2 *p, long i, float) { return p[i]; }
-  __device__ static inline void store(float2 *p, long i, float2 v, float) { p[i] = v; }
+  __device__ static inline float2 load(const float2 *p, unsigned i, float) { return gld(p, i); }
+  __device__ static inline void store(float2 *p, unsigned i, float2 v, float) { gst(p, i, v); }
 };
 template <>
 struct DualIO<bf16x2> {
-  __device__ static inline float2 load(const bf16x2 *p, long i, float) {
-    uint32_t w = *reinterpret_cast<const uint32_t *>(p + i);
+  __device__ static inline float2 load(const bf16x2 *p, unsigned i, float) {
+    const uint32_t w = gld(reinterpret_cast<const uint32_t *>(p), i);
     return make_float2(u2f(w << 16), u2f(w & 0xffff0000u));
   }
-  __device__ static inline void store(bf16x2 *p, long i, float2 v, float) {
-    uint32_t w = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
-    *reinterpret_cast<uint32_t *>(p + i) = w;
+  __device__ static inline void store(bf16x2 *p, unsigned i, float2 v, float) {
+    const uint32_t w = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
+    gst(reinterpret_cast<uint32_t *>(p), i, w);
   }
 };
 template <>
 struct DualIO<q16x2> {
-  __device__ static inline float2 load(const q16x2 *p, long i, float r) {
-    const uint32_t w = *reinterpret_cast<const uint32_t *>(p + i);
+  __device__ static inline float2 load(const q16x2 *p, unsigned i, float r) {
+    const uint32_t w = gld(reinterpret_cast<const uint32_t *>(p), i);
     const float s = r * (1.f / kQ16Max);
     return make_float2((float)(int16_t)(w & 0xffffu) * s, (float)(int16_t)(w >> 16) * s);
   }
-  __device__ static inline void store(q16x2 *p, long i, float2 v, float r) {
-    const float inv = kQ16Max / r;
+  // The encode scale 32767 / r through the hardware reciprocal (1 ulp): the
+  // stored value may differ from the correctly rounded quotient's by one
+  // step of the int16 grid, far below the format's own resolution.
+  __device__ static inline void store(q16x2 *p, unsigned i, float2 v, float r) {
+    const float inv = kQ16Max * __builtin_amdgcn_rcpf(r);
     const uint32_t w = (uint32_t)(uint16_t)f32_to_q16(v.x, inv) |
                        ((uint32_t)(uint16_t)f32_to_q16(v.y, inv) << 16);
-    *reinterpret_cast<uint32_t *>(p + i) = w;
+    gst(reinterpret_cast<uint32_t *>(p), i, w);
   }
 };
 

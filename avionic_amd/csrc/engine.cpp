@@ -195,6 +195,8 @@ void check_desc(const avionic_request_desc *d) {
   AV_CHECK(d->method >= 0 && d->method <= 2, "unknown method");
   AV_CHECK(d->width > 0 && d->height > 0 && d->coils > 0 && d->frames > 0, "empty request");
   AV_CHECK(d->data != nullptr && d->b1 != nullptr, "request data / b1 missing");
+  check_grid(Grid3{(int)d->width, (int)d->height, (int)d->frames,
+                   (long)d->width * d->height * d->frames});
 }
 
 }  // namespace

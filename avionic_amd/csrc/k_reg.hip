@@ -95,7 +95,7 @@ __global__ __launch_bounds__(RT) void k_reg_norm(Grid3 g, RegParams rp, double a
       if (M == kTGV2) {
         grad3(X, 0, v, g, d1, gr);
       } else {
-        SubAcc UW{x, 4 * N};
+        DiffAcc UW{x, x + 4 * N};
         grad3(UW, 0, v, g, d1, gr);
       }
       for (int c = 0; c < 3; ++c) gr[c] = csub(gr[c], x[(1 + c) * N + i]);
@@ -238,6 +238,7 @@ static inline unsigned nblk(long n) { return (unsigned)((n + RT - 1) / RT); }
 
 void launch_dual(int method, int ds, const Grid3 &g, const RegParams &rp, const float2 *ext, void *y,
                  const DevScalars *sc, hipStream_t s) {
+  check_grid(g);
   dim3 grid((unsigned)step_blocks(g)), block(RT);
 #define AV_DUAL(M, DT) hipLaunchKernelGGL((k_dual<M, DT>), grid, block, 0, s, g, rp, ext, (DT *)y, sc)
 #define AV_DUAL_M(DT)                        \
@@ -256,6 +257,7 @@ void launch_dual(int method, int ds, const Grid3 &g, const RegParams &rp, const 
 
 void launch_primal(int method, int ds, const Grid3 &g, const RegParams &rp, float2 *x, float2 *ext,
                    const void *y, const float2 *khz, const DevScalars *sc, hipStream_t s) {
+  check_grid(g);
   dim3 grid((unsigned)step_blocks(g)), block(RT);
 #define AV_PRIMAL(M, DT) \
   hipLaunchKernelGGL((k_primal<M, DT>), grid, block, 0, s, g, rp, x, ext, (const DT *)y, khz, sc)
@@ -278,6 +280,7 @@ int norm_blocks(const Grid3 &g) { return (int)nblk(g.N); }
 
 void launch_step_reg(int method, const Grid3 &g, const RegParams &rp, const float2 *ext,
                      const float2 *x, double *partial, hipStream_t s) {
+  check_grid(g);
   dim3 grid((unsigned)step_blocks(g)), block(RT);
   if (method == kTV) hipLaunchKernelGGL(k_step_reg<kTV>, grid, block, 0, s, g, rp, ext, x, partial);
   else if (method == kTGV2) hipLaunchKernelGGL(k_step_reg<kTGV2>, grid, block, 0, s, g, rp, ext, x, partial);
@@ -300,6 +303,7 @@ void launch_iter_bump(DevScalars *sc, hipStream_t s) {
 
 void launch_reg_norm(int method, const Grid3 &g, const RegParams &rp, double alpha0, double alpha1,
                      double alpha, const float2 *x, double *partial, hipStream_t s) {
+  check_grid(g);
   dim3 grid(nblk(g.N)), block(RT);
   double wa = 1.0, wb = 1.0;
   if (method == kICTGV2) {
@@ -315,6 +319,7 @@ void launch_reg_norm(int method, const Grid3 &g, const RegParams &rp, double alp
 
 void launch_gstar_terms(int method, const Grid3 &g, const RegParams &rp, const float2 *khz,
                         const float2 *y, double *partial, hipStream_t s) {
+  check_grid(g);
   dim3 grid(nblk(g.N)), block(RT);
   if (method == kTV) hipLaunchKernelGGL(k_gstar_terms<kTV>, grid, block, 0, s, g, rp, khz, y, partial);
   else if (method == kTGV2) hipLaunchKernelGGL(k_gstar_terms<kTGV2>, grid, block, 0, s, g, rp, khz, y, partial);
@@ -343,21 +348,25 @@ double reduce_host(const double *partial, int n, int stride, int offset, hipStre
 
 void launch_gradient(const Grid3 &g, float idx, float idy, float idt, const float2 *u, float2 *out,
                      hipStream_t s) {
+  check_grid(g);
   hipLaunchKernelGGL(k_gradient, dim3(nblk(g.N)), dim3(RT), 0, s, g, Inv3{idx, idy, idt}, u, out);
   AV_LAUNCH_CHECK();
 }
 void launch_symgrad(const Grid3 &g, float idx, float idy, float idt, const float2 *v, float2 *out,
                     hipStream_t s) {
+  check_grid(g);
   hipLaunchKernelGGL(k_symgrad, dim3(nblk(g.N)), dim3(RT), 0, s, g, Inv3{idx, idy, idt}, v, out);
   AV_LAUNCH_CHECK();
 }
 void launch_divergence(const Grid3 &g, float idx, float idy, float idt, const float2 *p,
                        float2 *out, hipStream_t s) {
+  check_grid(g);
   hipLaunchKernelGGL(k_divergence, dim3(nblk(g.N)), dim3(RT), 0, s, g, Inv3{idx, idy, idt}, p, out);
   AV_LAUNCH_CHECK();
 }
 void launch_symdiv(const Grid3 &g, float idx, float idy, float idt, const float2 *q, float2 *out,
                    hipStream_t s) {
+  check_grid(g);
   hipLaunchKernelGGL(k_symdiv, dim3(nblk(g.N)), dim3(RT), 0, s, g, Inv3{idx, idy, idt}, q, out);
   AV_LAUNCH_CHECK();
 }

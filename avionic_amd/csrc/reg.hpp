@@ -18,6 +18,13 @@ struct Grid3 {
   long N;
 };
 
+// Voxel kernels index inside one component with 32 bits of byte offset.
+constexpr long kMaxVoxels = 1L << 29;
+inline void check_grid(const Grid3 &g) {
+  AV_CHECK(g.W >= 1 && g.H >= 1 && g.T >= 1 && g.N < kMaxVoxels,
+           "width * height * frames must be below 2^29 voxels");
+}
+
 // Inverse step sizes of the two derivative sets and the projection radii
 // (radius = 0 means "no projection").
 struct RegParams {

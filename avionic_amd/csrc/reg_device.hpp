@@ -13,14 +13,19 @@ namespace avionic {
 
 constexpr int RT = kRedThreads;
 
+// A voxel and its linear index within one component plane stack
+// (t HW + y W + x).  Component c of a state lives at base + c N; kernels add
+// c N to the (uniform) base pointer and index with the 32-bit i, so every
+// access is a scalar base plus a 32-bit lane offset (N < 2^29 is checked on
+// the host, which keeps byte offsets within 32 bits).
 struct Vox {
   int x, y, t;
-  long i;
+  unsigned i;
 };
 
 __device__ inline Vox vox_of(const Grid3 &g, long i) {
   Vox v;
-  v.i = i;
+  v.i = (unsigned)i;
   v.x = (int)(i % g.W);
   const long r = i / g.W;
   v.y = (int)(r % g.H);
@@ -41,54 +46,53 @@ __host__ __device__ inline long tiles_per_frame(const Grid3 &g) {
 }
 template <bool INTERLEAVE>
 __device__ inline bool vox_of_block(const Grid3 &g, long lb, int tid, Vox &v) {
-  const long HW = (long)g.W * g.H;
-  const long tiles = (HW + RT - 1) / RT;
+  const int HW = g.W * g.H;
+  const int tiles = (HW + RT - 1) / RT;
   const int t = (int)(INTERLEAVE ? lb % g.T : lb / tiles);
-  const long p = (INTERLEAVE ? lb / g.T : lb % tiles) * RT + tid;
+  const int p = (int)(INTERLEAVE ? lb / g.T : lb % tiles) * RT + tid;
   if (p >= HW) return false;
   v.t = t;
-  v.y = (int)(p / g.W);
-  v.x = (int)(p % g.W);
-  v.i = (long)t * HW + p;
+  v.y = p / g.W;
+  v.x = p - v.y * g.W;
+  v.i = (unsigned)t * (unsigned)HW + (unsigned)p;
   return true;
 }
 
+// Accessors: a(j) reads element j of a component; a.at(o) is the accessor of
+// the component at element offset o.
 struct PlainAcc {
   const float2 *p;
-  __device__ float2 operator()(long j) const { return p[j]; }
+  __device__ PlainAcc at(long o) const { return {p + o}; }
+  __device__ float2 operator()(unsigned j) const { return gld(p, j); }
 };
 template <typename DT>
 struct DualAcc {
   const DT *p;
   float r;  // projection radius of the component group read through it
-  __device__ float2 operator()(long j) const { return DualIO<DT>::load(p, j, r); }
+  __device__ DualAcc at(long o) const { return {p + o, r}; }
+  __device__ float2 operator()(unsigned j) const { return DualIO<DT>::load(p, j, r); }
 };
 // a[j] - b[j]
 struct DiffAcc {
   const float2 *a, *b;
-  __device__ float2 operator()(long j) const { return csub(a[j], b[j]); }
+  __device__ DiffAcc at(long o) const { return {a + o, b + o}; }
+  __device__ float2 operator()(unsigned j) const { return csub(gld(a, j), gld(b, j)); }
 };
-// a[j] - a[j + off]
-struct SubAcc {
-  const float2 *a;
-  long off;
-  __device__ float2 operator()(long j) const { return csub(a[j], a[j + off]); }
-};
-// (a[j] - b[j]) - (a[j + off] - b[j + off])
-struct DiffSubAcc {
-  const float2 *a, *b;
-  long off;
-  __device__ float2 operator()(long j) const {
-    return csub(csub(a[j], b[j]), csub(a[j + off], b[j + off]));
+// (a[j] - b[j]) - (c[j] - d[j])
+struct Diff2Acc {
+  const float2 *a, *b, *c, *d;
+  __device__ Diff2Acc at(long o) const { return {a + o, b + o, c + o, d + o}; }
+  __device__ float2 operator()(unsigned j) const {
+    return csub(csub(gld(a, j), gld(b, j)), csub(gld(c, j), gld(d, j)));
   }
 };
 
 template <class A>
-__device__ inline float2 fdiff(const A &a, long i, long s, int pos, int len) {
+__device__ inline float2 fdiff(const A &a, unsigned i, unsigned s, int pos, int len) {
   return (pos < len - 1) ? csub(a(i + s), a(i)) : czero();
 }
 template <class A>
-__device__ inline float2 bdiff(const A &a, long i, long s, int pos, int len) {
+__device__ inline float2 bdiff(const A &a, unsigned i, unsigned s, int pos, int len) {
   if (len == 1) return czero();
   if (pos == 0) return a(i);
   if (pos == len - 1) return cscale(a(i - s), -1.f);
@@ -101,9 +105,10 @@ struct Inv3 {
 
 // grad of the scalar field at component offset o.
 template <class A>
-__device__ inline void grad3(const A &a, long o, const Vox &v, const Grid3 &g, Inv3 d, float2 *out) {
-  const long i = o + v.i, sy = g.W, st = (long)g.W * g.H;
-  out[0] = cscale(fdiff(a, i, 1, v.x, g.W), d.x);
+__device__ inline void grad3(const A &a0, long o, const Vox &v, const Grid3 &g, Inv3 d, float2 *out) {
+  const A a = a0.at(o);
+  const unsigned i = v.i, sy = g.W, st = (unsigned)g.W * g.H;
+  out[0] = cscale(fdiff(a, i, 1u, v.x, g.W), d.x);
   out[1] = cscale(fdiff(a, i, sy, v.y, g.H), d.y);
   out[2] = cscale(fdiff(a, i, st, v.t, g.T), d.t);
 }
@@ -112,24 +117,24 @@ __device__ inline void grad3(const A &a, long o, const Vox &v, const Grid3 &g, I
 template <class A>
 __device__ inline void symgrad6(const A &a, long o0, long o1, long o2, const Vox &v, const Grid3 &g,
                                 Inv3 d, float2 *e) {
-  const long sy = g.W, st = (long)g.W * g.H;
-  const long i0 = o0 + v.i, i1 = o1 + v.i, i2 = o2 + v.i;
-  e[0] = cscale(bdiff(a, i0, 1, v.x, g.W), d.x);
-  e[1] = cscale(bdiff(a, i1, sy, v.y, g.H), d.y);
-  e[2] = cscale(bdiff(a, i2, st, v.t, g.T), d.t);
-  e[3] = cscale(cadd(cscale(bdiff(a, i0, sy, v.y, g.H), d.y), cscale(bdiff(a, i1, 1, v.x, g.W), d.x)), 0.5f);
-  e[4] = cscale(cadd(cscale(bdiff(a, i0, st, v.t, g.T), d.t), cscale(bdiff(a, i2, 1, v.x, g.W), d.x)), 0.5f);
-  e[5] = cscale(cadd(cscale(bdiff(a, i1, st, v.t, g.T), d.t), cscale(bdiff(a, i2, sy, v.y, g.H), d.y)), 0.5f);
+  const unsigned i = v.i, sy = g.W, st = (unsigned)g.W * g.H;
+  const A a0 = a.at(o0), a1 = a.at(o1), a2 = a.at(o2);
+  e[0] = cscale(bdiff(a0, i, 1u, v.x, g.W), d.x);
+  e[1] = cscale(bdiff(a1, i, sy, v.y, g.H), d.y);
+  e[2] = cscale(bdiff(a2, i, st, v.t, g.T), d.t);
+  e[3] = cscale(cadd(cscale(bdiff(a0, i, sy, v.y, g.H), d.y), cscale(bdiff(a1, i, 1u, v.x, g.W), d.x)), 0.5f);
+  e[4] = cscale(cadd(cscale(bdiff(a0, i, st, v.t, g.T), d.t), cscale(bdiff(a2, i, 1u, v.x, g.W), d.x)), 0.5f);
+  e[5] = cscale(cadd(cscale(bdiff(a1, i, st, v.t, g.T), d.t), cscale(bdiff(a2, i, sy, v.y, g.H), d.y)), 0.5f);
 }
 
 // div of the 3-vector at o0,o1,o2 (= -grad^T).
 template <class A>
 __device__ inline float2 div3(const A &a, long o0, long o1, long o2, const Vox &v, const Grid3 &g,
                               Inv3 d) {
-  const long sy = g.W, st = (long)g.W * g.H;
-  return cadd(cadd(cscale(bdiff(a, o0 + v.i, 1, v.x, g.W), d.x),
-                   cscale(bdiff(a, o1 + v.i, sy, v.y, g.H), d.y)),
-              cscale(bdiff(a, o2 + v.i, st, v.t, g.T), d.t));
+  const unsigned i = v.i, sy = g.W, st = (unsigned)g.W * g.H;
+  return cadd(cadd(cscale(bdiff(a.at(o0), i, 1u, v.x, g.W), d.x),
+                   cscale(bdiff(a.at(o1), i, sy, v.y, g.H), d.y)),
+              cscale(bdiff(a.at(o2), i, st, v.t, g.T), d.t));
 }
 
 // symmetric divergence of the 6-vector starting at component offset o
@@ -137,28 +142,32 @@ __device__ inline float2 div3(const A &a, long o0, long o1, long o2, const Vox &
 template <class A>
 __device__ inline void symdiv3(const A &a, long o, long N, const Vox &v, const Grid3 &g, Inv3 d,
                                float2 *out) {
-  const long sy = g.W, st = (long)g.W * g.H;
-  const long q0 = o + v.i, q1 = q0 + N, q2 = q0 + 2 * N, q3 = q0 + 3 * N, q4 = q0 + 4 * N,
-             q5 = q0 + 5 * N;
-  out[0] = cadd(cadd(cscale(fdiff(a, q0, 1, v.x, g.W), d.x), cscale(fdiff(a, q3, sy, v.y, g.H), d.y)),
-                cscale(fdiff(a, q4, st, v.t, g.T), d.t));
-  out[1] = cadd(cadd(cscale(fdiff(a, q3, 1, v.x, g.W), d.x), cscale(fdiff(a, q1, sy, v.y, g.H), d.y)),
-                cscale(fdiff(a, q5, st, v.t, g.T), d.t));
-  out[2] = cadd(cadd(cscale(fdiff(a, q4, 1, v.x, g.W), d.x), cscale(fdiff(a, q5, sy, v.y, g.H), d.y)),
-                cscale(fdiff(a, q2, st, v.t, g.T), d.t));
+  const unsigned i = v.i, sy = g.W, st = (unsigned)g.W * g.H;
+  const A q0 = a.at(o), q1 = a.at(o + N), q2 = a.at(o + 2 * N), q3 = a.at(o + 3 * N),
+          q4 = a.at(o + 4 * N), q5 = a.at(o + 5 * N);
+  out[0] = cadd(cadd(cscale(fdiff(q0, i, 1u, v.x, g.W), d.x), cscale(fdiff(q3, i, sy, v.y, g.H), d.y)),
+                cscale(fdiff(q4, i, st, v.t, g.T), d.t));
+  out[1] = cadd(cadd(cscale(fdiff(q3, i, 1u, v.x, g.W), d.x), cscale(fdiff(q1, i, sy, v.y, g.H), d.y)),
+                cscale(fdiff(q5, i, st, v.t, g.T), d.t));
+  out[2] = cadd(cadd(cscale(fdiff(q4, i, 1u, v.x, g.W), d.x), cscale(fdiff(q5, i, sy, v.y, g.H), d.y)),
+                cscale(fdiff(q2, i, st, v.t, g.T), d.t));
 }
 
+// p / max(1, |p| / r) (norm_3.m, norm_6.m; utils.cpp:1297-1351) evaluated as
+// p * min(1, r / |p|): one reciprocal square root instead of a square root and
+// two divisions (the scale differs from the quotient form by at most a few
+// ulp).  n2 = |p|^2 with the off-diagonal weight 2 for norm_6.
+__device__ inline float proj_scale(float n2, float r) {
+  return n2 > r * r ? r * __builtin_amdgcn_rsqf(n2) : 1.f;
+}
 __device__ inline void proj3(float2 *p, float r) {
-  const float n = sqrtf(cabs2(p[0]) + cabs2(p[1]) + cabs2(p[2]));
-  const float s = fmaxf(1.f, n / r);
-  const float is = 1.f / s;
+  const float is = proj_scale(cabs2(p[0]) + cabs2(p[1]) + cabs2(p[2]), r);
   p[0] = cscale(p[0], is); p[1] = cscale(p[1], is); p[2] = cscale(p[2], is);
 }
 __device__ inline void proj6(float2 *q, float r) {
   const float n2 = cabs2(q[0]) + cabs2(q[1]) + cabs2(q[2]) +
                    2.f * (cabs2(q[3]) + cabs2(q[4]) + cabs2(q[5]));
-  const float s = fmaxf(1.f, sqrtf(n2) / r);
-  const float is = 1.f / s;
+  const float is = proj_scale(n2, r);
 #pragma unroll
   for (int k = 0; k < 6; ++k) q[k] = cscale(q[k], is);
 }
@@ -197,55 +206,43 @@ __device__ inline void block_reduce2(double a, double b, double *out2) {
 template <int M, typename DT>
 __device__ inline void dual_voxel(const Grid3 &g, const RegParams &rp, const float2 *__restrict__ ext,
                                   DT *__restrict__ y, float sig, const Vox &v) {
-  const long N = g.N, i = v.i;
+  const long N = g.N;
+  const unsigned i = v.i;
   const Inv3 d1{rp.idx, rp.idy, rp.idt}, d2{rp.idx2, rp.idy2, rp.idt2};
   PlainAcc E{ext};
   float2 gr[3], e[6], p[6];
+  // components [c0, c0 + n) of y: p += sigma * k, project, store
+  auto ascend = [&](int c0, int n, const float2 *k, float r) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (c < n) p[c] = cadd(DualIO<DT>::load(y + (c0 + c) * N, i, r), cscale(k[c], sig));
+    if (n == 3) proj3(p, r); else proj6(p, r);
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (c < n) DualIO<DT>::store(y + (c0 + c) * N, i, p[c], r);
+  };
   if (M == kTV) {
     grad3(E, 0, v, g, d1, gr);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) p[c] = cadd(DualIO<DT>::load(y, c * N + i, rp.r1), cscale(gr[c], sig));
-    proj3(p, rp.r1);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, c * N + i, p[c], rp.r1);
+    ascend(0, 3, gr, rp.r1);
     return;
   }
   if (M == kTGV2) {
     grad3(E, 0, v, g, d1, gr);
   } else {
-    SubAcc UW{ext, 4 * N};  // u - w
-    grad3(UW, 0, v, g, d1, gr);
+    grad3(DiffAcc{ext, ext + 4 * N}, 0, v, g, d1, gr);  // u - w
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const float2 t = csub(gr[c], ext[(1 + c) * N + i]);
-    p[c] = cadd(DualIO<DT>::load(y, c * N + i, rp.r1), cscale(t, sig));
-  }
-  proj3(p, rp.r1);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, c * N + i, p[c], rp.r1);
+  for (int c = 0; c < 3; ++c) gr[c] = csub(gr[c], gld(ext + (1 + c) * N, i));
+  ascend(0, 3, gr, rp.r1);
   symgrad6(E, N, 2 * N, 3 * N, v, g, d1, e);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) p[k] = cadd(DualIO<DT>::load(y, (3 + k) * N + i, rp.r2), cscale(e[k], sig));
-  proj6(p, rp.r2);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) DualIO<DT>::store(y, (3 + k) * N + i, p[k], rp.r2);
+  ascend(3, 6, e, rp.r2);
   if (M == kICTGV2) {
     grad3(E, 4 * N, v, g, d2, gr);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float2 t = csub(gr[c], ext[(5 + c) * N + i]);
-      p[c] = cadd(DualIO<DT>::load(y, (9 + c) * N + i, rp.r3), cscale(t, sig));
-    }
-    proj3(p, rp.r3);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) DualIO<DT>::store(y, (9 + c) * N + i, p[c], rp.r3);
+    for (int c = 0; c < 3; ++c) gr[c] = csub(gr[c], gld(ext + (5 + c) * N, i));
+    ascend(9, 3, gr, rp.r3);
     symgrad6(E, 5 * N, 6 * N, 7 * N, v, g, d2, e);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) p[k] = cadd(DualIO<DT>::load(y, (12 + k) * N + i, rp.r4), cscale(e[k], sig));
-    proj6(p, rp.r4);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) DualIO<DT>::store(y, (12 + k) * N + i, p[k], rp.r4);
+    ascend(12, 6, e, rp.r4);
   }
 }
 
@@ -255,37 +252,38 @@ template <int M, typename DT>
 __device__ inline void primal_voxel(const Grid3 &g, const RegParams &rp, float2 *__restrict__ x,
                                     float2 *__restrict__ ext, const DT *__restrict__ y,
                                     const float2 *__restrict__ khz, float tau, const Vox &v) {
-  const long N = g.N, i = v.i;
+  const long N = g.N;
+  const unsigned i = v.i;
   const Inv3 d1{rp.idx, rp.idy, rp.idt}, d2{rp.idx2, rp.idy2, rp.idt2};
   const DualAcc<DT> Y1{y, rp.r1}, Y2{y, rp.r2}, Y3{y, rp.r3}, Y4{y, rp.r4};
   float2 upd[8];
   const float2 dv1 = div3(Y1, 0, N, 2 * N, v, g, d1);
-  upd[0] = csub(x[i], cscale(csub(khz[i], dv1), tau));
+  upd[0] = csub(gld(x, i), cscale(csub(gld(khz, i), dv1), tau));
   int P = 1;
   if (M != kTV) {
     float2 sd[3];
     symdiv3(Y2, 3 * N, N, v, g, d1, sd);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      upd[1 + c] = cadd(x[(1 + c) * N + i], cscale(cadd(Y1(c * N + i), sd[c]), tau));
+      upd[1 + c] = cadd(gld(x + (1 + c) * N, i), cscale(cadd(Y1.at(c * N)(i), sd[c]), tau));
     P = 4;
     if (M == kICTGV2) {
       const float2 dv3 = div3(Y3, 9 * N, 10 * N, 11 * N, v, g, d2);
-      upd[4] = csub(x[4 * N + i], cscale(csub(dv1, dv3), tau));
+      upd[4] = csub(gld(x + 4 * N, i), cscale(csub(dv1, dv3), tau));
       symdiv3(Y4, 12 * N, N, v, g, d2, sd);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        upd[5 + c] = cadd(x[(5 + c) * N + i], cscale(cadd(Y3((9 + c) * N + i), sd[c]), tau));
+        upd[5 + c] = cadd(gld(x + (5 + c) * N, i), cscale(cadd(Y3.at((9 + c) * N)(i), sd[c]), tau));
       P = 8;
     }
   }
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     if (c < P) {
-      const long j = c * N + i;
-      const float2 xo = x[j];
-      ext[j] = csub(cscale(upd[c], 2.f), xo);
-      x[j] = upd[c];
+      float2 *xc = x + c * N, *ec = ext + c * N;
+      const float2 xo = gld(xc, i);
+      gst(ec, i, csub(cscale(upd[c], 2.f), xo));
+      gst(xc, i, upd[c]);
     }
   }
 }
@@ -296,12 +294,13 @@ template <int M>
 __device__ inline void step_voxel(const Grid3 &g, const RegParams &rp, const float2 *__restrict__ ext,
                                   const float2 *__restrict__ x, const Vox &v, double &ak,
                                   double &ax) {
-  const long N = g.N, i = v.i;
+  const long N = g.N;
+  const unsigned i = v.i;
   const Inv3 d1{rp.idx, rp.idy, rp.idt}, d2{rp.idx2, rp.idy2, rp.idt2};
-  DiffAcc D{ext, x};
+  const DiffAcc D{ext, x};
   float2 gr[3], e[6];
   const int P = (M == kTV) ? 1 : (M == kTGV2 ? 4 : 8);
-  for (int c = 0; c < P; ++c) ax += (double)cabs2(D(c * N + i));
+  for (int c = 0; c < P; ++c) ax += (double)cabs2(D.at(c * N)(i));
   if (M == kTV) {
     grad3(D, 0, v, g, d1, gr);
     for (int c = 0; c < 3; ++c) ak += (double)cabs2(gr[c]);
@@ -310,15 +309,14 @@ __device__ inline void step_voxel(const Grid3 &g, const RegParams &rp, const flo
   if (M == kTGV2) {
     grad3(D, 0, v, g, d1, gr);
   } else {
-    DiffSubAcc DUW{ext, x, 4 * N};
-    grad3(DUW, 0, v, g, d1, gr);
+    grad3(Diff2Acc{ext, x, ext + 4 * N, x + 4 * N}, 0, v, g, d1, gr);
   }
-  for (int c = 0; c < 3; ++c) ak += (double)cabs2(csub(gr[c], D((1 + c) * N + i)));
+  for (int c = 0; c < 3; ++c) ak += (double)cabs2(csub(gr[c], D.at((1 + c) * N)(i)));
   symgrad6(D, N, 2 * N, 3 * N, v, g, d1, e);
   for (int k = 0; k < 6; ++k) ak += (k < 3 ? 1.0 : 2.0) * (double)cabs2(e[k]);
   if (M == kICTGV2) {
     grad3(D, 4 * N, v, g, d2, gr);
-    for (int c = 0; c < 3; ++c) ak += (double)cabs2(csub(gr[c], D((5 + c) * N + i)));
+    for (int c = 0; c < 3; ++c) ak += (double)cabs2(csub(gr[c], D.at((5 + c) * N)(i)));
     symgrad6(D, 5 * N, 6 * N, 7 * N, v, g, d2, e);
     for (int k = 0; k < 6; ++k) ak += (k < 3 ? 1.0 : 2.0) * (double)cabs2(e[k]);
   }
