@@ -49,8 +49,10 @@ struct ColPlanCt {
   static constexpr int H = H_, XB = XB_, R0 = R0_, R1 = R1_, R2 = R2_;
   static_assert(H_ * XB_ <= kColTileElems, "column tile exceeds the LDS tile");
   static_assert(R0_ * R1_ * R2_ == H_, "radices must multiply to the length");
+  static_assert(H_ <= 256, "compile-time plans are at most 256 long");
 };
 // The radices in make_fft_plan's order; CartGeom::cplan names one.
+constexpr int kColPlanMaxLen = 256;  // longest compile-time plan
 using ColPlan256 = ColPlanCt<256, 16, 16, 16>;
 using ColPlan128 = ColPlanCt<128, 32, 16, 8>;
 using ColPlan160 = ColPlanCt<160, 16, 16, 2, 5>;
@@ -80,26 +82,189 @@ __device__ __forceinline__ void col_fft(float2 *tile, const CartGeom &g, float s
   }
 }
 
+// Compile-time plans.  Every load of a lane's first stage is issued before
+// any of them is used (and the adjoint's coil-map loads before the last
+// stage's arithmetic): without the scheduling barrier the compiler places
+// each complex product right after its two loads, and the wait before it
+// leaves one pair of loads in flight per lane.
+#ifndef AVIONIC_COL_SPLIT
+#define AVIONIC_COL_SPLIT 0
+#endif
+// SPLIT: exchange between radix stages through a tile of floats, the real
+// parts and then the imaginary parts (half the LDS of a complex tile, two
+// more barriers per transform).
+constexpr bool kColSplit = AVIONIC_COL_SPLIT != 0;
+#ifndef AVIONIC_COL_ADJ_PRE
+#define AVIONIC_COL_ADJ_PRE 1
+#endif
+// adjoint: load the coil map values of the lane's outputs before the last
+// stage (32 more VGPRs) instead of next to each product
+constexpr bool kColAdjPre = AVIONIC_COL_ADJ_PRE != 0;
+
+template <int R_, int L, int B, int Ns>
+struct CtStage {
+  static constexpr int R = R_, NBF = L / R_, TOTAL = NBF * B, MAXB = (TOTAL + NTC - 1) / NTC;
+  // c-th butterfly of lane tid: column b, butterfly j; false past the end
+  __device__ static bool lane(int tid, int c, int &b, int &j) {
+    const int idx = tid + c * NTC;
+    b = idx % B;
+    j = idx / B;
+    return idx < TOTAL;
+  }
+  __device__ static int in_index(int j, int r) { return j + r * NBF; }
+  __device__ static int out_index(int j, int r) { return (j / Ns) * Ns * R + j % Ns + r * Ns; }
+  __device__ static void butterfly(float2 *v, int j, const float2 *tw, float sign) {
+    if (Ns > 1) {
+      const int k = j % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[r * k * (L / (Ns * R))];
+        w.y = -sign * w.y;  // tw holds exp(-i theta)
+        v[r] = cmul(v[r], w);
+      }
+    }
+    Dft<R>::run(v, sign);
+  }
+};
+
+// Outputs v of stage S0 become inputs w of stage S1 through the LDS tile
+// (element (b, i) at b + i B).  Callers sync before writing the tile again.
+template <int B, class S0, class S1>
+__device__ __forceinline__ void ct_exchange(void *tile, float2 (&v)[S0::MAXB][S0::R],
+                                            float2 (&w)[S1::MAXB][S1::R], int tid) {
+  int b, j;
+  if constexpr (!kColSplit) {
+    float2 *t = (float2 *)tile;
+#pragma unroll
+    for (int c = 0; c < S0::MAXB; ++c)
+      if (S0::lane(tid, c, b, j))
+#pragma unroll
+        for (int r = 0; r < S0::R; ++r) t[b + S0::out_index(j, r) * B] = v[c][r];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < S1::MAXB; ++c)
+      if (S1::lane(tid, c, b, j))
+#pragma unroll
+        for (int r = 0; r < S1::R; ++r) w[c][r] = t[b + S1::in_index(j, r) * B];
+  } else {
+    float *t = (float *)tile;
+#pragma unroll
+    for (int c = 0; c < S0::MAXB; ++c)
+      if (S0::lane(tid, c, b, j))
+#pragma unroll
+        for (int r = 0; r < S0::R; ++r) t[b + S0::out_index(j, r) * B] = v[c][r].x;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < S1::MAXB; ++c)
+      if (S1::lane(tid, c, b, j))
+#pragma unroll
+        for (int r = 0; r < S1::R; ++r) w[c][r].x = t[b + S1::in_index(j, r) * B];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < S0::MAXB; ++c)
+      if (S0::lane(tid, c, b, j))
+#pragma unroll
+        for (int r = 0; r < S0::R; ++r) t[b + S0::out_index(j, r) * B] = v[c][r].y;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < S1::MAXB; ++c)
+      if (S1::lane(tid, c, b, j))
+#pragma unroll
+        for (int r = 0; r < S1::R; ++r) w[c][r].y = t[b + S1::in_index(j, r) * B];
+  }
+}
+
+// Last stage S: prefetch what the store functor reads, transform, store.
+template <class S, class ST>
+__device__ __forceinline__ void ct_last(float2 (&w)[S::MAXB][S::R], int tid, const float2 *tw,
+                                        float sign, ST &st) {
+  int b, j;
+#pragma unroll
+  for (int c = 0; c < S::MAXB; ++c)
+    if (S::lane(tid, c, b, j))
+#pragma unroll
+      for (int r = 0; r < S::R; ++r) st.prefetch(b, S::out_index(j, r), c * S::R + r);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int c = 0; c < S::MAXB; ++c)
+    if (S::lane(tid, c, b, j)) {
+      S::butterfly(w[c], j, tw, sign);
+#pragma unroll
+      for (int r = 0; r < S::R; ++r) st(b, S::out_index(j, r), w[c][r], c * S::R + r);
+    }
+}
+
+// Column transform of one tile on a compile-time plan, first stage from
+// `ld` (raw loads, then finish), last stage to `st`.
+template <class P, class LD, class ST>
+__device__ __forceinline__ void col_fft_ct(void *tile, float sign, int tid, const float2 *tw,
+                                           LD &ld, ST &st) {
+  asm volatile("" : "+v"(tid));  // per-lane index math stays inside the coil loop
+  constexpr int L = P::H, B = P::XB;
+  using S0 = CtStage<P::R0, L, B, 1>;
+  using S1 = CtStage<P::R1, L, B, P::R0>;
+  int b, j;
+  typename LD::Raw raw[S0::MAXB][S0::R];
+#pragma unroll
+  for (int c = 0; c < S0::MAXB; ++c)
+    if (S0::lane(tid, c, b, j))
+#pragma unroll
+      for (int r = 0; r < S0::R; ++r) raw[c][r] = ld.raw(b, S0::in_index(j, r));
+  __builtin_amdgcn_sched_barrier(0);
+  float2 v[S0::MAXB][S0::R];
+#pragma unroll
+  for (int c = 0; c < S0::MAXB; ++c)
+    if (S0::lane(tid, c, b, j)) {
+#pragma unroll
+      for (int r = 0; r < S0::R; ++r) v[c][r] = ld.finish(raw[c][r], b);
+      S0::butterfly(v[c], j, tw, sign);
+    }
+  float2 w[S1::MAXB][S1::R];
+  ct_exchange<B, S0, S1>(tile, v, w, tid);
+  if constexpr (P::R2 == 1) {
+    ct_last<S1>(w, tid, tw, sign, st);
+  } else {
+    using S2 = CtStage<P::R2, L, B, P::R0 * P::R1>;
+#pragma unroll
+    for (int c = 0; c < S1::MAXB; ++c)
+      if (S1::lane(tid, c, b, j)) S1::butterfly(w[c], j, tw, sign);
+    float2 x[S2::MAXB][S2::R];
+    __syncthreads();  // every lane has read the previous exchange
+    ct_exchange<B, S1, S2>(tile, w, x, tid);
+    ct_last<S2>(x, tid, tw, sign, st);
+  }
+}
+
 // ------------------------------------------------------------ column forward
 // 32-bit offsets from per-(t, c) base pointers (uniform, in SGPRs) keep one
 // VGPR per in-flight address.  The image tile is re-read per coil (an L2
 // hit): keeping it in registers costs occupancy, and on MI355X 256-lane
 // blocks without that cache were fastest (0.91 ms against 1.94 ms for
 // 512-lane blocks with it, ICTGV2 256x256x30x16, 8 reconstructions).
-// HS: the transform length when compile-time (0: runtime, H).
+// HS: the transform length when compile-time (0: runtime, H).  Lanes of
+// columns past W load a valid element and drop it: a load in a divergent
+// branch whose value is used after it gets its own wait, which serialises the
+// lane's loads.
 template <bool DELTA, int HS>
 struct ColFwdLoad {
   const float2 *ut, *st, *bc;
   int H, W, xb0, centered;
-  __device__ float2 operator()(int b, int i, int) const {
+  struct Raw {
+    float2 b, u, s;
+  };
+  __device__ Raw raw(int b, int i) const {
     const int Hh = HS ? HS : H;
     const int x = xb0 + b;
-    if (x >= W) return czero();
+    const int xc = x < W ? x : W - 1;  // every lane loads in bounds: no branch
     const int sy = centered ? wrap_half(i, Hh) : i;
-    const int sx = centered ? wrap_half(x, W) : x;
+    const int sx = centered ? wrap_half(xc, W) : xc;
     const unsigned si = (unsigned)(sy * W + sx);
-    return cmul(gld(bc, si), DELTA ? csub(gld(ut, si), gld(st, si)) : gld(ut, si));
+    return Raw{gld(bc, si), gld(ut, si), DELTA ? gld(st, si) : czero()};
   }
+  __device__ float2 finish(const Raw &r, int b) const {
+    return xb0 + b < W ? cmul(r.b, DELTA ? csub(r.u, r.s) : r.u) : czero();
+  }
+  __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i), b); }
 };
 template <int HS>
 struct ColFwdStore {
@@ -109,11 +274,11 @@ struct ColFwdStore {
   __device__ void operator()(int b, int i, float2 v, int) {
     const int Hh = HS ? HS : H;
     const int x = xb0 + b;
-    if (x >= W) return;
     const int ky = centered ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
-    if (k >= 0) gst(kc, (unsigned)(k * W + x), v);
+    if (x < W && k >= 0) gst(kc, (unsigned)(k * W + x), v);
   }
+  __device__ void prefetch(int, int, int) {}
 };
 
 // The frame's rowmap (compact row of each k-space row, -1 if not sampled)
@@ -128,7 +293,7 @@ __device__ inline const int *stage_rowmap(const CartGeom &g, int t, int *lds, in
 template <bool DELTA, class P>
 __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, const float2 *u,
                                          const float2 *usub, const float2 *b1, float2 *kcol,
-                                         float2 *tile, float2 *twl, int *rml) {
+                                         void *tile, float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
   const int XB = P::XB ? P::XB : g.XBF;
@@ -147,53 +312,73 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
   for (int c = 0; c < g.C; ++c) {
     ld.bc = b1 + c * HW;
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
-    col_fft<P>(tile, g, +1.f, tid, tw, ld, st);
+    if constexpr (P::H == 0)
+      col_fft<P>((float2 *)tile, g, +1.f, tid, tw, ld, st);
+    else
+      col_fft_ct<P>(tile, +1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
 
 // ------------------------------------------------------------ column adjoint
+// Lanes of unsampled rows (and columns past W) load element 0 of zh, a
+// cached dummy, so that the loads of a lane's outputs are not in divergent
+// branches: a load in a branch whose value is used after it gets its own
+// wait, which serialises the lane's loads.
 template <int HS>
 struct ColAdjLoad {
-  const float2 *zh;  // zhat rows of (t, c)
+  const float2 *zh;  // zhat rows of (t, c); when the frame has none, any valid element
   const int *rmap;
   int H, W, xb0, centered;
-  __device__ float2 operator()(int b, int i, int) const {
+  struct Raw {
+    float2 z;
+    bool use;
+  };
+  __device__ Raw raw(int b, int i) const {
     const int Hh = HS ? HS : H;
     const int x = xb0 + b;
-    if (x >= W) return czero();
     const int ky = centered ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
-    if (k < 0) return czero();
-    return gld(zh, (unsigned)(k * W + x));
+    const bool use = x < W && k >= 0;
+    return Raw{gld(zh, use ? (unsigned)(k * W + x) : 0u), use};
   }
+  __device__ float2 finish(const Raw &r, int) const { return r.use ? r.z : czero(); }
+  __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i), 0); }
 };
 // The coil sum stays in registers, one accumulator per output of the lane.
-template <int HS>
+// PRE: the coil map values are loaded by prefetch() ahead of the last stage.
+template <int HS, bool PRE>
 struct ColAdjStore {
   float2 acc[kColSlots];
+  float2 bpre[PRE ? kColSlots : 1];
   const float2 *bc;
   float2 *out;  // image frame t
   int H, W, xb0, centered;
   bool last;
   float scale;
-  __device__ void operator()(int b, int i, float2 v, int slot) {
+  __device__ unsigned index(int b, int i) const {
     const int Hh = HS ? HS : H;
     const int x = xb0 + b;
-    if (x >= W) return;
+    const int xc = x < W ? x : W - 1;  // b1 load in bounds for every lane: no branch
     const int y = centered ? wrap_half(i, Hh) : i;
-    const unsigned o = (unsigned)(y * W + x);
-    const float2 a = cadd(acc[slot], cmulc(gld(bc, o), v));
-    if (last)
-      gst(out, o, cscale(a, scale));
-    else
+    return (unsigned)(y * W + xc);
+  }
+  __device__ void prefetch(int b, int i, int slot) {
+    if (PRE) bpre[slot] = gld(bc, index(b, i));
+  }
+  __device__ void operator()(int b, int i, float2 v, int slot) {
+    const unsigned o = index(b, i);
+    const float2 a = cadd(acc[slot], cmulc(PRE ? bpre[slot] : gld(bc, o), v));
+    if (!last)
       acc[slot] = a;
+    else if (xb0 + b < W)
+      gst(out, o, cscale(a, scale));
   }
 };
 
 template <class P>
 __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, const float2 *zhat,
-                                         const float2 *b1, float2 *out, float2 *tile,
+                                         const float2 *b1, float2 *out, void *tile,
                                          float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
@@ -208,7 +393,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   ld.W = g.W;
   ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColAdjStore<P::H> st;
+  ColAdjStore<P::H, P::H != 0 && kColAdjPre> st;
 #pragma unroll
   for (int k = 0; k < kColSlots; ++k) st.acc[k] = czero();
   st.out = out + t * HW;
@@ -218,10 +403,13 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   st.centered = g.centered;
   st.scale = g.scale;
   for (int c = 0; c < g.C; ++c) {
-    ld.zh = zhat + ((long)g.C * r0 + (long)c * S) * g.W;
+    ld.zh = S > 0 ? zhat + ((long)g.C * r0 + (long)c * S) * g.W : zhat;
     st.bc = b1 + c * HW;
     st.last = (c == g.C - 1);
-    col_fft<P>(tile, g, -1.f, tid, tw, ld, st);
+    if constexpr (P::H == 0)
+      col_fft<P>((float2 *)tile, g, -1.f, tid, tw, ld, st);
+    else
+      col_fft_ct<P>(tile, -1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }

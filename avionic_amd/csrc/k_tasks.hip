@@ -85,9 +85,10 @@ __global__ __launch_bounds__(RT) void kt_step_reg(const Task *__restrict__ tasks
 // plans only; a launch skips the blocks of the other kind).
 template <bool DELTA, bool CT>
 __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n) {
-  __shared__ float2 tile[kColTileElems];
-  __shared__ float2 twl[kMaxFftLen];
-  __shared__ int rml[kMaxFftLen];
+  // compile-time plans: transforms of at most 256, optionally a float tile
+  __shared__ float2 tile[CT && kColSplit ? kColTileElems / 2 : kColTileElems];
+  __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
+  __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
   if ((T.cg.cplan != kColPlanRt) != CT) return;
   if (!task_active(T.sc)) return;
@@ -141,23 +142,25 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
 }
 
 // The adjoint column pass keeps one accumulator per output element across
-// the coil loop; asking for 4 waves per SIMD (the limit the LDS tile sets
-// anyway) trims it from 130 to 128 VGPRs (col_adjoint 1.17 -> 0.98 ms,
-// ICTGV2 256x256x30x16, 8 reconstructions).
-#ifndef AVIONIC_COL_ADJ_WAVES
-#define AVIONIC_COL_ADJ_WAVES 4
+// the coil loop.  Runtime plans: asking for 4 waves per SIMD (the limit the
+// LDS tile sets anyway) trims it from 130 to 128 VGPRs (col_adjoint 1.17 ->
+// 0.98 ms, ICTGV2 256x256x30x16, 8 reconstructions).  Compile-time plans
+// also hold the prefetched coil map values; AVIONIC_COL_ADJ_WAVES_CT > 0
+// asks for that many waves per SIMD.
+#ifndef AVIONIC_COL_ADJ_WAVES_CT
+#define AVIONIC_COL_ADJ_WAVES_CT 0
 #endif
-#if AVIONIC_COL_ADJ_WAVES > 0
-#define AV_COL_ATTR __attribute__((amdgpu_waves_per_eu(AVIONIC_COL_ADJ_WAVES)))
+#if AVIONIC_COL_ADJ_WAVES_CT > 0
+#define AV_COL_ATTR_CT __attribute__((amdgpu_waves_per_eu(AVIONIC_COL_ADJ_WAVES_CT)))
 #else
-#define AV_COL_ATTR
+#define AV_COL_ATTR_CT
 #endif
 
 template <bool CT>
-__global__ __launch_bounds__(NTC) AV_COL_ATTR void kt_col_adjoint(const Task *__restrict__ tasks, int n) {
-  __shared__ float2 tile[kColTileElems];
-  __shared__ float2 twl[kMaxFftLen];
-  __shared__ int rml[kMaxFftLen];
+__device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks, int n) {
+  __shared__ float2 tile[CT && kColSplit ? kColTileElems / 2 : kColTileElems];
+  __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
+  __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
   if ((T.cg.cplan != kColPlanRt) != CT) return;
   if (!task_active(T.sc)) return;
@@ -180,6 +183,14 @@ __global__ __launch_bounds__(NTC) AV_COL_ATTR void kt_col_adjoint(const Task *__
       default: break;
     }
   }
+}
+__global__ __launch_bounds__(NTC) __attribute__((amdgpu_waves_per_eu(4)))
+void kt_col_adjoint_rt(const Task *__restrict__ tasks, int n) {
+  col_adjoint_block<false>(tasks, n);
+}
+__global__ __launch_bounds__(NTC) AV_COL_ATTR_CT
+void kt_col_adjoint_ct(const Task *__restrict__ tasks, int n) {
+  col_adjoint_block<true>(tasks, n);
 }
 
 // One block per task: step rule for adapting tasks, then the iteration
@@ -233,8 +244,8 @@ static void launch_col_fwd(const TaskGrid &gr, const Task *d_tasks, int n, hipSt
 }
 static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
   if (gr.col <= 0) return;
-  if (gr.col_ct) hipLaunchKernelGGL(kt_col_adjoint<true>, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
-  if (gr.col_rt) hipLaunchKernelGGL(kt_col_adjoint<false>, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+  if (gr.col_ct) hipLaunchKernelGGL(kt_col_adjoint_ct, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+  if (gr.col_rt) hipLaunchKernelGGL(kt_col_adjoint_rt, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
 }
 
 void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
