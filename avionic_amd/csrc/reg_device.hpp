@@ -87,16 +87,20 @@ struct Diff2Acc {
   }
 };
 
+// Branch-free, so that every load of a voxel is issued unconditionally (a
+// load under a divergent branch whose value is used after it gets its own
+// wait, which serialises the voxel's loads).  The neighbour index is clamped
+// to i at the boundary: the forward difference is then exactly 0 there, and
+// the backward difference c mc - p mp with the 0/1 factors mc = [pos != len-1],
+// mp = [pos != 0] gives c, -p and c - p exactly (finite values).
 template <class A>
 __device__ inline float2 fdiff(const A &a, unsigned i, unsigned s, int pos, int len) {
-  return (pos < len - 1) ? csub(a(i + s), a(i)) : czero();
+  return csub(a(pos < len - 1 ? i + s : i), a(i));
 }
 template <class A>
 __device__ inline float2 bdiff(const A &a, unsigned i, unsigned s, int pos, int len) {
-  if (len == 1) return czero();
-  if (pos == 0) return a(i);
-  if (pos == len - 1) return cscale(a(i - s), -1.f);
-  return csub(a(i), a(i - s));
+  const float mc = pos != len - 1 ? 1.f : 0.f, mp = pos != 0 ? 1.f : 0.f;
+  return csub(cscale(a(i), mc), cscale(a(pos > 0 ? i - s : i), mp));
 }
 
 struct Inv3 {
