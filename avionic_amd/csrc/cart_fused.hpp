@@ -500,6 +500,206 @@ __device__ inline void row_dual_fused(const CartGeom &g, int blk, const float2 *
   fft_io<NT, RTILE, false, false, true>(tile2, g.pw, nr, W, 1, -1.f, tid, tw, nl, st2);
 }
 
+// ------------------------------------------------- row pass, compile-time
+// Rows of length L = R0 * R1 (256: 16 x 16 on 16 rows per block, 128: 16 x 8
+// on 32 rows), every lane one butterfly of every stage.  The lane mapping is
+// rows-major (butterfly j fastest): the loads of a row are contiguous.  Per
+// block:
+//   forward stage 0 from kcol (all loads issued first) -> exchange ->
+//   z, d and mask values of the lane's outputs prefetched -> forward stage 1
+//   -> z ascent and prox per output (store z) -> M z stays in registers: the
+//   outputs i = j + R0 r of a lane's last forward butterfly are the inputs of
+//   its first inverse butterfly, and the centring shifts of K and K^H cancel
+//   ((i + W/2) + (W+1)/2 = i + W) -> inverse stage 0 -> exchange -> inverse
+//   stage 1 -> zhat.
+// The tile holds element (b, i) at b L + (i ^ ((i >> 4) & 15)): the stage-0
+// writes (i = 16 j + r over lanes j) and stage-1 reads (i = j + 16 r) then hit
+// distinct banks.
+template <int L_, int R0_, int R1_>
+struct RowPlanCt {
+  static constexpr int L = L_, R0 = R0_, R1 = R1_, ROWS = NT * R0_ / L_;
+  static_assert(R0_ * R1_ == L_ && L_ % 16 == 0, "two-stage plan on 16-element groups");
+  static_assert(ROWS * L_ <= kFftTileElems, "row tile exceeds the LDS tile");
+};
+using RowPlan256 = RowPlanCt<256, 16, 16>;
+using RowPlan128 = RowPlanCt<128, 16, 8>;
+constexpr int kRowPlanMaxRows = 32;
+
+template <int R_, int L, int ROWS, int Ns>
+struct RtStage {
+  static constexpr int R = R_, NBF = L / R_, TOTAL = NBF * ROWS, MAXB = (TOTAL + NT - 1) / NT;
+  static_assert(TOTAL % NT == 0, "every lane busy");
+  __device__ static void lane(int tid, int c, int &b, int &j) {
+    const int idx = tid + c * NT;
+    j = idx % NBF;
+    b = idx / NBF;
+  }
+  __device__ static int in_index(int j, int r) { return j + r * NBF; }
+  __device__ static int out_index(int j, int r) { return (j / Ns) * Ns * R + j % Ns + r * Ns; }
+  __device__ static void butterfly(float2 *v, int j, const float2 *tw, float sign) {
+    CtStage<R, L, 1, Ns>::butterfly(v, j, tw, sign);
+  }
+};
+__device__ __forceinline__ int row_swz(int b, int i, int L) { return b * L + (i ^ ((i >> 4) & 15)); }
+
+template <class S0, class S1, int L>
+__device__ __forceinline__ void rt_exchange(float2 *t, float2 (&v)[S0::MAXB][S0::R],
+                                            float2 (&w)[S1::MAXB][S1::R], int tid) {
+  int b, j;
+#pragma unroll
+  for (int c = 0; c < S0::MAXB; ++c) {
+    S0::lane(tid, c, b, j);
+#pragma unroll
+    for (int r = 0; r < S0::R; ++r) t[row_swz(b, S0::out_index(j, r), L)] = v[c][r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < S1::MAXB; ++c) {
+    S1::lane(tid, c, b, j);
+#pragma unroll
+    for (int r = 0; r < S1::R; ++r) w[c][r] = t[row_swz(b, S1::in_index(j, r), L)];
+  }
+}
+
+// MODE kRowDualUpdate: z update, zhat = F_row^-1(M z); kRowNorm2: returns the
+// lane's sum of |M F_row kcol|^2 (scaled).
+template <int MODE, class P>
+__device__ inline double row_fused_ct(const CartGeom &g, int blk, const float2 *kcol, float2 *z,
+                                      const float2 *d, float2 *zhat, const DevScalars *sc,
+                                      float2 *tile, float2 *twl, int *mrl) {
+  constexpr int L = P::L, ROWS = P::ROWS;
+  using F0 = RtStage<P::R0, L, ROWS, 1>;
+  using F1 = RtStage<P::R1, L, ROWS, P::R0>;
+  int tid = threadIdx.x;
+  const long g0 = (long)blk * ROWS;
+  const int nr = (int)min((long)ROWS, g.nrows_c - g0);
+  const float2 *tw = stage_twiddles(g.pw, twl, tid, NT);
+  const int *mrow = stage_mask_rows(g, g0, nr, mrl, tid, NT);
+  __syncthreads();
+  asm volatile("" : "+v"(tid));
+  const float2 *src = kcol + g0 * L;
+  int b, j;
+  // forward stage 0: rows past the block's last clamp to its last row
+  float2 v[F0::MAXB][F0::R];
+#pragma unroll
+  for (int c = 0; c < F0::MAXB; ++c) {
+    F0::lane(tid, c, b, j);
+    const int bc = min(b, nr - 1);
+#pragma unroll
+    for (int r = 0; r < F0::R; ++r) v[c][r] = gld(src, (unsigned)(bc * L + F0::in_index(j, r)));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int c = 0; c < F0::MAXB; ++c) {
+    F0::lane(tid, c, b, j);
+    F0::butterfly(v[c], j, tw, +1.f);
+  }
+  float2 w[F1::MAXB][F1::R];
+  rt_exchange<F0, F1, L>(tile, v, w, tid);
+  // prefetch what the outputs need
+  const int centered = g.centered;
+  float mk[F1::MAXB][F1::R];
+  float2 zz[F1::MAXB][F1::R], dd[F1::MAXB][F1::R];
+#pragma unroll
+  for (int c = 0; c < F1::MAXB; ++c) {
+    F1::lane(tid, c, b, j);
+    const int bc = min(b, nr - 1);
+    const unsigned mbase = (unsigned)(mrow[bc] * L);
+#pragma unroll
+    for (int r = 0; r < F1::R; ++r) {
+      const int i = F1::out_index(j, r);
+      const int kx = centered ? wrap_half(i, L) : i;
+      mk[c][r] = gld(g.mrows, mbase + kx);
+      if (MODE == kRowDualUpdate) {
+        const unsigned gi = (unsigned)(bc * L + kx);
+        zz[c][r] = gld(z + g0 * L, gi);
+        dd[c][r] = gld(d + g0 * L, gi);
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  const float scale = g.scale;
+  if (MODE == kRowNorm2) {
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < F1::MAXB; ++c) {
+      F1::lane(tid, c, b, j);
+      F1::butterfly(w[c], j, tw, +1.f);
+      if (b < nr)
+#pragma unroll
+        for (int r = 0; r < F1::R; ++r) acc += (double)cabs2(cscale(w[c][r], scale * mk[c][r]));
+    }
+    return acc;
+  }
+  const float sigma = (float)sc->sigma;
+  const float inv = (float)(1.0 / (1.0 + sc->sigma / sc->lambda));
+  float2 *zb = z + g0 * L;
+  // forward stage 1, z update per output: M z at natural index i
+#pragma unroll
+  for (int c = 0; c < F1::MAXB; ++c) {
+    F1::lane(tid, c, b, j);
+    F1::butterfly(w[c], j, tw, +1.f);
+#pragma unroll
+    for (int r = 0; r < F1::R; ++r) {
+      const int i = F1::out_index(j, r);
+      const int kx = centered ? wrap_half(i, L) : i;
+      const float2 val = cscale(w[c][r], scale * mk[c][r]);
+      float2 zn = cadd(zz[c][r], cscale(val, sigma));
+      zn = cscale(csub(zn, cscale(dd[c][r], sigma)), inv);
+      if (b < nr) gst(zb, (unsigned)(b * L + kx), zn);
+      w[c][r] = cscale(zn, mk[c][r]);
+    }
+  }
+  // inverse: stage 0 takes M z at natural indices j + R0 r of its row
+  using I0 = F0;
+  using I1 = F1;
+  float2 u[I0::MAXB][I0::R];
+  if constexpr (P::R0 == P::R1) {
+    // same lane mapping: a lane's forward outputs are its inverse inputs
+#pragma unroll
+    for (int c = 0; c < I0::MAXB; ++c)
+#pragma unroll
+      for (int r = 0; r < I0::R; ++r) u[c][r] = w[c][r];
+    __syncthreads();  // every lane has read the forward exchange
+  } else {
+    __syncthreads();  // every lane has read the forward exchange
+#pragma unroll
+    for (int c = 0; c < F1::MAXB; ++c) {
+      F1::lane(tid, c, b, j);
+#pragma unroll
+      for (int r = 0; r < F1::R; ++r) tile[row_swz(b, F1::out_index(j, r), L)] = w[c][r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < I0::MAXB; ++c) {
+      I0::lane(tid, c, b, j);
+#pragma unroll
+      for (int r = 0; r < I0::R; ++r) u[c][r] = tile[row_swz(b, I0::in_index(j, r), L)];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < I0::MAXB; ++c) {
+    I0::lane(tid, c, b, j);
+    I0::butterfly(u[c], j, tw, -1.f);
+  }
+  float2 x[I1::MAXB][I1::R];
+  rt_exchange<I0, I1, L>(tile, u, x, tid);
+  float2 *hb = zhat + g0 * L;
+#pragma unroll
+  for (int c = 0; c < I1::MAXB; ++c) {
+    I1::lane(tid, c, b, j);
+    I1::butterfly(x[c], j, tw, -1.f);
+    if (b < nr)
+#pragma unroll
+      for (int r = 0; r < I1::R; ++r) {
+        const int i = I1::out_index(j, r);
+        gst(hb, (unsigned)(b * L + (centered ? wrap_half(i, L) : i)), x[c][r]);
+      }
+  }
+  return 0.0;
+}
+
 __device__ inline double row_norm2_fused(const CartGeom &g, int blk, const float2 *kcol,
                                          float2 *tile, float2 *twl, int *mrl) {
   const int tid = threadIdx.x;

@@ -27,6 +27,8 @@ struct CartGeom {
   int nxb;
   int XBF, nxbf;  // columns per tile / tiles of the hot loop's fused column passes
   int cplan;      // compile-time plan of the fused column passes (ColPlanId)
+  int rplan;      // compile-time plan of the fused row pass (RowPlanId)
+  int RBF;        // rows per block of the fused row pass
   long nrows;          // sampled rows over all frames (row_off[T])
   long nrows_c;        // C * nrows
   const int *row_off;  // [T+1]
@@ -41,6 +43,8 @@ struct CartGeom {
 // Compile-time column plans (transform length H, XBF columns per tile); any
 // other geometry runs the runtime plan.
 enum ColPlanId { kColPlanRt = 0, kColPlan256, kColPlan128, kColPlan160, kColPlan192, kColPlan224 };
+// Compile-time row plans (row length W); rows per block 16 and 32.
+enum RowPlanId { kRowPlanRt = 0, kRowPlan256, kRowPlan128 };
 
 // Row-pass modes.
 enum RowMode {
@@ -72,6 +76,7 @@ struct CartPlan {
   void init(int W, int H, int C, int T, const float *mask, int centered, hipStream_t s);
   long kspace_elems() const { return g.nrows_c * (long)g.W; }
   int row_blocks() const { return (int)((g.nrows_c + g.RB - 1) / g.RB); }
+  int fused_row_blocks() const { return (int)((g.nrows_c + g.RBF - 1) / g.RBF); }
 };
 
 void make_fft_plan(int L, FftPlan &p, void **d_tw);

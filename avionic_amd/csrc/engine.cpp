@@ -107,6 +107,7 @@ struct avionic_engine {
     for (Slot *s : g.slots) {
       Task t = s->task;
       (t.cg.cplan != kColPlanRt ? g.grid.col_ct : g.grid.col_rt) = true;
+      (t.cg.rplan != kRowPlanRt ? g.grid.row_ct : g.grid.row_rt) = true;
       t.b_vox = bv;
       t.b_col = bc;
       t.b_row = br;
@@ -287,7 +288,7 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->tlog.ensure(sizeof(double) * std::max(1, s->maxit));
     const Grid3 g{s->W, s->H, s->T, N};
     const int nvb = step_blocks(g);
-    const int nrb = s->op->plan().row_blocks();
+    const int nrb = s->op->plan().fused_row_blocks();
     s->preg.ensure(sizeof(double) * 2 * nvb);
     s->pk.ensure(sizeof(double) * std::max(1, nrb));
     AV_HIP(hipMemsetAsync(s->x.p, 0, s->x.bytes, st));

@@ -168,6 +168,8 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   g.XBF = pick_xb(H, W, kColTileElems);
   g.nxbf = (W + g.XBF - 1) / g.XBF;
   g.cplan = pick_col_plan(H, g.XBF);
+  g.rplan = W == 256 ? kRowPlan256 : (W == 128 ? kRowPlan128 : kRowPlanRt);
+  g.RBF = g.rplan == kRowPlan256 ? 16 : (g.rplan == kRowPlan128 ? 32 : g.RB);
   g.scale = (float)(1.0 / std::sqrt((double)H * W));
   const long HW = (long)H * W;
   std::vector<float> hm;

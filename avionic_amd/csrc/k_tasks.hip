@@ -115,21 +115,33 @@ __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ t
   }
 }
 
-template <int MODE>
+// Row pass.  CT = true: tasks whose row length has a compile-time plan
+// (16 or 32 rows per block), CT = false: the others (RB rows per block).
+template <int MODE, bool CT>
 __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int n) {
-  __shared__ float2 tile[RTILE];
-  __shared__ float2 tile2[(MODE == kRowDualUpdate) ? RTILE : 1];
-  __shared__ float2 twl[kMaxFftLen];
+  __shared__ float2 tile[CT ? kFftTileElems : RTILE];
+  __shared__ float2 tile2[(MODE == kRowDualUpdate && !CT) ? RTILE : 1];
+  __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ double red[NT];
-  __shared__ int mrl[kRowMaxRows];
+  __shared__ int mrl[CT ? kRowPlanMaxRows : kRowMaxRows];
   const Task &T = tasks[find_task<kRow>(tasks, n, blockIdx.x)];
+  if ((T.cg.rplan != kRowPlanRt) != CT) return;
   if (!task_active(T.sc)) return;
   if (MODE == kRowNorm2 && !task_adapt(T.sc)) return;
   const int lb = blockIdx.x - T.b_row;
-  if (MODE == kRowDualUpdate) {
-    row_dual_fused(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, tile2, twl, mrl);
+  double a = 0.0;
+  if constexpr (CT) {
+    if (T.cg.rplan == kRowPlan256)
+      a = row_fused_ct<MODE, RowPlan256>(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, twl, mrl);
+    else
+      a = row_fused_ct<MODE, RowPlan128>(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, twl, mrl);
   } else {
-    const double a = row_norm2_fused(T.cg, lb, T.kcol, tile, twl, mrl);
+    if (MODE == kRowDualUpdate)
+      row_dual_fused(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, tile2, twl, mrl);
+    else
+      a = row_norm2_fused(T.cg, lb, T.kcol, tile, twl, mrl);
+  }
+  if (MODE == kRowNorm2) {
     const int tid = threadIdx.x;
     red[tid] = a;
     __syncthreads();
@@ -248,6 +260,13 @@ static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipSt
   if (gr.col_rt) hipLaunchKernelGGL(kt_col_adjoint_rt, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
 }
 
+template <int MODE>
+static void launch_row(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
+  if (gr.row <= 0) return;
+  if (gr.row_ct) hipLaunchKernelGGL((kt_row<MODE, true>), dim3(gr.row), dim3(NT), 0, s, d_tasks, n);
+  if (gr.row_rt) hipLaunchKernelGGL((kt_row<MODE, false>), dim3(gr.row), dim3(NT), 0, s, d_tasks, n);
+}
+
 void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
                    hipStream_t s) {
   if (n <= 0) return;
@@ -271,7 +290,7 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
   switch (id) {
     case kKDual: if (gr.vox) AV_BYMETHOD2(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
     case kKColFwd: launch_col_fwd<false>(gr, d_tasks, n, s); break;
-    case kKRowDual: if (gr.row) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
+    case kKRowDual: launch_row<kRowDualUpdate>(gr, d_tasks, n, s); break;
     case kKColAdj: launch_col_adj(gr, d_tasks, n, s); break;
     case kKPrimal: if (gr.vox) AV_BYMETHOD2(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
     case kKStepReg:
@@ -282,7 +301,7 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
       }
       break;
     case kKColFwdDelta: launch_col_fwd<true>(gr, d_tasks, n, s); break;
-    case kKRowNorm2: if (gr.row) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n); break;
+    case kKRowNorm2: launch_row<kRowNorm2>(gr, d_tasks, n, s); break;
     case kKTail: hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n); break;
     default: break;
   }
@@ -329,7 +348,7 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
   AV_LAUNCH_CHECK();
   launch_col_fwd<false>(gr, d_tasks, n, s);
-  if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowDualUpdate>, dim3(gr.row), bNT, 0, s, d_tasks, n);
+  launch_row<kRowDualUpdate>(gr, d_tasks, n, s);
   launch_col_adj(gr, d_tasks, n, s);
   AV_LAUNCH_CHECK();
   join();
@@ -345,7 +364,7 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
     }
     AV_LAUNCH_CHECK();
     launch_col_fwd<true>(gr, d_tasks, n, s);
-    if (gr.row > 0) hipLaunchKernelGGL(kt_row<kRowNorm2>, dim3(gr.row), bNT, 0, s, d_tasks, n);
+    launch_row<kRowNorm2>(gr, d_tasks, n, s);
     AV_LAUNCH_CHECK();
     join();
   }

@@ -45,6 +45,7 @@ struct Task {
 struct TaskGrid {
   int vox = 0, col = 0, row = 0;
   bool col_ct = false, col_rt = false;  // tasks with a compile-time / runtime column plan
+  bool row_ct = false, row_rt = false;  // ... row plan
 };
 
 // Streams of one iteration: the regulariser kernels that do not depend on

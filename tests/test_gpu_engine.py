@@ -87,7 +87,9 @@ def test_engine_mixed_batch_matches_alone_and_oracle(gpu):
 
 # One case per compile-time column plan (transform length 256 / 128 / 160 /
 # 192 / 224 with its tile width; widths that leave a partial last tile, both
-# centring conventions) and one runtime-plan geometry, resident together.
+# centring conventions), one runtime-plan geometry, and the two compile-time
+# row plans (row lengths 256 and 128, sampled-row counts that leave a partial
+# last row block), resident together.
 PLAN_CASES = [  # method, T, H, W, C, maxIt, centered
     (2, 2, 256, 40, 2, 12, False),
     (1, 2, 128, 48, 3, 12, True),
@@ -95,6 +97,8 @@ PLAN_CASES = [  # method, T, H, W, C, maxIt, centered
     (2, 2, 192, 20, 2, 12, True),
     (1, 2, 224, 36, 2, 12, False),
     (2, 2, 96, 40, 2, 12, False),
+    (2, 2, 64, 256, 3, 12, True),    # compile-time row plans (W 256 and 128)
+    (0, 3, 40, 128, 2, 12, False),
 ]
 
 
