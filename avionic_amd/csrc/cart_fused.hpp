@@ -245,7 +245,9 @@ __device__ __forceinline__ void col_fft_ct(void *tile, float sign, int tid, cons
 // columns past W load a valid element and drop it: a load in a divergent
 // branch whose value is used after it gets its own wait, which serialises the
 // lane's loads.
-template <bool DELTA, int HS>
+// CEN: the centring flag when compile-time (0 / 1; -1: runtime, centered).
+// Index products use 24-bit multiplies (rows and columns are below 2^12).
+template <bool DELTA, int HS, int CEN = -1>
 struct ColFwdLoad {
   const float2 *ut, *st, *bc;
   int H, W, xb0, centered;
@@ -254,11 +256,12 @@ struct ColFwdLoad {
   };
   __device__ Raw raw(int b, int i) const {
     const int Hh = HS ? HS : H;
+    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
     const int x = xb0 + b;
     const int xc = x < W ? x : W - 1;  // every lane loads in bounds: no branch
-    const int sy = centered ? wrap_half(i, Hh) : i;
-    const int sx = centered ? wrap_half(xc, W) : xc;
-    const unsigned si = (unsigned)(sy * W + sx);
+    const int sy = cen ? wrap_half(i, Hh) : i;
+    const int sx = cen ? wrap_half(xc, W) : xc;
+    const unsigned si = __umul24((unsigned)sy, (unsigned)W) + (unsigned)sx;
     return Raw{gld(bc, si), gld(ut, si), DELTA ? gld(st, si) : czero()};
   }
   __device__ float2 finish(const Raw &r, int b) const {
@@ -266,17 +269,18 @@ struct ColFwdLoad {
   }
   __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i), b); }
 };
-template <int HS>
+template <int HS, int CEN = -1>
 struct ColFwdStore {
   float2 *kc;       // kcol + (C r0 + c S) W : rows of (t, c)
   const int *rmap;  // LDS: compact row of each k-space row of frame t
   int H, W, xb0, centered;
   __device__ void operator()(int b, int i, float2 v, int) {
     const int Hh = HS ? HS : H;
+    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
     const int x = xb0 + b;
-    const int ky = centered ? wrap_half(i, Hh) : i;
+    const int ky = cen ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
-    if (x < W && k >= 0) gst(kc, (unsigned)(k * W + x), v);
+    if (x < W && k >= 0) gst(kc, __umul24((unsigned)k, (unsigned)W) + (unsigned)x, v);
   }
   __device__ void prefetch(int, int, int) {}
 };
@@ -290,7 +294,7 @@ __device__ inline const int *stage_rowmap(const CartGeom &g, int t, int *lds, in
   return lds;
 }
 
-template <bool DELTA, class P>
+template <bool DELTA, class P, int CEN = -1>
 __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, const float2 *u,
                                          const float2 *usub, const float2 *b1, float2 *kcol,
                                          void *tile, float2 *twl, int *rml) {
@@ -301,14 +305,14 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
   const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColFwdLoad<DELTA, P::H> ld;
+  ColFwdLoad<DELTA, P::H, CEN> ld;
   ld.ut = u + t * HW;
   ld.st = DELTA ? usub + t * HW : nullptr;
   ld.H = g.H;
   ld.W = g.W;
   ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColFwdStore<P::H> st{nullptr, rmap, g.H, g.W, xbi * XB, g.centered};
+  ColFwdStore<P::H, CEN> st{nullptr, rmap, g.H, g.W, xbi * XB, g.centered};
   for (int c = 0; c < g.C; ++c) {
     ld.bc = b1 + c * HW;
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
@@ -325,7 +329,7 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
 // cached dummy, so that the loads of a lane's outputs are not in divergent
 // branches: a load in a branch whose value is used after it gets its own
 // wait, which serialises the lane's loads.
-template <int HS>
+template <int HS, int CEN = -1>
 struct ColAdjLoad {
   const float2 *zh;  // zhat rows of (t, c); when the frame has none, any valid element
   const int *rmap;
@@ -336,18 +340,19 @@ struct ColAdjLoad {
   };
   __device__ Raw raw(int b, int i) const {
     const int Hh = HS ? HS : H;
+    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
     const int x = xb0 + b;
-    const int ky = centered ? wrap_half(i, Hh) : i;
+    const int ky = cen ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
     const bool use = x < W && k >= 0;
-    return Raw{gld(zh, use ? (unsigned)(k * W + x) : 0u), use};
+    return Raw{gld(zh, use ? __umul24((unsigned)k, (unsigned)W) + (unsigned)x : 0u), use};
   }
   __device__ float2 finish(const Raw &r, int) const { return r.use ? r.z : czero(); }
   __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i), 0); }
 };
 // The coil sum stays in registers, one accumulator per output of the lane.
 // PRE: the coil map values are loaded by prefetch() ahead of the last stage.
-template <int HS, bool PRE>
+template <int HS, bool PRE, int CEN = -1>
 struct ColAdjStore {
   float2 acc[kColSlots];
   float2 bpre[PRE ? kColSlots : 1];
@@ -358,10 +363,11 @@ struct ColAdjStore {
   float scale;
   __device__ unsigned index(int b, int i) const {
     const int Hh = HS ? HS : H;
+    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
     const int x = xb0 + b;
     const int xc = x < W ? x : W - 1;  // b1 load in bounds for every lane: no branch
-    const int y = centered ? wrap_half(i, Hh) : i;
-    return (unsigned)(y * W + xc);
+    const int y = cen ? wrap_half(i, Hh) : i;
+    return __umul24((unsigned)y, (unsigned)W) + (unsigned)xc;
   }
   __device__ void prefetch(int b, int i, int slot) {
     if (PRE) bpre[slot] = gld(bc, index(b, i));
@@ -376,7 +382,7 @@ struct ColAdjStore {
   }
 };
 
-template <class P>
+template <class P, int CEN = -1>
 __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, const float2 *zhat,
                                          const float2 *b1, float2 *out, void *tile,
                                          float2 *twl, int *rml) {
@@ -387,13 +393,13 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColAdjLoad<P::H> ld;
+  ColAdjLoad<P::H, CEN> ld;
   ld.rmap = rmap;
   ld.H = g.H;
   ld.W = g.W;
   ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColAdjStore<P::H, P::H != 0 && kColAdjPre> st;
+  ColAdjStore<P::H, P::H != 0 && kColAdjPre, CEN> st;
 #pragma unroll
   for (int k = 0; k < kColSlots; ++k) st.acc[k] = czero();
   st.out = out + t * HW;

@@ -106,7 +106,7 @@ struct avionic_engine {
     g.grid = TaskGrid{};
     for (Slot *s : g.slots) {
       Task t = s->task;
-      (t.cg.cplan != kColPlanRt ? g.grid.col_ct : g.grid.col_rt) = true;
+      g.grid.col_kinds |= 1u << col_kind(t.cg);
       (t.cg.rplan != kRowPlanRt ? g.grid.row_ct : g.grid.row_rt) = true;
       t.b_vox = bv;
       t.b_col = bc;

@@ -1,6 +1,8 @@
 // Task-list kernels of the PD iteration (see tasks.hpp).  Each block finds
 // its task, checks the task's device-side predicates (uniform per block, so a
 // whole block returns together) and runs the shared block / voxel bodies.
+#include <type_traits>
+
 #include "cart_fused.hpp"
 #include "reg_device.hpp"
 #include "tasks.hpp"
@@ -78,41 +80,49 @@ __global__ __launch_bounds__(RT) void kt_step_reg(const Task *__restrict__ tasks
   block_reduce2(ak, ax, T.part_reg + 2 * lb);
 }
 
+// Column passes, one kernel per kind of column plan (col_kind): kind 0 is the
+// runtime plan, kind 1 + 2 (plan - 1) + centred a compile-time plan with its
+// centring fixed.  Each kernel's registers are allocated for its own body
+// only (one kernel holding every plan's body needed 2-3 times the VGPRs of
+// any single one); a launch skips the blocks of other kinds, and a kind's
+// kernel is launched only when such a task is resident.
+template <int KIND>
+struct ColKind;
+template <>
+struct ColKind<0> {
+  using P = ColPlanRt;
+  static constexpr int CEN = -1;
+};
+template <int KIND>
+struct ColKind {
+  static constexpr int plan = 1 + (KIND - 1) / 2;
+  using P = std::conditional_t<
+      plan == kColPlan256, ColPlan256,
+      std::conditional_t<plan == kColPlan128, ColPlan128,
+                         std::conditional_t<plan == kColPlan160, ColPlan160,
+                                            std::conditional_t<plan == kColPlan192, ColPlan192,
+                                                               ColPlan224>>>>;
+  static constexpr int CEN = (KIND - 1) % 2;
+};
+
 // DELTA = false: K of the extragradient image (dual update);
 // DELTA = true : K of ext - x (step rule, adapting tasks only).
-// CT = true runs the tasks whose geometry has a compile-time column plan,
-// CT = false the others (each kernel's registers are allocated for its own
-// plans only; a launch skips the blocks of the other kind).
-template <bool DELTA, bool CT>
+template <bool DELTA, int KIND>
 __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n) {
+  constexpr bool CT = KIND != 0;
   // compile-time plans: transforms of at most 256, optionally a float tile
   __shared__ float2 tile[CT && kColSplit ? kColTileElems / 2 : kColTileElems];
   __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
-  if ((T.cg.cplan != kColPlanRt) != CT) return;
+  if (col_kind(T.cg) != KIND) return;
   if (!task_active(T.sc)) return;
   if (DELTA && !task_adapt(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
   const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
   const float2 *usub = DELTA ? T.x : nullptr;
-  if constexpr (!CT) {
-    col_forward_fused<DELTA, ColPlanRt>(T.cg, xbi, t, T.ext, usub, T.b1, T.kcol, tile, twl, rml);
-  } else {
-    switch (T.cg.cplan) {
-#define AV_COLF(ID, P)                                                                         \
-  case ID:                                                                                     \
-    col_forward_fused<DELTA, P>(T.cg, xbi, t, T.ext, usub, T.b1, T.kcol, tile, twl, rml);   \
-    break;
-      AV_COLF(kColPlan256, ColPlan256)
-      AV_COLF(kColPlan128, ColPlan128)
-      AV_COLF(kColPlan160, ColPlan160)
-      AV_COLF(kColPlan192, ColPlan192)
-      AV_COLF(kColPlan224, ColPlan224)
-#undef AV_COLF
-      default: break;
-    }
-  }
+  col_forward_fused<DELTA, typename ColKind<KIND>::P, ColKind<KIND>::CEN>(
+      T.cg, xbi, t, T.ext, usub, T.b1, T.kcol, tile, twl, rml);
 }
 
 // Row pass.  CT = true: tasks whose row length has a compile-time plan
@@ -168,41 +178,28 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
 #define AV_COL_ATTR_CT
 #endif
 
-template <bool CT>
+template <int KIND>
 __device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks, int n) {
+  constexpr bool CT = KIND != 0;
   __shared__ float2 tile[CT && kColSplit ? kColTileElems / 2 : kColTileElems];
   __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
-  if ((T.cg.cplan != kColPlanRt) != CT) return;
+  if (col_kind(T.cg) != KIND) return;
   if (!task_active(T.sc)) return;
   const int lb = blockIdx.x - T.b_col;
   const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
-  if constexpr (!CT) {
-    col_adjoint_fused<ColPlanRt>(T.cg, xbi, t, T.zhat, T.b1, T.khz, tile, twl, rml);
-  } else {
-    switch (T.cg.cplan) {
-#define AV_COLA(ID, P)                                                            \
-  case ID:                                                                        \
-    col_adjoint_fused<P>(T.cg, xbi, t, T.zhat, T.b1, T.khz, tile, twl, rml);   \
-    break;
-      AV_COLA(kColPlan256, ColPlan256)
-      AV_COLA(kColPlan128, ColPlan128)
-      AV_COLA(kColPlan160, ColPlan160)
-      AV_COLA(kColPlan192, ColPlan192)
-      AV_COLA(kColPlan224, ColPlan224)
-#undef AV_COLA
-      default: break;
-    }
-  }
+  col_adjoint_fused<typename ColKind<KIND>::P, ColKind<KIND>::CEN>(T.cg, xbi, t, T.zhat, T.b1,
+                                                                   T.khz, tile, twl, rml);
 }
 __global__ __launch_bounds__(NTC) __attribute__((amdgpu_waves_per_eu(4)))
 void kt_col_adjoint_rt(const Task *__restrict__ tasks, int n) {
-  col_adjoint_block<false>(tasks, n);
+  col_adjoint_block<0>(tasks, n);
 }
+template <int KIND>
 __global__ __launch_bounds__(NTC) AV_COL_ATTR_CT
 void kt_col_adjoint_ct(const Task *__restrict__ tasks, int n) {
-  col_adjoint_block<true>(tasks, n);
+  col_adjoint_block<KIND>(tasks, n);
 }
 
 // One block per task: step rule for adapting tasks, then the iteration
@@ -248,16 +245,25 @@ const char *kernel_name(int id) {
 }
 
 // Column passes: one launch per kind of column plan present among the tasks.
-template <bool DELTA>
+template <bool DELTA, int K = 0>
 static void launch_col_fwd(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
-  if (gr.col <= 0) return;
-  if (gr.col_ct) hipLaunchKernelGGL((kt_col_forward<DELTA, true>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
-  if (gr.col_rt) hipLaunchKernelGGL((kt_col_forward<DELTA, false>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+  if constexpr (K < kColKinds) {
+    if (gr.col > 0 && (gr.col_kinds >> K & 1u))
+      hipLaunchKernelGGL((kt_col_forward<DELTA, K>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+    launch_col_fwd<DELTA, K + 1>(gr, d_tasks, n, s);
+  }
 }
+template <int K = 0>
 static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
-  if (gr.col <= 0) return;
-  if (gr.col_ct) hipLaunchKernelGGL(kt_col_adjoint_ct, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
-  if (gr.col_rt) hipLaunchKernelGGL(kt_col_adjoint_rt, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+  if constexpr (K < kColKinds) {
+    if (gr.col > 0 && (gr.col_kinds >> K & 1u)) {
+      if constexpr (K == 0)
+        hipLaunchKernelGGL(kt_col_adjoint_rt, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+      else
+        hipLaunchKernelGGL((kt_col_adjoint_ct<K>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+    }
+    launch_col_adj<K + 1>(gr, d_tasks, n, s);
+  }
 }
 
 template <int MODE>

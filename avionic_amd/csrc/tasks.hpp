@@ -44,7 +44,7 @@ struct Task {
 // tasks: device array; the host mirror gives the grid sizes.
 struct TaskGrid {
   int vox = 0, col = 0, row = 0;
-  bool col_ct = false, col_rt = false;  // tasks with a compile-time / runtime column plan
+  unsigned col_kinds = 0;               // bit col_kind(cg) of every task
   bool row_ct = false, row_rt = false;  // ... row plan
 };
 
