@@ -87,20 +87,6 @@ __device__ __forceinline__ void col_fft(float2 *tile, const CartGeom &g, float s
 // stage's arithmetic): without the scheduling barrier the compiler places
 // each complex product right after its two loads, and the wait before it
 // leaves one pair of loads in flight per lane.
-#ifndef AVIONIC_COL_SPLIT
-#define AVIONIC_COL_SPLIT 0
-#endif
-// SPLIT: exchange between radix stages through a tile of floats, the real
-// parts and then the imaginary parts (half the LDS of a complex tile, two
-// more barriers per transform).
-constexpr bool kColSplit = AVIONIC_COL_SPLIT != 0;
-#ifndef AVIONIC_COL_ADJ_PRE
-#define AVIONIC_COL_ADJ_PRE 1
-#endif
-// adjoint: load the coil map values of the lane's outputs before the last
-// stage (32 more VGPRs) instead of next to each product
-constexpr bool kColAdjPre = AVIONIC_COL_ADJ_PRE != 0;
-
 template <int R_, int L, int B, int Ns>
 struct CtStage {
   static constexpr int R = R_, NBF = L / R_, TOTAL = NBF * B, MAXB = (TOTAL + NTC - 1) / NTC;
@@ -130,48 +116,20 @@ struct CtStage {
 // Outputs v of stage S0 become inputs w of stage S1 through the LDS tile
 // (element (b, i) at b + i B).  Callers sync before writing the tile again.
 template <int B, class S0, class S1>
-__device__ __forceinline__ void ct_exchange(void *tile, float2 (&v)[S0::MAXB][S0::R],
+__device__ __forceinline__ void ct_exchange(float2 *t, float2 (&v)[S0::MAXB][S0::R],
                                             float2 (&w)[S1::MAXB][S1::R], int tid) {
   int b, j;
-  if constexpr (!kColSplit) {
-    float2 *t = (float2 *)tile;
 #pragma unroll
-    for (int c = 0; c < S0::MAXB; ++c)
-      if (S0::lane(tid, c, b, j))
+  for (int c = 0; c < S0::MAXB; ++c)
+    if (S0::lane(tid, c, b, j))
 #pragma unroll
-        for (int r = 0; r < S0::R; ++r) t[b + S0::out_index(j, r) * B] = v[c][r];
-    __syncthreads();
+      for (int r = 0; r < S0::R; ++r) t[b + S0::out_index(j, r) * B] = v[c][r];
+  __syncthreads();
 #pragma unroll
-    for (int c = 0; c < S1::MAXB; ++c)
-      if (S1::lane(tid, c, b, j))
+  for (int c = 0; c < S1::MAXB; ++c)
+    if (S1::lane(tid, c, b, j))
 #pragma unroll
-        for (int r = 0; r < S1::R; ++r) w[c][r] = t[b + S1::in_index(j, r) * B];
-  } else {
-    float *t = (float *)tile;
-#pragma unroll
-    for (int c = 0; c < S0::MAXB; ++c)
-      if (S0::lane(tid, c, b, j))
-#pragma unroll
-        for (int r = 0; r < S0::R; ++r) t[b + S0::out_index(j, r) * B] = v[c][r].x;
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < S1::MAXB; ++c)
-      if (S1::lane(tid, c, b, j))
-#pragma unroll
-        for (int r = 0; r < S1::R; ++r) w[c][r].x = t[b + S1::in_index(j, r) * B];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < S0::MAXB; ++c)
-      if (S0::lane(tid, c, b, j))
-#pragma unroll
-        for (int r = 0; r < S0::R; ++r) t[b + S0::out_index(j, r) * B] = v[c][r].y;
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < S1::MAXB; ++c)
-      if (S1::lane(tid, c, b, j))
-#pragma unroll
-        for (int r = 0; r < S1::R; ++r) w[c][r].y = t[b + S1::in_index(j, r) * B];
-  }
+      for (int r = 0; r < S1::R; ++r) w[c][r] = t[b + S1::in_index(j, r) * B];
 }
 
 // Last stage S: prefetch what the store functor reads, transform, store.
@@ -197,7 +155,7 @@ __device__ __forceinline__ void ct_last(float2 (&w)[S::MAXB][S::R], int tid, con
 // Column transform of one tile on a compile-time plan, first stage from
 // `ld` (raw loads, then finish), last stage to `st`.
 template <class P, class LD, class ST>
-__device__ __forceinline__ void col_fft_ct(void *tile, float sign, int tid, const float2 *tw,
+__device__ __forceinline__ void col_fft_ct(float2 *tile, float sign, int tid, const float2 *tw,
                                            LD &ld, ST &st) {
   asm volatile("" : "+v"(tid));  // per-lane index math stays inside the coil loop
   constexpr int L = P::H, B = P::XB;
@@ -297,7 +255,7 @@ __device__ inline const int *stage_rowmap(const CartGeom &g, int t, int *lds, in
 template <bool DELTA, class P, int CEN = -1>
 __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, const float2 *u,
                                          const float2 *usub, const float2 *b1, float2 *kcol,
-                                         void *tile, float2 *twl, int *rml) {
+                                         float2 *tile, float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
   const int XB = P::XB ? P::XB : g.XBF;
@@ -317,7 +275,7 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
     ld.bc = b1 + c * HW;
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
     if constexpr (P::H == 0)
-      col_fft<P>((float2 *)tile, g, +1.f, tid, tw, ld, st);
+      col_fft<P>(tile, g, +1.f, tid, tw, ld, st);
     else
       col_fft_ct<P>(tile, +1.f, tid, tw, ld, st);
     __syncthreads();
@@ -384,7 +342,7 @@ struct ColAdjStore {
 
 template <class P, int CEN = -1>
 __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, const float2 *zhat,
-                                         const float2 *b1, float2 *out, void *tile,
+                                         const float2 *b1, float2 *out, float2 *tile,
                                          float2 *twl, int *rml) {
   const int tid = threadIdx.x;
   const long HW = (long)g.H * g.W;
@@ -399,7 +357,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   ld.W = g.W;
   ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColAdjStore<P::H, P::H != 0 && kColAdjPre, CEN> st;
+  ColAdjStore<P::H, P::H != 0, CEN> st;
 #pragma unroll
   for (int k = 0; k < kColSlots; ++k) st.acc[k] = czero();
   st.out = out + t * HW;
@@ -413,7 +371,7 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
     st.bc = b1 + c * HW;
     st.last = (c == g.C - 1);
     if constexpr (P::H == 0)
-      col_fft<P>((float2 *)tile, g, -1.f, tid, tw, ld, st);
+      col_fft<P>(tile, g, -1.f, tid, tw, ld, st);
     else
       col_fft_ct<P>(tile, -1.f, tid, tw, ld, st);
     __syncthreads();

@@ -14,14 +14,10 @@ namespace {
 #ifndef AVIONIC_NC_COLA_THREADS
 #define AVIONIC_NC_COLA_THREADS 512
 #endif
-#ifndef AVIONIC_NC_UCACHE
-#define AVIONIC_NC_UCACHE 0
-#endif
 constexpr int NCF = AVIONIC_NC_COLF_THREADS;  // threads of a forward column-pass block
 constexpr int NCC = AVIONIC_NC_COLA_THREADS;  // threads of an adjoint column-pass block
 constexpr int NCR = 256;   // threads of a row-pass / sample-pass block
 constexpr int NCE = 4096;  // complex elements of an LDS transform tile
-constexpr bool kNcUcache = AVIONIC_NC_UCACHE != 0;  // image tile kept in registers across coils
 constexpr int kColSlotsNc = max_slots(NCE, NCF);
 constexpr int kRowSlotsNc = max_slots(NCE, NCR);
 
@@ -29,26 +25,18 @@ __device__ inline int wrap_up(int v, int G) { return v >= G ? v - G : v; }
 
 // ------------------------------------------------------------ K, column pass
 struct NcColFwdLoad {
-  float2 ucache[kNcUcache ? kColSlotsNc : 1];
   const float2 *ut, *us, *bc;
   const float *dxi, *dyi;
   int W, H, Gy, xb0;
-  bool first, delta;
-  __device__ float2 operator()(int b, int i, int slot) {
+  bool delta;
+  __device__ float2 operator()(int b, int i, int) const {
     const int x = xb0 + b;
     if (x >= W) return czero();
     const int y = wrap_up(i + H / 2, Gy);
     if (y >= H) return czero();
     const unsigned si = (unsigned)(y * W + x);
-    if (!kNcUcache) {
-      const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
-      return cmul(bc[si], cscale(v, dxi[x] * dyi[y]));
-    }
-    if (first) {
-      const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
-      ucache[slot] = cscale(v, dxi[x] * dyi[y]);
-    }
-    return cmul(bc[si], ucache[slot]);
+    const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
+    return cmul(bc[si], cscale(v, dxi[x] * dyi[y]));
   }
 };
 struct NcColFwdStore {
@@ -87,7 +75,6 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
   const int c0 = blockIdx.z * cpg, c1 = min(g.C, c0 + cpg);
   for (int c = c0; c < c1; ++c) {
     ld.bc = b1 + c * HW;
-    ld.first = (c == c0);
     st.tc = tmp + ((long)t * g.C + c) * g.Gy * g.W;
     fft_io<NCF, NCE, true, true, true>(tile, g.py, g.XBf, 1, g.XBf, -1.f, tid, tw, ld, st);
     __syncthreads();

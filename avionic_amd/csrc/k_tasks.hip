@@ -110,8 +110,8 @@ struct ColKind {
 template <bool DELTA, int KIND>
 __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n) {
   constexpr bool CT = KIND != 0;
-  // compile-time plans: transforms of at most 256, optionally a float tile
-  __shared__ float2 tile[CT && kColSplit ? kColTileElems / 2 : kColTileElems];
+  // compile-time plans: transforms of at most 256
+  __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
@@ -167,21 +167,12 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
 // the coil loop.  Runtime plans: asking for 4 waves per SIMD (the limit the
 // LDS tile sets anyway) trims it from 130 to 128 VGPRs (col_adjoint 1.17 ->
 // 0.98 ms, ICTGV2 256x256x30x16, 8 reconstructions).  Compile-time plans
-// also hold the prefetched coil map values; AVIONIC_COL_ADJ_WAVES_CT > 0
-// asks for that many waves per SIMD.
-#ifndef AVIONIC_COL_ADJ_WAVES_CT
-#define AVIONIC_COL_ADJ_WAVES_CT 0
-#endif
-#if AVIONIC_COL_ADJ_WAVES_CT > 0
-#define AV_COL_ATTR_CT __attribute__((amdgpu_waves_per_eu(AVIONIC_COL_ADJ_WAVES_CT)))
-#else
-#define AV_COL_ATTR_CT
-#endif
-
+// also hold the prefetched coil map values (3 waves per SIMD at 256; asking
+// for 4 spills).
 template <int KIND>
 __device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks, int n) {
   constexpr bool CT = KIND != 0;
-  __shared__ float2 tile[CT && kColSplit ? kColTileElems / 2 : kColTileElems];
+  __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
   const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
@@ -197,8 +188,7 @@ void kt_col_adjoint_rt(const Task *__restrict__ tasks, int n) {
   col_adjoint_block<0>(tasks, n);
 }
 template <int KIND>
-__global__ __launch_bounds__(NTC) AV_COL_ATTR_CT
-void kt_col_adjoint_ct(const Task *__restrict__ tasks, int n) {
+__global__ __launch_bounds__(NTC) void kt_col_adjoint_ct(const Task *__restrict__ tasks, int n) {
   col_adjoint_block<KIND>(tasks, n);
 }
 
@@ -364,7 +354,7 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   // writes kcol / z / zhat / khz); the primal update needs both
   if (two && gr.col < col_resident_blocks()) {
     // A column pass that cannot fill the GPU alone (one reconstruction of
-    // cfg2 size: 480 workgroups) is submitted first, so that all of its
+    // 256x256x30x16: 480 workgroups) is submitted first, so that all of its
     // workgroups are resident and the dual fills the rest of the CUs; the
     // dual still depends only on what precedes the column pass.  With more
     // column workgroups than fit at once the dual goes first (measured on

@@ -218,7 +218,7 @@ def serving_pool(seed=4, frames=30, coils=20):
     return pool
 
 
-# ------------------------------------------------------------ radial (cfg3)
+# ------------------------------------------------------------ radial TGV2
 @dataclasses.dataclass
 class RadialWorkload:
     name: str = "tgv2_radial_256x30x12"
