@@ -101,19 +101,31 @@ struct avionic_engine {
 
   void rebuild(Group &g) {
     g.drop_graphs();
+    // tasks ordered by column kind, then row kind: each kind's kernel is
+    // launched over its own span of blocks only
+    auto row_kind = [](const Slot *s) { return s->task.cg.rplan != kRowPlanRt ? 0 : 1; };
+    std::stable_sort(g.slots.begin(), g.slots.end(), [&](const Slot *a, const Slot *b) {
+      const int ka = col_kind(a->task.cg), kb = col_kind(b->task.cg);
+      return ka != kb ? ka < kb : row_kind(a) < row_kind(b);
+    });
     std::vector<Task> tasks;
     int bv = 0, bc = 0, br = 0;
     g.grid = TaskGrid{};
+    bool col_seen[kColKinds] = {}, row_seen[2] = {};
     for (Slot *s : g.slots) {
       Task t = s->task;
-      g.grid.col_kinds |= 1u << col_kind(t.cg);
-      (t.cg.rplan != kRowPlanRt ? g.grid.row_ct : g.grid.row_rt) = true;
       t.b_vox = bv;
       t.b_col = bc;
       t.b_row = br;
+      const int ck = col_kind(t.cg), rk = row_kind(s);
+      if (!col_seen[ck]) g.grid.col_lo[ck] = bc;
+      if (!row_seen[rk]) g.grid.row_lo[rk] = br;
+      col_seen[ck] = row_seen[rk] = true;
       bv += t.nvox_blocks;
       bc += t.ncol_blocks;
       br += t.nrow_blocks;
+      g.grid.col_hi[ck] = bc;
+      g.grid.row_hi[rk] = br;
       tasks.push_back(t);
     }
     g.grid.vox = bv;

@@ -108,17 +108,19 @@ struct ColKind {
 // DELTA = false: K of the extragradient image (dual update);
 // DELTA = true : K of ext - x (step rule, adapting tasks only).
 template <bool DELTA, int KIND>
-__global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n) {
+__global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n,
+                                                      int blk0) {
   constexpr bool CT = KIND != 0;
   // compile-time plans: transforms of at most 256
   __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
-  const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
+  const int blk = blk0 + (int)blockIdx.x;
+  const Task &T = tasks[find_task<kCol>(tasks, n, blk)];
   if (col_kind(T.cg) != KIND) return;
   if (!task_active(T.sc)) return;
   if (DELTA && !task_adapt(T.sc)) return;
-  const int lb = blockIdx.x - T.b_col;
+  const int lb = blk - T.b_col;
   const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
   const float2 *usub = DELTA ? T.x : nullptr;
   col_forward_fused<DELTA, typename ColKind<KIND>::P, ColKind<KIND>::CEN>(
@@ -128,17 +130,18 @@ __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ t
 // Row pass.  CT = true: tasks whose row length has a compile-time plan
 // (16 or 32 rows per block), CT = false: the others (RB rows per block).
 template <int MODE, bool CT>
-__global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int n) {
+__global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int n, int blk0) {
   __shared__ float2 tile[CT ? kFftTileElems : RTILE];
   __shared__ float2 tile2[(MODE == kRowDualUpdate && !CT) ? RTILE : 1];
   __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ double red[NT];
   __shared__ int mrl[CT ? kRowPlanMaxRows : kRowMaxRows];
-  const Task &T = tasks[find_task<kRow>(tasks, n, blockIdx.x)];
+  const int blk = blk0 + (int)blockIdx.x;
+  const Task &T = tasks[find_task<kRow>(tasks, n, blk)];
   if ((T.cg.rplan != kRowPlanRt) != CT) return;
   if (!task_active(T.sc)) return;
   if (MODE == kRowNorm2 && !task_adapt(T.sc)) return;
-  const int lb = blockIdx.x - T.b_row;
+  const int lb = blk - T.b_row;
   double a = 0.0;
   if constexpr (CT) {
     if (T.cg.rplan == kRowPlan256)
@@ -170,26 +173,28 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
 // also hold the prefetched coil map values (3 waves per SIMD at 256; asking
 // for 4 spills).
 template <int KIND>
-__device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks, int n) {
+__device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks, int n, int blk0) {
   constexpr bool CT = KIND != 0;
   __shared__ float2 tile[kColTileElems];
   __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
   __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
-  const Task &T = tasks[find_task<kCol>(tasks, n, blockIdx.x)];
+  const int blk = blk0 + (int)blockIdx.x;
+  const Task &T = tasks[find_task<kCol>(tasks, n, blk)];
   if (col_kind(T.cg) != KIND) return;
   if (!task_active(T.sc)) return;
-  const int lb = blockIdx.x - T.b_col;
+  const int lb = blk - T.b_col;
   const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
   col_adjoint_fused<typename ColKind<KIND>::P, ColKind<KIND>::CEN>(T.cg, xbi, t, T.zhat, T.b1,
                                                                    T.khz, tile, twl, rml);
 }
 __global__ __launch_bounds__(NTC) __attribute__((amdgpu_waves_per_eu(4)))
-void kt_col_adjoint_rt(const Task *__restrict__ tasks, int n) {
-  col_adjoint_block<0>(tasks, n);
+void kt_col_adjoint_rt(const Task *__restrict__ tasks, int n, int blk0) {
+  col_adjoint_block<0>(tasks, n, blk0);
 }
 template <int KIND>
-__global__ __launch_bounds__(NTC) void kt_col_adjoint_ct(const Task *__restrict__ tasks, int n) {
-  col_adjoint_block<KIND>(tasks, n);
+__global__ __launch_bounds__(NTC) void kt_col_adjoint_ct(const Task *__restrict__ tasks, int n,
+                                                         int blk0) {
+  col_adjoint_block<KIND>(tasks, n, blk0);
 }
 
 // One block per task: step rule for adapting tasks, then the iteration
@@ -250,19 +255,23 @@ static int col_resident_blocks() {
 template <bool DELTA, int K = 0>
 static void launch_col_fwd(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
   if constexpr (K < kColKinds) {
-    if (gr.col > 0 && (gr.col_kinds >> K & 1u))
-      hipLaunchKernelGGL((kt_col_forward<DELTA, K>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+    const int nb = gr.col_hi[K] - gr.col_lo[K];
+    if (nb > 0)
+      hipLaunchKernelGGL((kt_col_forward<DELTA, K>), dim3(nb), dim3(NTC), 0, s, d_tasks, n,
+                         gr.col_lo[K]);
     launch_col_fwd<DELTA, K + 1>(gr, d_tasks, n, s);
   }
 }
 template <int K = 0>
 static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
   if constexpr (K < kColKinds) {
-    if (gr.col > 0 && (gr.col_kinds >> K & 1u)) {
+    const int nb = gr.col_hi[K] - gr.col_lo[K];
+    if (nb > 0) {
       if constexpr (K == 0)
-        hipLaunchKernelGGL(kt_col_adjoint_rt, dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+        hipLaunchKernelGGL(kt_col_adjoint_rt, dim3(nb), dim3(NTC), 0, s, d_tasks, n, gr.col_lo[K]);
       else
-        hipLaunchKernelGGL((kt_col_adjoint_ct<K>), dim3(gr.col), dim3(NTC), 0, s, d_tasks, n);
+        hipLaunchKernelGGL((kt_col_adjoint_ct<K>), dim3(nb), dim3(NTC), 0, s, d_tasks, n,
+                           gr.col_lo[K]);
     }
     launch_col_adj<K + 1>(gr, d_tasks, n, s);
   }
@@ -270,9 +279,11 @@ static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipSt
 
 template <int MODE>
 static void launch_row(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
-  if (gr.row <= 0) return;
-  if (gr.row_ct) hipLaunchKernelGGL((kt_row<MODE, true>), dim3(gr.row), dim3(NT), 0, s, d_tasks, n);
-  if (gr.row_rt) hipLaunchKernelGGL((kt_row<MODE, false>), dim3(gr.row), dim3(NT), 0, s, d_tasks, n);
+  const int nct = gr.row_hi[0] - gr.row_lo[0], nrt = gr.row_hi[1] - gr.row_lo[1];
+  if (nct > 0)
+    hipLaunchKernelGGL((kt_row<MODE, true>), dim3(nct), dim3(NT), 0, s, d_tasks, n, gr.row_lo[0]);
+  if (nrt > 0)
+    hipLaunchKernelGGL((kt_row<MODE, false>), dim3(nrt), dim3(NT), 0, s, d_tasks, n, gr.row_lo[1]);
 }
 
 void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,

@@ -44,8 +44,11 @@ struct Task {
 // tasks: device array; the host mirror gives the grid sizes.
 struct TaskGrid {
   int vox = 0, col = 0, row = 0;
-  unsigned col_kinds = 0;               // bit col_kind(cg) of every task
-  bool row_ct = false, row_rt = false;  // ... row plan
+  // Blocks [lo, hi) spanning the tasks of each column kind (col_kind) and of
+  // each row kind (0: compile-time row plan, 1: runtime); the engine orders
+  // its tasks by kind, so each span holds only that kind's blocks.
+  int col_lo[kColKinds] = {}, col_hi[kColKinds] = {};
+  int row_lo[2] = {}, row_hi[2] = {};
 };
 
 // Streams of one iteration: the regulariser kernels that do not depend on
