@@ -203,9 +203,8 @@ __device__ __forceinline__ void col_fft_ct(float2 *tile, float sign, int tid, co
 // columns past W load a valid element and drop it: a load in a divergent
 // branch whose value is used after it gets its own wait, which serialises the
 // lane's loads.
-// CEN: the centring flag when compile-time (0 / 1; -1: runtime, centered).
 // Index products use 24-bit multiplies (rows and columns are below 2^12).
-template <bool DELTA, int HS, int CEN = -1>
+template <bool DELTA, int HS>
 struct ColFwdLoad {
   const float2 *ut, *st, *bc;
   int H, W, xb0, centered;
@@ -214,7 +213,7 @@ struct ColFwdLoad {
   };
   __device__ Raw raw(int b, int i) const {
     const int Hh = HS ? HS : H;
-    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
+    const bool cen = centered != 0;
     const int x = xb0 + b;
     const int xc = x < W ? x : W - 1;  // every lane loads in bounds: no branch
     const int sy = cen ? wrap_half(i, Hh) : i;
@@ -227,14 +226,14 @@ struct ColFwdLoad {
   }
   __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i), b); }
 };
-template <int HS, int CEN = -1>
+template <int HS>
 struct ColFwdStore {
   float2 *kc;       // kcol + (C r0 + c S) W : rows of (t, c)
   const int *rmap;  // LDS: compact row of each k-space row of frame t
   int H, W, xb0, centered;
   __device__ void operator()(int b, int i, float2 v, int) {
     const int Hh = HS ? HS : H;
-    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
+    const bool cen = centered != 0;
     const int x = xb0 + b;
     const int ky = cen ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
@@ -252,7 +251,7 @@ __device__ inline const int *stage_rowmap(const CartGeom &g, int t, int *lds, in
   return lds;
 }
 
-template <bool DELTA, class P, int CEN = -1>
+template <bool DELTA, class P>
 __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, const float2 *u,
                                          const float2 *usub, const float2 *b1, float2 *kcol,
                                          float2 *tile, float2 *twl, int *rml) {
@@ -263,14 +262,14 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
   const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColFwdLoad<DELTA, P::H, CEN> ld;
+  ColFwdLoad<DELTA, P::H> ld;
   ld.ut = u + t * HW;
   ld.st = DELTA ? usub + t * HW : nullptr;
   ld.H = g.H;
   ld.W = g.W;
   ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColFwdStore<P::H, CEN> st{nullptr, rmap, g.H, g.W, xbi * XB, g.centered};
+  ColFwdStore<P::H> st{nullptr, rmap, g.H, g.W, xbi * XB, g.centered};
   for (int c = 0; c < g.C; ++c) {
     ld.bc = b1 + c * HW;
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
@@ -287,7 +286,7 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
 // cached dummy, so that the loads of a lane's outputs are not in divergent
 // branches: a load in a branch whose value is used after it gets its own
 // wait, which serialises the lane's loads.
-template <int HS, int CEN = -1>
+template <int HS>
 struct ColAdjLoad {
   const float2 *zh;  // zhat rows of (t, c); when the frame has none, any valid element
   const int *rmap;
@@ -298,7 +297,7 @@ struct ColAdjLoad {
   };
   __device__ Raw raw(int b, int i) const {
     const int Hh = HS ? HS : H;
-    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
+    const bool cen = centered != 0;
     const int x = xb0 + b;
     const int ky = cen ? wrap_half(i, Hh) : i;
     const int k = rmap[ky];
@@ -310,7 +309,7 @@ struct ColAdjLoad {
 };
 // The coil sum stays in registers, one accumulator per output of the lane.
 // PRE: the coil map values are loaded by prefetch() ahead of the last stage.
-template <int HS, bool PRE, int CEN = -1>
+template <int HS, bool PRE>
 struct ColAdjStore {
   float2 acc[kColSlots];
   float2 bpre[PRE ? kColSlots : 1];
@@ -321,7 +320,7 @@ struct ColAdjStore {
   float scale;
   __device__ unsigned index(int b, int i) const {
     const int Hh = HS ? HS : H;
-    const bool cen = CEN < 0 ? centered != 0 : CEN != 0;
+    const bool cen = centered != 0;
     const int x = xb0 + b;
     const int xc = x < W ? x : W - 1;  // b1 load in bounds for every lane: no branch
     const int y = cen ? wrap_half(i, Hh) : i;
@@ -340,7 +339,7 @@ struct ColAdjStore {
   }
 };
 
-template <class P, int CEN = -1>
+template <class P>
 __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, const float2 *zhat,
                                          const float2 *b1, float2 *out, float2 *tile,
                                          float2 *twl, int *rml) {
@@ -351,13 +350,13 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColAdjLoad<P::H, CEN> ld;
+  ColAdjLoad<P::H> ld;
   ld.rmap = rmap;
   ld.H = g.H;
   ld.W = g.W;
   ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColAdjStore<P::H, P::H != 0, CEN> st;
+  ColAdjStore<P::H, P::H != 0> st;
 #pragma unroll
   for (int k = 0; k < kColSlots; ++k) st.acc[k] = czero();
   st.out = out + t * HW;

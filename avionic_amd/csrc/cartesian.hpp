@@ -43,12 +43,10 @@ struct CartGeom {
 // Compile-time column plans (transform length H, XBF columns per tile); any
 // other geometry runs the runtime plan.
 enum ColPlanId { kColPlanRt = 0, kColPlan256, kColPlan128, kColPlan160, kColPlan192, kColPlan224 };
-// Kind of a geometry's fused column passes: 0 runtime plan, else
-// 1 + 2 (plan - 1) + centred (one kernel per kind).
-constexpr int kColKinds = 1 + 2 * (kColPlan224 - kColPlan256 + 1);
-__host__ __device__ inline int col_kind(const CartGeom &g) {
-  return g.cplan == kColPlanRt ? 0 : 1 + 2 * (g.cplan - 1) + (g.centered ? 1 : 0);
-}
+// Kind of a geometry's fused column passes: 0 runtime plan, 1 compile-time
+// plan (one kernel per kind).
+constexpr int kColKinds = 2;
+__host__ __device__ inline int col_kind(const CartGeom &g) { return g.cplan == kColPlanRt ? 0 : 1; }
 // Compile-time row plans (row length W); rows per block 16 and 32.
 enum RowPlanId { kRowPlanRt = 0, kRowPlan256, kRowPlan128 };
 

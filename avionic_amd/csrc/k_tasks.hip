@@ -81,29 +81,24 @@ __global__ __launch_bounds__(RT) void kt_step_reg(const Task *__restrict__ tasks
 }
 
 // Column passes, one kernel per kind of column plan (col_kind): kind 0 is the
-// runtime plan, kind 1 + 2 (plan - 1) + centred a compile-time plan with its
-// centring fixed.  Each kernel's registers are allocated for its own body
-// only (one kernel holding every plan's body needed 2-3 times the VGPRs of
-// any single one); a launch skips the blocks of other kinds, and a kind's
-// kernel is launched only when such a task is resident.
-template <int KIND>
-struct ColKind;
-template <>
-struct ColKind<0> {
-  using P = ColPlanRt;
-  static constexpr int CEN = -1;
-};
-template <int KIND>
-struct ColKind {
-  static constexpr int plan = 1 + (KIND - 1) / 2;
-  using P = std::conditional_t<
-      plan == kColPlan256, ColPlan256,
-      std::conditional_t<plan == kColPlan128, ColPlan128,
-                         std::conditional_t<plan == kColPlan160, ColPlan160,
-                                            std::conditional_t<plan == kColPlan192, ColPlan192,
-                                                               ColPlan224>>>>;
-  static constexpr int CEN = (KIND - 1) % 2;
-};
+// runtime plan, kind 1 the compile-time plans (a switch over the plan; the
+// centring flag stays a runtime value: one body per plan and centring needed
+// 2-3 times the VGPRs, and one kernel per plan served mixed sizes 6% slower
+// than one kernel for all of them).  A launch covers its kind's span of
+// blocks and is made only when such a task is resident.
+template <bool DELTA, class P>
+__device__ __forceinline__ void col_forward_task(const Task &T, int lb, float2 *tile, float2 *twl,
+                                                 int *rml) {
+  const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
+  col_forward_fused<DELTA, P>(T.cg, xbi, t, T.ext, DELTA ? T.x : nullptr, T.b1, T.kcol, tile, twl,
+                              rml);
+}
+template <class P>
+__device__ __forceinline__ void col_adjoint_task(const Task &T, int lb, float2 *tile, float2 *twl,
+                                                 int *rml) {
+  const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
+  col_adjoint_fused<P>(T.cg, xbi, t, T.zhat, T.b1, T.khz, tile, twl, rml);
+}
 
 // DELTA = false: K of the extragradient image (dual update);
 // DELTA = true : K of ext - x (step rule, adapting tasks only).
@@ -121,10 +116,18 @@ __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ t
   if (!task_active(T.sc)) return;
   if (DELTA && !task_adapt(T.sc)) return;
   const int lb = blk - T.b_col;
-  const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
-  const float2 *usub = DELTA ? T.x : nullptr;
-  col_forward_fused<DELTA, typename ColKind<KIND>::P, ColKind<KIND>::CEN>(
-      T.cg, xbi, t, T.ext, usub, T.b1, T.kcol, tile, twl, rml);
+  if constexpr (!CT) {
+    col_forward_task<DELTA, ColPlanRt>(T, lb, tile, twl, rml);
+  } else {
+    switch (T.cg.cplan) {
+      case kColPlan256: col_forward_task<DELTA, ColPlan256>(T, lb, tile, twl, rml); break;
+      case kColPlan128: col_forward_task<DELTA, ColPlan128>(T, lb, tile, twl, rml); break;
+      case kColPlan160: col_forward_task<DELTA, ColPlan160>(T, lb, tile, twl, rml); break;
+      case kColPlan192: col_forward_task<DELTA, ColPlan192>(T, lb, tile, twl, rml); break;
+      case kColPlan224: col_forward_task<DELTA, ColPlan224>(T, lb, tile, twl, rml); break;
+      default: break;
+    }
+  }
 }
 
 // Row pass.  CT = true: tasks whose row length has a compile-time plan
@@ -183,9 +186,18 @@ __device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks
   if (col_kind(T.cg) != KIND) return;
   if (!task_active(T.sc)) return;
   const int lb = blk - T.b_col;
-  const int xbi = lb % T.cg.nxbf, t = lb / T.cg.nxbf;
-  col_adjoint_fused<typename ColKind<KIND>::P, ColKind<KIND>::CEN>(T.cg, xbi, t, T.zhat, T.b1,
-                                                                   T.khz, tile, twl, rml);
+  if constexpr (!CT) {
+    col_adjoint_task<ColPlanRt>(T, lb, tile, twl, rml);
+  } else {
+    switch (T.cg.cplan) {
+      case kColPlan256: col_adjoint_task<ColPlan256>(T, lb, tile, twl, rml); break;
+      case kColPlan128: col_adjoint_task<ColPlan128>(T, lb, tile, twl, rml); break;
+      case kColPlan160: col_adjoint_task<ColPlan160>(T, lb, tile, twl, rml); break;
+      case kColPlan192: col_adjoint_task<ColPlan192>(T, lb, tile, twl, rml); break;
+      case kColPlan224: col_adjoint_task<ColPlan224>(T, lb, tile, twl, rml); break;
+      default: break;
+    }
+  }
 }
 __global__ __launch_bounds__(NTC) __attribute__((amdgpu_waves_per_eu(4)))
 void kt_col_adjoint_rt(const Task *__restrict__ tasks, int n, int blk0) {
