@@ -239,7 +239,15 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
   __syncthreads();
   NcColAdjLoad ld{cps, g.ce, z + (long)plane * g.M, g.Gx, gx0, stride};
   NcColAdjStore st{tmp + (long)plane * g.H * g.Gx, g.H, g.Gx, g.Gy, gx0};
-  fft_io<NCC, NCE, true, true, true>(tile, g.py, g.XBa, 1, g.XBa, +1.f, tid, tw, ld, st);
+  // The spreading gathers run on every lane, one cell after another, into
+  // the tile; the transform then starts from LDS.  (As the first stage's
+  // load functor only the lanes of that stage's butterflies gathered, 16
+  // cells each: 192 of 512 lanes at Gy = 384, 8 columns.)
+  const int XB = g.XBa, ncell = XB * g.Gy;
+  for (int k = tid; k < ncell; k += NCC) tile[k] = ld(k % XB, k / XB, 0);
+  __syncthreads();
+  NoLoad nl;
+  fft_io<NCC, NCE, true, false, true>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
 }
 
 // ------------------------------------------------------------ K^H, row pass
