@@ -11,6 +11,7 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "solver.hpp"
@@ -75,6 +76,10 @@ struct Group {
 }  // namespace
 
 struct avionic_engine {
+  // Every entry point taking an engine holds this for its whole call: calls
+  // from several host threads are serialised (submit while another thread
+  // steps or polls is safe).
+  mutable std::mutex mu;
   avionic_engine_config cfg{};
   hipStream_t stream = nullptr;       // compute
   hipStream_t side = nullptr;         // compute, overlapped regulariser kernels
@@ -263,6 +268,7 @@ size_t avionic_request_footprint(const avionic_request_desc *d) {
 int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long long *id) {
   return eguard([&] {
     AV_CHECK(e && id, "null argument");
+    std::lock_guard<std::mutex> lock(e->mu);
     check_desc(d);
     auto s = std::make_unique<Slot>();
     s->method = (int)d->method;
@@ -365,6 +371,7 @@ int avionic_engine_step(avionic_engine *e, void *stream) {
   int resident = 0;
   const int rc = eguard([&] {
     AV_CHECK(e, "engine is null");
+    std::lock_guard<std::mutex> lock(e->mu);
     hipStream_t us = (hipStream_t)stream;
     AV_HIP(hipEventRecord(e->ev_in, us));
     AV_HIP(hipStreamWaitEvent(e->stream, e->ev_in, 0));
@@ -404,6 +411,7 @@ int avionic_engine_poll(avionic_engine *e, long long id, float *x_out, void *str
   int result = 0;
   const int rc = eguard([&] {
     AV_CHECK(e, "engine is null");
+    std::lock_guard<std::mutex> lock(e->mu);
     auto it = e->slots.find(id);
     if (it == e->slots.end()) throw Error(AVIONIC_E_NOTFOUND, "unknown request id");
     Slot *s = it->second.get();
@@ -437,6 +445,7 @@ int avionic_engine_poll(avionic_engine *e, long long id, float *x_out, void *str
 int avionic_engine_cancel(avionic_engine *e, long long id) {
   return eguard([&] {
     AV_CHECK(e, "engine is null");
+    std::lock_guard<std::mutex> lock(e->mu);
     auto it = e->slots.find(id);
     if (it == e->slots.end()) throw Error(AVIONIC_E_NOTFOUND, "unknown request id");
     Slot *s = it->second.get();
@@ -457,6 +466,7 @@ int avionic_engine_cancel(avionic_engine *e, long long id) {
 int avionic_engine_stats(const avionic_engine *e, int *resident, size_t *bytes) {
   return eguard([&] {
     AV_CHECK(e, "engine is null");
+    std::lock_guard<std::mutex> lock(e->mu);
     if (resident) *resident = (int)e->slots.size();
     if (bytes) *bytes = e->used;
   });
@@ -471,6 +481,7 @@ int avionic_engine_time_kernels(avionic_engine *e, int reps, double *ms, const c
                                 size_t *n) {
   return eguard([&] {
     AV_CHECK(e && ms && n && reps > 0, "null argument");
+    std::lock_guard<std::mutex> lock(e->mu);
     Group *g = nullptr;
     for (auto &kv : e->groups)
       if (!kv.second.slots.empty()) {
@@ -507,6 +518,7 @@ int avionic_engine_get_step_log(const avionic_engine *e, long long id, double *s
                                 size_t *n) {
   return eguard([&] {
     AV_CHECK(e && n, "null argument");
+    std::lock_guard<std::mutex> lock(e->mu);
     auto it = e->slots.find(id);
     if (it == e->slots.end()) throw Error(AVIONIC_E_NOTFOUND, "unknown request id");
     const Slot *s = it->second.get();

@@ -271,6 +271,8 @@ int avionic_bin_write(const char *path, const double *re, const double *im,
 /* Batched execution and serving (no reference counterpart: the reference   */
 /* runs one reconstruction per process, main.cpp:789)                        */
 /* ------------------------------------------------------------------------ */
+/* Engine calls are thread-safe: each holds the engine's lock for its whole
+ * duration, so one thread may submit while another steps or polls. */
 
 typedef struct avionic_engine avionic_engine;
 

@@ -229,3 +229,52 @@ def test_engine_full_size_matches_oracle(gpu):
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         print(f"[full size] {ds} duals, {its} iterations: NRMSE vs oracle {err:.2e}", flush=True)
         assert err < tol
+
+
+def test_engine_concurrent_submit_and_step(gpu):
+    """Engine calls from several host threads (ctypes releases the GIL): four
+    threads submit while the main thread steps; every result equals the same
+    request run alone."""
+    import threading
+    import torch
+    import avionic_amd as A
+    data, b1, mask = _inputs(3, 24, 24, 2, seed=31)
+    p = _params(1, 25, 0, mask, 24, 24, 3)
+    dev = lambda a, dt=None: _dev(torch, a, dt)
+    req = lambda: A.Request(1, dev(data), dev(b1), mask=dev(mask, np.float32), params=p)
+    alone = A.Engine(use_graphs=False, max_batch=1)
+    rid = alone.submit(req())
+    ref = torch.empty(3, 24, 24, dtype=torch.complex64, device="cuda")
+    while not alone.poll(rid, ref):
+        alone.step()
+    ref = ref.cpu().numpy()
+    e = A.Engine(use_graphs=True)
+    ids, errors = [], []
+    reqs = [req() for _ in range(8)]
+    torch.cuda.synchronize()
+
+    def submit(chunk):
+        try:
+            for r in chunk:
+                ids.append(e.submit(r))
+        except Exception as ex:  # reported below
+            errors.append(ex)
+
+    threads = [threading.Thread(target=submit, args=(reqs[2 * k:2 * k + 2],)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for _ in range(5):
+        e.step()
+    for t in threads:
+        t.join()
+    assert not errors and len(ids) == 8 and None not in ids
+    outs = {}
+    while len(outs) < len(ids):
+        e.step()
+        for rid in ids:
+            if rid not in outs:
+                buf = torch.empty(3, 24, 24, dtype=torch.complex64, device="cuda")
+                if e.poll(rid, buf):
+                    outs[rid] = buf.cpu().numpy()
+    for rid in ids:
+        assert np.array_equal(outs[rid], ref)
