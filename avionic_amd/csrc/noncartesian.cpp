@@ -12,7 +12,13 @@
 // row pass and the cell -> point list of the adjoint's spreading) are built
 // once on the host here and kept on the device.
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "cartesian.hpp"
@@ -34,15 +40,95 @@ double bessel_i0(double x) {
   return s;
 }
 
-// Launch shapes of the gridding passes (MI355X measurements pick them).
-#ifndef AVIONIC_NC_ADJ_ROWS
-#define AVIONIC_NC_ADJ_ROWS 4
-#endif
-#ifndef AVIONIC_NC_FWD_CG
-#define AVIONIC_NC_FWD_CG 4
-#endif
-constexpr int kNcAdjRows = AVIONIC_NC_ADJ_ROWS;      // image rows per adjoint row block, at most
-constexpr int kNcFwdCoilGroups = AVIONIC_NC_FWD_CG;  // coil groups of the forward column pass
+// Launch shapes of the gridding passes that are picked per geometry: the
+// coil groups of the forward column pass (CG, blocks per image tile) and the
+// image rows per adjoint row block (RBa).  No output depends on them (every
+// element is computed the same way whatever the shape), so they are tuned by
+// timing: the first operator of a geometry runs each candidate a few times
+// on scratch data and keeps the fastest for the process.
+// AVIONIC_NC_TILES="cg,rows" forces a shape; AVIONIC_NC_TUNE=0 keeps the
+// defaults (4 coil groups, 4 rows; the fastest at the 256x256x30x12 radial
+// geometry when measured by hand on MI355X).
+constexpr int kNcAdjRows = 4;
+constexpr int kNcFwdCoilGroups = 4;
+
+struct NcTiles {
+  int cg, rows;
+};
+std::mutex g_tune_mu;
+std::map<std::array<long, 7>, NcTiles> g_tune_cache;
+
+// Mean duration in ms of `reps` runs of f on stream s (after one warm-up).
+template <class F>
+double time_ms(F &&f, hipStream_t s, int reps) {
+  hipEvent_t a, b;
+  AV_HIP(hipEventCreate(&a));
+  AV_HIP(hipEventCreate(&b));
+  f();
+  AV_HIP(hipEventRecord(a, s));
+  for (int r = 0; r < reps; ++r) f();
+  AV_HIP(hipEventRecord(b, s));
+  AV_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  AV_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+NcTiles tune_tiles(NcGeom g, float2 *tmp, float2 *part) {
+  const char *force = std::getenv("AVIONIC_NC_TILES");
+  if (force) {
+    NcTiles t{kNcFwdCoilGroups, kNcAdjRows};
+    if (std::sscanf(force, "%d,%d", &t.cg, &t.rows) == 2) {
+      t.cg = std::max(1, std::min(t.cg, g.C));
+      t.rows = std::max(1, std::min(t.rows, std::max(1, 4096 / g.Gx)));
+      return t;
+    }
+  }
+  const NcTiles def{std::max(1, std::min(g.C, kNcFwdCoilGroups)),
+                    std::max(1, std::min(4096 / g.Gx, kNcAdjRows))};
+  const char *tune = std::getenv("AVIONIC_NC_TUNE");
+  if (tune && std::strcmp(tune, "0") == 0) return def;
+  const std::array<long, 7> key{g.W, g.H, g.C, g.T, g.M, g.Gx, g.Gy};
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune_cache.find(key);
+  if (it != g_tune_cache.end()) return it->second;
+  // scratch inputs (zeros: the passes' work does not depend on the values)
+  DevBuf img, b1, z;
+  img.ensure(sizeof(float2) * (size_t)g.T * g.H * g.W);
+  b1.ensure(sizeof(float2) * (size_t)g.C * g.H * g.W);
+  z.ensure(sizeof(float2) * (size_t)g.T * g.C * g.M);
+  hipStream_t s;
+  AV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  AV_HIP(hipMemsetAsync(img.p, 0, img.bytes, s));
+  AV_HIP(hipMemsetAsync(b1.p, 0, b1.bytes, s));
+  AV_HIP(hipMemsetAsync(z.p, 0, z.bytes, s));
+  NcTiles best = def;
+  double bt = 1e30;
+  for (int cg : {1, 2, 4, 8}) {
+    if (cg > g.C) break;
+    g.CG = cg;
+    const double t = time_ms([&] {
+      launch_nc_forward_grid(g, img.as<float2>(), nullptr, b1.as<float2>(), tmp, part, s);
+    }, s, 3);
+    if (t < bt) { bt = t; best.cg = cg; }
+  }
+  g.CG = best.cg;
+  bt = 1e30;
+  for (int rows : {1, 2, 4, 8}) {
+    if (rows * g.Gx > 4096) break;
+    g.RBa = rows;
+    const double t = time_ms([&] {
+      launch_nc_adjoint(g, z.as<float2>(), b1.as<float2>(), tmp, img.as<float2>(), s);
+    }, s, 3);
+    if (t < bt) { bt = t; best.rows = rows; }
+  }
+  AV_HIP(hipStreamSynchronize(s));
+  (void)hipStreamDestroy(s);
+  g_tune_cache[key] = best;
+  return best;
+}
 
 int pick_cols(int L, int n) {
   int xb = 1;
@@ -202,6 +288,10 @@ class NoncartesianOperator : public Operator {
     g_.rwx = rwx_.as<float4>();
     g_.cptr = cptr_.as<int>();
     g_.ce = ce_.as<int2>();
+    scratch();
+    const NcTiles tiles = tune_tiles(g_, tmp_.as<float2>(), part_.as<float2>());
+    g_.CG = tiles.cg;
+    g_.RBa = tiles.rows;
   }
 
   long dense_elems() const override { return (long)T * C * g_.M; }

@@ -79,6 +79,25 @@ def test_adjointness_and_determinism(gpu):
     assert bool((op.ForwardOperation(z, b1) == khz).all())
 
 
+def test_launch_shapes_do_not_change_results(gpu, monkeypatch):
+    """The coil-group count of the forward column pass and the rows per
+    adjoint row block are tuned per geometry; every shape gives bit-identical
+    K and K^H."""
+    W, H, C, T, nfe, spf = 64, 64, 6, 2, 128, 9
+    tr, dcf = radial(nfe, spf, T)
+    x, b1, z = gpu._t(crandn(T, H, W)), gpu._t(crandn(C, H, W)), gpu._t(crandn(T, C, nfe * spf))
+    outs = []
+    for tiles in ("1,1", "2,4", "4,2", "8,8", None):
+        if tiles is None:
+            monkeypatch.delenv("AVIONIC_NC_TILES", raising=False)
+        else:
+            monkeypatch.setenv("AVIONIC_NC_TILES", tiles)
+        op = make_op(gpu, W, H, C, T, nfe, spf, tr, dcf, 3, 1.5)
+        outs.append((op.BackwardOperation(x, b1), op.ForwardOperation(z, b1)))
+    for kx, khz in outs[1:]:
+        assert bool((kx == outs[0][0]).all()) and bool((khz == outs[0][1]).all())
+
+
 def test_adapt_lambda_formula(gpu):
     tr, dcf = radial(32, 5, 1)
     op = make_op(gpu, 32, 32, 1, 1, 32, 5, tr, dcf, 3, 1.5)
