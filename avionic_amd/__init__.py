@@ -100,6 +100,7 @@ def _declare(L):
         "avionic_bin_read": (_I, [ctypes.c_char_p, _P, _P, _SZ, ctypes.POINTER(_SZ),
                                   ctypes.POINTER(_I)]),
         "avionic_bin_write": (_I, [ctypes.c_char_p, _P, _P, _SZ]),
+        "avionic_normalization_factor": (_I, [_P, _SZ, ctypes.POINTER(ctypes.c_float)]),
         "avionic_cartesian_operator_create": (_I, [_U, _U, _U, _U, _P, _I, ctypes.POINTER(_P)]),
         "avionic_noncartesian_operator_create": (_I, [_U, _U, _U, _U, _U, _U, _P, _P, _I, _D,
                                                       ctypes.POINTER(_P)]),
@@ -491,6 +492,16 @@ def write_bin(path, v):
     im_ = np.ascontiguousarray(v.imag, np.float64) if np.iscomplexobj(v) else None
     _check(lib().avionic_bin_write(os.fsencode(path), re_.ctypes.data_as(_P),
                                    None if im_ is None else im_.ctypes.data_as(_P), re_.size))
+
+
+def normalization_factor(magnitude):
+    """RawDataPreparation::FindNormalizationFactor (raw_data_preparation.cpp
+    :661-686): median of the float32 magnitudes >= 0.9 * max, counted from the
+    first maximum onwards."""
+    v = np.ascontiguousarray(np.asarray(magnitude, np.float32).ravel())
+    out = ctypes.c_float()
+    _check(lib().avionic_normalization_factor(v.ctypes.data_as(_P), v.size, ctypes.byref(out)))
+    return out.value
 
 
 def ellipke(m):

@@ -7,8 +7,8 @@
 // Supported: methods TV, TGV2, ICTGV2; Cartesian and (-n) non-Cartesian data;
 // coil maps (-s) and initial image (-u); the .cfg parameter file (-p) with
 // the reference's keys; -i maxIt, -j stopPDGap, -a adaptlambda, -w density,
-// -e extra results, -v verbose, -g debugstep.  Not in this build (it stops
-// with a message): raw-data import (-r), normalisation (-o), coil-map
+// -e extra results, -v verbose, -g debugstep, -o data normalisation.  Not in
+// this build (it stops with a message): raw-data import (-r), coil-map
 // estimation when -s is missing, and the other methods.
 // --dry-run reads and checks every input and prints the resolved settings as
 // JSON without touching the GPU.
@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -124,7 +125,7 @@ Options parse_args(int argc, char **argv) {
           "  -m, --method TV|TGV2|ICTGV2   -i, --maxIt N     -j, --stopPDGap G\n"
           "  -p, --params file.cfg         -s, --sens b1.bin -u, --uzero u0.bin\n"
           "  -n, --nonuniform  -w, --dens dcf.bin  -a, --adaptlambda  -e, --extradata\n"
-          "  -v, --verbose     -g, --debugstep N   --dry-run\n");
+          "  -v, --verbose     -g, --debugstep N   -o, --normalize   --dry-run\n");
       std::exit(0);
     } else if (a == "-d" || a == "--dims") {
       o.dims = parse_dims(need(i));
@@ -167,7 +168,6 @@ Options parse_args(int argc, char **argv) {
     }
   }
   if (o.rawdata) die("raw-data import (-r) is not part of this build", 3);
-  if (o.normalize) die("data normalisation (-o) needs the coil-map estimation, which is not part of this build", 3);
   if (pos.size() != 3) die("expected <kdata> <mask/traj> <output> (got " + std::to_string(pos.size()) + ")");
   o.kdata = pos[0];
   o.mask = pos[1];
@@ -266,6 +266,93 @@ void upload(Dev &d, const std::vector<float> &h) {
     die("host-to-device copy failed", 1);
 }
 
+// Magnitude of the root-sum-of-squares over coils of coil images produced
+// one coil at a time by `image(c, out)` (out: HW complex values).
+std::vector<float> rss_of(size_t C, size_t HW, const std::function<void(size_t, std::vector<float> &)> &image) {
+  std::vector<float> u(HW, 0.f), img(2 * HW);
+  for (size_t c = 0; c < C; ++c) {
+    image(c, img);
+    for (size_t i = 0; i < HW; ++i) {
+      const float a = std::hypot(img[2 * i], img[2 * i + 1]);
+      u[i] += a * a;
+    }
+  }
+  for (float &v : u) v = std::sqrt(v);
+  return u;
+}
+
+// Normalisation factor of -o (main.cpp:575-579), computed from the prepared
+// data.  Cartesian (RawDataPreparation::NormalizeCartData,
+// raw_data_preparation.cpp:821-844): the time-averaged coil images
+// F(sum_t d_t / max(1, sum_t M_t)) of a centred operator
+// (cartesian_coil_construction.cpp:23-76; the operator of
+// raw_data_preparation.cpp:371-375 is centred), their root-sum-of-squares, and
+// 255 / median.  Radial (NormalizeNonCartData, :736-793): the adjoint
+// gridding of every frame's points together into one image per coil
+// (noncartesian_coil_construction.cpp:21-123; kernel width 3, grid
+// oversampling 2, :84-86), its root-sum-of-squares, and 255 * frames / median.
+float datanorm_of(bool nonuniform, unsigned W, unsigned H, unsigned C, unsigned T, size_t M,
+                  unsigned nfe, unsigned spf, const std::vector<float> &kdh,
+                  const std::vector<float> &maskh, const std::vector<float> &wh) {
+  const size_t HW = (size_t)W * H;
+  std::vector<float> ones(2 * HW, 0.f);
+  for (size_t i = 0; i < HW; ++i) ones[2 * i] = 1.f;
+  Dev dones(sizeof(float) * ones.size()), dimg(sizeof(float) * 2 * HW);
+  upload(dones, ones);
+  avionic_operator *op = nullptr;
+  std::vector<float> u;
+  auto download = [&](std::vector<float> &out) {
+    if (hipMemcpy(out.data(), dimg.p, sizeof(float) * out.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      die("device-to-host copy failed", 1);
+  };
+  if (!nonuniform) {
+    std::vector<float> cnt(HW, 0.f);
+    for (size_t t = 0; t < T; ++t)
+      for (size_t i = 0; i < HW; ++i) cnt[i] += maskh[t * HW + i];
+    for (float &v : cnt) v = std::max(v, 1.f);
+    check(avionic_cartesian_operator_create(W, H, 1, 1, nullptr, 1, &op), "normalisation operator");
+    Dev davg(sizeof(float) * 2 * HW);
+    std::vector<float> avg(2 * HW);
+    u = rss_of(C, HW, [&](size_t c, std::vector<float> &img) {
+      std::fill(avg.begin(), avg.end(), 0.f);
+      for (size_t t = 0; t < T; ++t)
+        for (size_t i = 0; i < 2 * HW; ++i) avg[i] += kdh[2 * (t * C + c) * HW + i];
+      for (size_t i = 0; i < HW; ++i) {
+        avg[2 * i] /= cnt[i];
+        avg[2 * i + 1] /= cnt[i];
+      }
+      upload(davg, avg);
+      check(avionic_operator_forward(op, davg.f(), dimg.f(), dones.f(), nullptr), "time-averaged image");
+      download(img);
+    });
+  } else {
+    // every frame's points as one frame: trajectory [2][T*M], density [T*M]
+    const size_t P = (size_t)T * M;
+    std::vector<float> traj(2 * P), zc(2 * P);
+    for (size_t t = 0; t < T; ++t)
+      for (size_t m = 0; m < M; ++m) {
+        traj[t * M + m] = maskh[2 * t * M + m];
+        traj[P + t * M + m] = maskh[2 * t * M + M + m];
+      }
+    Dev dtraj(sizeof(float) * traj.size()), dw(sizeof(float) * wh.size()), dz(sizeof(float) * zc.size());
+    upload(dtraj, traj);
+    upload(dw, wh);
+    check(avionic_noncartesian_operator_create(W, H, 1, 1, nfe, spf * T, dtraj.f(), dw.f(), 3, 2.0, &op),
+          "normalisation operator");
+    u = rss_of(C, HW, [&](size_t c, std::vector<float> &img) {
+      for (size_t t = 0; t < T; ++t)
+        std::memcpy(&zc[2 * t * M], &kdh[2 * (t * C + c) * M], sizeof(float) * 2 * M);
+      upload(dz, zc);
+      check(avionic_operator_forward(op, dz.f(), dimg.f(), dones.f(), nullptr), "time-averaged image");
+      download(img);
+    });
+  }
+  avionic_operator_destroy(op);
+  float median = 0.f;
+  check(avionic_normalization_factor(u.data(), u.size(), &median), "normalisation factor");
+  return nonuniform ? 255.f * (float)T / median : 255.f / median;
+}
+
 void export_cb(const char *name, const float *data, size_t count, void *user) {
   const std::string path = *(const std::string *)user + "/" + name;
   std::vector<double> re(count), im(count);
@@ -332,11 +419,13 @@ int main(int argc, char **argv) {
         "\"sigma\": %g, \"tau\": %g, \"sigmaTauRatio\": %g, \"timeSpaceWeight\": %g, "
         "\"timeSpaceWeight2\": %g, \"lambda\": %g, \"adaptLambda\": %d, \"adaptLambdaK\": %g, "
         "\"adaptLambdaD\": %g, \"alpha0\": %g, \"alpha1\": %g, \"alpha\": %g, "
-        "\"kernelWidth\": %d, \"osf\": %g, \"initial_image\": \"%s\", \"output\": \"%s\"}\n",
+        "\"kernelWidth\": %d, \"osf\": %g, \"initial_image\": \"%s\", \"normalize\": %s, "
+        "\"output\": \"%s\"}\n",
         o.method.c_str(), W, H, C, T, o.nonuniform ? "true" : "false", M, p.maxIt, p.stopPDGap,
         p.sigma, p.tau, p.sigmaTauRatio, p.timeSpaceWeight, p.timeSpaceWeight2, p.lambda,
         p.adaptLambda, p.adaptLambdaK, p.adaptLambdaD, p.alpha0, p.alpha1, p.alpha, kw, osf,
-        o.u0.empty() ? (o.nonuniform ? "zero" : "time-averaged") : "file", o.output.c_str());
+        o.u0.empty() ? (o.nonuniform ? "zero" : "time-averaged") : "file",
+        o.normalize ? "true" : "false", o.output.c_str());
     return 0;
   }
 
@@ -368,6 +457,14 @@ int main(int argc, char **argv) {
           z[0] *= s;
           z[1] *= s;
         }
+  }
+
+  // ---- normalisation (-o, main.cpp:575-579): scale the prepared data
+  float datanorm = 1.f;
+  if (o.normalize) {
+    datanorm = datanorm_of(o.nonuniform, W, H, C, T, M, d.readouts, d.enc1, kdh, maskh, wh);
+    for (float &v : kdh) v *= datanorm;
+    if (o.verbose) std::printf("avionic: datanorm factor %g\n", datanorm);
   }
 
   Dev dk(sizeof(float) * kdh.size()), dmask(sizeof(float) * maskh.size()),
@@ -442,6 +539,13 @@ int main(int argc, char **argv) {
     im[i] = xh[2 * i + 1];
   }
   check(avionic_bin_write(o.output.c_str(), re.data(), im.data(), N), "writing the result");
+  // the factor next to the output, as a one-element complex vector
+  // (main.cpp:792-796)
+  if (o.normalize) {
+    const double re1 = datanorm, im1 = 0.0;
+    check(avionic_bin_write((dir_of(o.output) + "/datanorm_factor.bin").c_str(), &re1, &im1, 1),
+          "writing datanorm_factor.bin");
+  }
   if (o.extradata) {
     std::string dir = dir_of(o.output);
     check(avionic_recon_export(rec, export_cb, &dir), "exporting additional results");

@@ -148,6 +148,7 @@ void make_fft_plan(int L, FftPlan &p, void **d_tw) {
   }
   AV_HIP(hipMalloc(d_tw, sizeof(float2) * L));
   AV_HIP(hipMemcpy(*d_tw, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+  AV_HIP(hipStreamSynchronize(nullptr));  // landed before any stream reads it
   p.tw = (const float2 *)*d_tw;
 }
 
@@ -227,6 +228,12 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   g.rowmap = (const int *)d_rowmap;
   make_fft_plan(W, g.pw, &d_twx);
   make_fft_plan(H, g.ph, &d_twy);
+  // The tables above come from pageable host memory: such a copy (on the
+  // null stream) may return before its data has landed, and the kernels that
+  // index with them may run on non-blocking streams, which do not wait for
+  // the null stream.  Waiting for the null stream alone leaves those streams'
+  // own work running.
+  AV_HIP(hipStreamSynchronize(nullptr));
 }
 
 void launch_col_forward(const CartGeom &g, const float2 *u, const float2 *usub, const float2 *b1,

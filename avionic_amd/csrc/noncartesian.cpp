@@ -271,6 +271,11 @@ class NoncartesianOperator : public Operator {
     up(ce_, ce.data(), sizeof(int2) * ce.size());
     make_fft_plan(Gx, g_.px, &twx_.p);
     make_fft_plan(Gy, g_.py, &twy_.p);
+    // The tables come from pageable host memory: such a copy may return before
+    // its data has landed.  The row pass writes partials at indices it reads
+    // from them, and the launch-shape timing below (like a caller's
+    // non-blocking stream) does not wait for the copies by itself.
+    AV_HIP(hipStreamSynchronize(nullptr));
 
     g_.W = W; g_.H = H; g_.C = C; g_.T = T; g_.M = (int)M;
     g_.Gx = Gx; g_.Gy = Gy;

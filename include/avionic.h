@@ -252,6 +252,17 @@ int avionic_fft2(float *data, unsigned height, unsigned width, unsigned batch,
                  int sign, int centered, int unitary, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Raw-data normalisation (host memory; no device needed)                    */
+/* ------------------------------------------------------------------------ */
+/* RawDataPreparation::FindNormalizationFactor (raw_data_preparation.cpp
+ * :661-686, utils::Median utils.cpp:1442-1462): the median of the magnitudes
+ * >= 0.9 * max, taken from the first maximum onwards (values before it are not
+ * counted, as in the reference).  The CLI's -o scales the data by
+ * 255 / median (Cartesian, :795-818) or 255 * frames / median (radial,
+ * :736-793).  Fails on n == 0 or a non-positive median. */
+int avionic_normalization_factor(const float *magnitude, size_t n, float *median);
+
+/* ------------------------------------------------------------------------ */
 /* .bin vector files (host memory; no device needed)                         */
 /* ------------------------------------------------------------------------ */
 /* The reference's vector file: uint8 iscomplex, uint32 n, n float64 real

@@ -83,6 +83,8 @@ def test_cli_resolves_config_and_overrides(tmp_path):
     cfg = json.loads(_run(["-m", "tv", "-i", "5", "-a"] + base).stdout)
     assert cfg["method"] == "TV" and cfg["maxIt"] == 5
     assert cfg["adaptLambda"] == 1 and cfg["adaptLambdaK"] == 0.5 and cfg["adaptLambdaD"] == 2
+    assert cfg["normalize"] is False
+    assert json.loads(_run(["-o"] + base).stdout)["normalize"] is True
 
 
 @pytest.mark.skipif(not os.path.exists(CLI), reason="avionic CLI not built")
@@ -96,7 +98,7 @@ def test_cli_rejects_what_it_cannot_do(tmp_path):
     r = _run(["-d", "8:6:1:8:6:1:3:3", "-s", str(tmp_path / "b1.bin"), "-p",
               str(tmp_path / "p.cfg")] + files + ["--dry-run"], ok=False)
     assert r.returncode == 2 and "k-space data has" in r.stderr
-    for flag, code in (("-r", 3), ("-o", 3), ("--bogus", 2)):
+    for flag, code in (("-r", 3), ("--bogus", 2)):
         r = _run([flag, "-d", dims] + files, ok=False)
         assert r.returncode == code, (flag, r.stderr)
     r = _run(["-m", "ICTV", "-d", dims, "-s", str(tmp_path / "b1.bin"), "-p",

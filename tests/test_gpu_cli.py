@@ -84,3 +84,113 @@ def test_cli_radial_matches_library(gpu, tmp_path):
     rec.IterativeReconstruction(gpu._t(data), x, gpu._t(b1))
     ref = gpu._n(x)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def _median_kept(mag):
+    """Median of the magnitudes >= 0.9 * max from the first maximum onwards
+    (raw_data_preparation.cpp:661-686), in float64, with the distance of the
+    nearest magnitude to the threshold (a float32 computation agrees with this
+    selection when that distance is well above float32 rounding)."""
+    v = np.asarray(mag, np.float64).ravel()
+    i = int(np.argmax(v))
+    thr = v[i] * 0.9
+    kept = np.sort(v[i:][v[i:] >= thr])
+    k = kept.size
+    med = kept[k // 2] if k % 2 else 0.5 * (kept[k // 2 - 1] + kept[k // 2])
+    return med, np.min(np.abs(v[i:] - thr)) / thr
+
+
+def _disk(rng, T, H, W, r):
+    """Frames of a disk of magnitude about 10: a plateau, so that many pixels
+    lie within 10 % of the maximum and the selection matters."""
+    yy, xx = np.mgrid[0:H, 0:W]
+    d = ((yy - H / 2) ** 2 + (xx - W / 2) ** 2) < r * r
+    return np.repeat((d * (10 + 0.3 * rng.standard_normal((H, W))))[None], T, 0) + 0j
+
+
+def test_cli_normalization_cartesian(gpu, tmp_path):
+    """-o: the data is scaled by 255 / median of the time-averaged
+    root-sum-of-squares image of a centred operator
+    (raw_data_preparation.cpp:821-844, cartesian_coil_construction.cpp:23-76);
+    the factor is written to datanorm_factor.bin (main.cpp:792-796) and the
+    result is the reconstruction of the scaled data."""
+    rng = np.random.default_rng(11)
+    W, H, C, T = 32, 24, 3, 4
+    img = _disk(rng, T, H, W, 8)
+    b1 = rng.standard_normal((C, H, W)) * 0.05 + 1.0 + 0.1j
+    k = O.cart_K(img, b1, None, centered=True)
+    k = k + 0.05 * (rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape))
+    mask = (rng.random((T, H, 1)) < 0.5) * np.ones((1, 1, W))
+    u0 = rng.standard_normal((H, W)) + 0j
+    for name, v in (("k", k), ("m", mask.astype(float)), ("b1", b1), ("u0", u0)):
+        A.write_bin(tmp_path / f"{name}.bin", v)
+    (tmp_path / "p.cfg").write_text("[tgv2]\nlambda = 3.0\n")
+    _run(["-o", "-m", "TGV2", "-i", "30", "-p", str(tmp_path / "p.cfg"), "-d",
+          f"{W}:{H}:1:{W}:{H}:1:{C}:{T}", "-s", str(tmp_path / "b1.bin"), "-u",
+          str(tmp_path / "u0.bin"), str(tmp_path / "k.bin"), str(tmp_path / "m.bin"),
+          str(tmp_path / "x.bin")])
+    fac = A.read_bin(tmp_path / "datanorm_factor.bin")
+    assert fac.size == 1 and np.iscomplexobj(fac) and fac[0].imag == 0
+    # the factor from the fp64 oracle: centred unitary transform of each coil's
+    # time average, root-sum-of-squares over coils
+    km = k * mask[:, None]
+    avg = km.sum(0) / np.maximum(mask.sum(0), 1.0)[None]
+    ones = np.ones((1, H, W), complex)
+    rss = np.sqrt(sum(np.abs(O.cart_KH(avg[c][None, None], ones, None, centered=True)[0]) ** 2
+                      for c in range(C)))
+    med, margin = _median_kept(rss)
+    assert margin > 1e-4, "fixture too close to the selection threshold"
+    # 22 pixels lie above 0.9 * max, 3 of them from the first maximum on
+    v = rss.ravel()
+    assert (v >= 0.9 * v.max()).sum() > 10 and (v[np.argmax(v):] >= 0.9 * v.max()).sum() < 10
+    print(f"[gpu cli] datanorm {fac[0].real:.7g} vs oracle {255 / med:.7g}", flush=True)
+    assert fac[0].real == pytest.approx(255.0 / med, rel=1e-5)
+    # the result: the library's reconstruction of the scaled data
+    got = A.read_bin(tmp_path / "x.bin").reshape(T, H, W)
+    op = A.CartesianOperator(W, H, C, T, gpu._r(mask), centered=False)
+    rec = A.TGV2(W, H, C, T, op, None)
+    p = rec.GetParams()
+    p.update(maxIt=30, adaptLambda=0)
+    p["lambda"] = 3.0
+    rec.SetParams(p)
+    x = gpu._t(np.repeat(u0[None], T, 0))
+    data = (km.astype(np.complex64) * np.float32(fac[0].real)).astype(np.complex64)
+    rec.IterativeReconstruction(gpu._t(data), x, gpu._t(b1))
+    ref = gpu._n(x)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def test_cli_normalization_radial(gpu, tmp_path):
+    """-o on radial data: 255 * frames / median of the root-sum-of-squares of
+    the adjoint gridding of all frames' points together, kernel width 3 and
+    grid oversampling 2 (raw_data_preparation.cpp:736-793,
+    noncartesian_coil_construction.cpp:21-123), against the fp64 gridding."""
+    rng = np.random.default_rng(12)
+    W = H = 32
+    C, T, nfe, spf = 2, 3, 48, 7
+    M = nfe * spf
+    tr = O.radial_trajectory(nfe, spf, T).astype(np.float32).astype(np.float64)
+    dcf = (np.sqrt(tr[:, 0] ** 2 + tr[:, 1] ** 2) + 0.01).astype(np.float32).astype(np.float64)
+    b1 = np.ones((C, H, W), complex)
+    k = O.nufft_K(_disk(rng, T, H, W, 9), b1, tr, None, J=3, osf=2.0)
+    k = k + 0.05 * (rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape))
+    for name, v in (("k", k), ("traj", tr), ("w", dcf), ("b1", b1)):
+        A.write_bin(tmp_path / f"{name}.bin", v)
+    (tmp_path / "p.cfg").write_text("[tv]\nlambda = 2.0\n")
+    _run(["-o", "-n", "-m", "TV", "-i", "10", "-p", str(tmp_path / "p.cfg"), "-d",
+          f"{W}:{H}:1:{nfe}:{spf}:1:{C}:{T}", "-s", str(tmp_path / "b1.bin"), "-w",
+          str(tmp_path / "w.bin"), str(tmp_path / "k.bin"), str(tmp_path / "traj.bin"),
+          str(tmp_path / "x.bin")])
+    fac = A.read_bin(tmp_path / "datanorm_factor.bin")
+    # all frames as one: trajectory [1][2][T*M], density [1][T*M]; the data
+    # carries sqrt(density) as prepared (main.cpp:546-570)
+    trc = np.concatenate([tr[:, 0].ravel()[None], tr[:, 1].ravel()[None]])[None]
+    dc = dcf.reshape(1, T * M)
+    kw = k * np.sqrt(dcf)[:, None, :]
+    ones = np.ones((1, H, W), complex)
+    rss = np.sqrt(sum(np.abs(O.nufft_KH(kw[:, c].reshape(1, 1, T * M), ones, trc, dc, J=3,
+                                        osf=2.0)[0]) ** 2 for c in range(C)))
+    med, margin = _median_kept(rss)
+    assert margin > 1e-4, "fixture too close to the selection threshold"
+    print(f"[gpu cli] radial datanorm {fac[0].real:.7g} vs oracle {255 * T / med:.7g}", flush=True)
+    assert fac[0].real == pytest.approx(255.0 * T / med, rel=1e-4)
