@@ -28,7 +28,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libavionic.so")
 
-TV_METHOD, TGV2_METHOD, ICTGV2_METHOD = 0, 1, 2
+TV_METHOD, TGV2_METHOD, ICTGV2_METHOD, TVTEMP_METHOD = 0, 1, 2, 3
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -390,6 +390,12 @@ class TGV2(PDRecon):
 class ICTGV2(PDRecon):
     """ICTGV2 (CUDA/include/ictgv2.h)."""
     METHOD = ICTGV2_METHOD
+
+
+class TVTEMP(PDRecon):
+    """TVTEMP, temporal TV (CUDA/include/tv_temp.h): parameter dt is used as
+    given; dx, dy and timeSpaceWeight do not apply."""
+    METHOD = TVTEMP_METHOD
 
 
 def default_params(method) -> dict:

@@ -137,8 +137,8 @@ size_t avionic_operator_kspace_size(const avionic_operator *op) {
 int avionic_default_params(avionic_method method, avionic_params *params) {
   return guard([&] {
     NEED(params);
-    AV_CHECK(method >= 0 && method <= 2, "unknown method");
-    Recon::defaults((int)method, params);
+    AV_CHECK(method >= 0 && method <= AVIONIC_TVTEMP, "unknown method");
+    Recon::defaults(kernel_method((int)method), params);
   });
 }
 

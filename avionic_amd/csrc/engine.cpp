@@ -195,9 +195,15 @@ avionic_params request_params(const avionic_request_desc *d, const Operator &op)
     p.lambda = op.adapt_lambda(p.adaptLambdaK, p.adaptLambdaD);
   }
   double ds, dt;
-  AV_CHECK(time_space_weights(p.timeSpaceWeight, ds, dt), "ComputeTimeSpaceWeights: Invalid ratio.");
-  p.dx = p.dy = ds;
-  p.dt = dt;
+  if (d->method == AVIONIC_TVTEMP) {
+    // temporal TV: zero spatial weights, dt as configured (tv_temp.cpp:98)
+    AV_CHECK(p.dt > 0.0, "dt must be positive");
+    p.dx = p.dy = INFINITY;
+  } else {
+    AV_CHECK(time_space_weights(p.timeSpaceWeight, ds, dt), "ComputeTimeSpaceWeights: Invalid ratio.");
+    p.dx = p.dy = ds;
+    p.dt = dt;
+  }
   if (d->method == AVIONIC_ICTGV2) {
     AV_CHECK(time_space_weights(p.timeSpaceWeight2, ds, dt), "ComputeTimeSpaceWeights: Invalid ratio.");
     p.dx2 = p.dy2 = ds;
@@ -210,7 +216,7 @@ avionic_params request_params(const avionic_request_desc *d, const Operator &op)
 
 void check_desc(const avionic_request_desc *d) {
   AV_CHECK(d != nullptr, "request descriptor is null");
-  AV_CHECK(d->method >= 0 && d->method <= 2, "unknown method");
+  AV_CHECK(d->method >= 0 && d->method <= AVIONIC_TVTEMP, "unknown method");
   AV_CHECK(d->width > 0 && d->height > 0 && d->coils > 0 && d->frames > 0, "empty request");
   AV_CHECK(d->data != nullptr && d->b1 != nullptr, "request data / b1 missing");
   check_grid(Grid3{(int)d->width, (int)d->height, (int)d->frames,
@@ -258,7 +264,7 @@ size_t avionic_request_footprint(const avionic_request_desc *d) {
     const long N = (long)d->width * d->height * d->frames;
     const int maxit = d->params ? (int)d->params->maxIt : 500;
     const int ds = d->params ? d->params->dualStorage : 0;
-    return slot_bytes((int)d->method, ds, N, plan.kspace_elems(),
+    return slot_bytes(kernel_method((int)d->method), ds, N, plan.kspace_elems(),
                       (long)d->coils * d->height * d->width, maxit);
   } catch (...) {
     return 0;
@@ -271,7 +277,7 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     std::lock_guard<std::mutex> lock(e->mu);
     check_desc(d);
     auto s = std::make_unique<Slot>();
-    s->method = (int)d->method;
+    s->method = kernel_method((int)d->method);
     s->W = (int)d->width; s->H = (int)d->height; s->C = (int)d->coils; s->T = (int)d->frames;
     s->N = (long)s->W * s->H * s->T;
     s->op = std::make_unique<CartesianOperator>(s->W, s->H, s->C, s->T, d->mask, d->centered);

@@ -9,6 +9,12 @@
 namespace avionic {
 
 enum Method { kTV = 0, kTGV2 = 1, kICTGV2 = 2 };
+// Temporal TV (tv_temp.cpp) runs on the TV kernels with zero spatial weights
+// (1/dx = 1/dy = 0): its spatial dual components then stay exactly zero, so
+// the norm_3 projection, the divergence and the step-rule sums equal the
+// one-component temporal ones of the reference bit for bit.
+constexpr int kTVTEMP = 3;
+inline int kernel_method(int m) { return m == kTVTEMP ? kTV : m; }
 
 inline int primal_components(int m) { return m == kTV ? 1 : m == kTGV2 ? 4 : 8; }
 inline int dual_components(int m) { return m == kTV ? 3 : m == kTGV2 ? 9 : 18; }

@@ -125,7 +125,8 @@ bool time_space_weights(double tsw, double &ds, double &dt) {
   return true;
 }
 
-// InitParams: tv.cpp:39-57, tgv2.cpp:36-56, ictgv2.cpp:37-64.
+// InitParams: tv.cpp:39-57, tgv2.cpp:36-56, ictgv2.cpp:37-64; TVTEMP
+// (tv_temp.cpp:39-54) sets the TV values it uses (dt = 1).
 void Recon::defaults(int method, avionic_params *p) {
   *p = avionic_params{};
   p->maxIt = 500;
@@ -155,8 +156,9 @@ void Recon::defaults(int method, avionic_params *p) {
 }
 
 Recon::Recon(int method_, int W_, int H_, int C_, int T_, const avionic_params *p, Operator *op_)
-    : method(method_), W(W_), H(H_), C(C_), T(T_), N((long)W_ * H_ * T_), op(op_) {
-  AV_CHECK(method >= 0 && method <= 2, "unknown method");
+    : method(kernel_method(method_)), temporal(method_ == kTVTEMP), W(W_), H(H_), C(C_), T(T_),
+      N((long)W_ * H_ * T_), op(op_) {
+  AV_CHECK(method_ >= 0 && method_ <= kTVTEMP, "unknown method");
   AV_CHECK(op != nullptr, "operator is null");
   AV_CHECK(op->W == W && op->H == H && op->C == C && op->T == T,
            "operator dimensions do not match the solver");
@@ -171,6 +173,7 @@ Recon::Recon(int method_, int W_, int H_, int C_, int T_, const avionic_params *
 
 void Recon::d6(double out[6]) const {
   out[0] = params.dx; out[1] = params.dy; out[2] = params.dt;
+  if (temporal) out[0] = out[1] = INFINITY;  // 1/dx = 1/dy = 0
   out[3] = params.dx2; out[4] = params.dy2; out[5] = params.dt2;
 }
 
@@ -281,10 +284,15 @@ double Recon::pdgap_dense(const float2 *x, const float2 *y, const float2 *z, con
 void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s) {
   // derivative weights from the time/space ratios (tv.cpp:103-104, ictgv2.cpp:300-306)
   double ds, dt;
-  AV_CHECK(time_space_weights(params.timeSpaceWeight, ds, dt),
-           "ComputeTimeSpaceWeights: Invalid ratio.");
-  params.dx = params.dy = ds;
-  params.dt = dt;
+  if (temporal) {
+    // TVTEMP uses dt as configured, no time/space weights (tv_temp.cpp:98)
+    AV_CHECK(params.dt > 0.0, "dt must be positive");
+  } else {
+    AV_CHECK(time_space_weights(params.timeSpaceWeight, ds, dt),
+             "ComputeTimeSpaceWeights: Invalid ratio.");
+    params.dx = params.dy = ds;
+    params.dt = dt;
+  }
   if (method == kICTGV2) {
     AV_CHECK(time_space_weights(params.timeSpaceWeight2, ds, dt),
              "ComputeTimeSpaceWeights: Invalid ratio.");

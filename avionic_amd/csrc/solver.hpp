@@ -101,7 +101,8 @@ class CartesianOperator : public Operator {
 class Recon {
  public:
   Recon(int method, int W, int H, int C, int T, const avionic_params *params, Operator *op);
-  int method;
+  int method;            // kernel method (kTV for temporal TV)
+  bool temporal = false;  // TVTEMP: spatial weights zero, dt as given
   int W, H, C, T;
   long N;
   Operator *op;

@@ -47,7 +47,9 @@ extern "C" {
 typedef enum avionic_method {
   AVIONIC_TV = 0,     /* TV     (CUDA/include/tv.h)     */
   AVIONIC_TGV2 = 1,   /* TGV2   (CUDA/include/tgv2.h)   */
-  AVIONIC_ICTGV2 = 2  /* ICTGV2 (CUDA/include/ictgv2.h) */
+  AVIONIC_ICTGV2 = 2, /* ICTGV2 (CUDA/include/ictgv2.h) */
+  AVIONIC_TVTEMP = 3  /* TVTEMP, temporal TV (CUDA/include/tv_temp.h): dt is
+                         used as given, dx, dy and timeSpaceWeight are unused */
 } avionic_method;
 
 /* Parameters: the union of PDParams (pd_recon.h:34-74), TGV2Params

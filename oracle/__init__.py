@@ -21,8 +21,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-TV, TGV2, ICTGV2 = 0, 1, 2
-METHODS = {"tv": TV, "tgv2": TGV2, "ictgv2": ICTGV2}
+TV, TGV2, ICTGV2, TVTEMP = 0, 1, 2, 3
+METHODS = {"tv": TV, "tgv2": TGV2, "ictgv2": ICTGV2, "tvtemp": TVTEMP}
 
 
 class OrcParams(ctypes.Structure):
@@ -41,6 +41,7 @@ class OrcParams(ctypes.Structure):
         ("schedule", ctypes.c_int),
         ("gapEvery", ctypes.c_int),
         ("centered", ctypes.c_int),
+        ("dt", ctypes.c_double),
     ]
 
 
@@ -86,6 +87,11 @@ def _declare(L):
     L.orc_adapt_step.argtypes = [_D, _D, _D, _P, _P]
     L.orc_tv_norm.argtypes = [_P, _I, _I, _I, _D, _D, _D]
     L.orc_tv_norm.restype = _D
+    for f in ("orc_tgrad", "orc_tgrad_T"):
+        getattr(L, f).argtypes = [_P, _I, _I, _I, _D, _P]
+        getattr(L, f).restype = None
+    L.orc_tvtemp_norm.argtypes = [_P, _I, _I, _I, _D]
+    L.orc_tvtemp_norm.restype = _D
     L.orc_tgv2_norm.argtypes = [_P, _P, _I, _I, _I, _D, _D, _D, _D, _D]
     L.orc_tgv2_norm.restype = _D
     L.orc_ictgv2_norm.argtypes = [_P, _P, _P, _P, _I, _I, _I] + [_D] * 9
@@ -235,6 +241,30 @@ def solver_adapt_step(method, delta, b1, mask, centered, d6, ratio, sigma, tau):
 
 
 # ------------------------------------------------------------------- functionals
+def tgrad(x, dt=1.0):
+    """TVTEMP's temporal forward difference (utils.cpp:134-143)."""
+    x = _c(x)
+    T, H, W = x.shape
+    out = np.zeros_like(x)
+    lib().orc_tgrad(_ptr(x), W, H, T, float(dt), _ptr(out))
+    return out
+
+
+def tgrad_T(y, dt=1.0):
+    """Adjoint of tgrad, i.e. minus the temporal divergence (utils.cpp:795-805)."""
+    y = _c(y)
+    T, H, W = y.shape
+    out = np.zeros_like(y)
+    lib().orc_tgrad_T(_ptr(y), W, H, T, float(dt), _ptr(out))
+    return out
+
+
+def tvtemp_norm(u, dt=1.0):
+    u = _c(u)
+    T, H, W = u.shape
+    return lib().orc_tvtemp_norm(_ptr(u), W, H, T, float(dt))
+
+
 def tv_norm(u, dx, dy, dt):
     u = _c(u)
     T, H, W = u.shape
@@ -280,7 +310,7 @@ def default_params(method):
     m = METHODS[method] if isinstance(method, str) else int(method)
     p = dict(method=m, maxIt=500, sigma=1.0 / 3.0, tau=1.0 / 3.0, sigmaTauRatio=1.0,
              tsw=5.0, tsw2=1.0, lambda_=1.0, alpha0=float(np.sqrt(2.0)), alpha1=1.0,
-             alpha=0.5, schedule=0, gapEvery=0, centered=0)
+             alpha=0.5, schedule=0, gapEvery=0, centered=0, dt=1.0)
     if m == ICTGV2:
         p.update(tsw=6.5, tsw2=2.5, alpha=0.6423)
     return p

@@ -407,10 +407,50 @@ void orc_adapt_step(double nKx, double nx, double ratio, double *sigma,
 /* ======================================================================== */
 
 int orc_primal_components(int method) {
-  return method == ORC_TV ? 1 : method == ORC_TGV2 ? 4 : 8;
+  return (method == ORC_TV || method == ORC_TVTEMP) ? 1 : method == ORC_TGV2 ? 4 : 8;
 }
 int orc_dual_components(int method) {
-  return method == ORC_TV ? 3 : method == ORC_TGV2 ? 9 : 18;
+  return method == ORC_TVTEMP ? 1 : method == ORC_TV ? 3 : method == ORC_TGV2 ? 9 : 18;
+}
+
+/* Temporal forward difference and its adjoint (TVTEMP: utils::Gradient_temp
+ * and Divergence_temp, utils.cpp:134-143 and 795-805; the difference is zero
+ * at the last frame, as for the t component of the full gradient).
+ * tgrad: out_t = (x_{t+1} - x_t) / dt for t < T-1, 0 at t = T-1.
+ * tgrad_T: the adjoint, out_t = (y_{t-1} [t > 0] - y_t [t < T-1]) / dt, which
+ * is minus the temporal divergence. */
+static void tgrad(const dc *x, int W, int H, int T, double dt, dc *out) {
+  long F = (long)W * H;
+  for (int t = 0; t < T; t++)
+    for (long i = 0; i < F; i++)
+      out[t * F + i] = t < T - 1 ? (x[(t + 1) * F + i] - x[t * F + i]) / dt : 0.0;
+}
+static void tgrad_T(const dc *y, int W, int H, int T, double dt, dc *out) {
+  long F = (long)W * H;
+  for (int t = 0; t < T; t++)
+    for (long i = 0; i < F; i++) {
+      dc v = 0.0;
+      if (t > 0) v += y[(t - 1) * F + i];
+      if (t < T - 1) v -= y[t * F + i];
+      out[t * F + i] = v / dt;
+    }
+}
+
+double orc_tvtemp_norm(const double *ur, int W, int H, int T, double dt) {
+  long N = (long)W * H * T;
+  dc *g = malloc(sizeof(dc) * N);
+  tgrad((const dc *)ur, W, H, T, dt, g);
+  double s = 0.0;
+  for (long i = 0; i < N; i++) s += cabs(g[i]);
+  free(g);
+  return s;
+}
+
+void orc_tgrad(const double *x, int W, int H, int T, double dt, double *out) {
+  tgrad((const dc *)x, W, H, T, dt, (dc *)out);
+}
+void orc_tgrad_T(const double *y, int W, int H, int T, double dt, double *out) {
+  tgrad_T((const dc *)y, W, H, T, dt, (dc *)out);
 }
 
 /* Regulariser part of K applied to a primal state (P comps) -> D comps.
@@ -421,6 +461,10 @@ int orc_dual_components(int method) {
 static void reg_K(int method, const dc *x, int W, int H, int T, const double *d6,
                   dc *out, dc *tmp) {
   long N = (long)W * H * T;
+  if (method == ORC_TVTEMP) {  /* [grad_t u] (tv_temp.cpp:130-132) */
+    tgrad(x, W, H, T, d6[2], out);
+    return;
+  }
   if (method == ORC_TV) {
     orc_grad((const double *)x, W, H, T, d6[0], d6[1], d6[2], (double *)out);
     return;
@@ -450,6 +494,10 @@ static void reg_K(int method, const dc *x, int W, int H, int T, const double *d6
 static void reg_KT(int method, const dc *y, int W, int H, int T, const double *d6,
                    dc *out) {
   long N = (long)W * H * T;
+  if (method == ORC_TVTEMP) {  /* [-div_t y] (tv_temp.cpp:145-147) */
+    tgrad_T(y, W, H, T, d6[2], out);
+    return;
+  }
   orc_div((const double *)y, W, H, T, d6[0], d6[1], d6[2], (double *)out);
   if (method == ORC_TV) {
     for (long i = 0; i < N; i++) out[i] = -out[i];
@@ -474,7 +522,7 @@ static void reg_KT(int method, const dc *y, int W, int H, int T, const double *d
  * utils.cpp:1398-1418): 2 on the off-diagonal symmetric-gradient entries. */
 static double dual_weight(int method, int comp) {
   int c = comp;
-  if (method == ORC_TV) return 1.0;
+  if (method == ORC_TV || method == ORC_TVTEMP) return 1.0;
   if (method == ORC_ICTGV2 && c >= 9) c -= 9;
   return (c >= 6) ? 2.0 : 1.0;
 }
@@ -483,6 +531,13 @@ static double dual_weight(int method, int comp) {
  * TV uses radius 1 (tv.cpp:150). */
 static void prox_all(int method, dc *y, long N, double alpha0, double alpha1,
                      double alpha) {
+  if (method == ORC_TVTEMP) {  /* ProximalMap1D(y, 1) (tv_temp.cpp:138) */
+    for (long i = 0; i < N; i++) {
+      double a = cabs(y[i]);
+      if (a > 1.0) y[i] /= a;
+    }
+    return;
+  }
   if (method == ORC_TV) { orc_prox3((double *)y, N, 1.0); return; }
   if (method == ORC_TGV2) {
     orc_prox3((double *)y, N, alpha1);
@@ -647,10 +702,15 @@ double orc_gstar(int method, const double *x1r, const double *yr,
   dc *img = malloc(sizeof(dc) * N), *dv = malloc(sizeof(dc) * N);
   dc *sd = malloc(sizeof(dc) * 3 * N);
   orc_cart_KH(zr, b1, mask, W, H, C, T, centered, (double *)img);
-  orc_div(yr, W, H, T, d6[0], d6[1], d6[2], (double *)dv);
+  if (method == ORC_TVTEMP) {  /* K^H z - div_t y (tv_temp.cpp:205-208) */
+    tgrad_T(y, W, H, T, d6[2], dv);
+    for (long i = 0; i < N; i++) dv[i] = -dv[i];
+  } else {
+    orc_div(yr, W, H, T, d6[0], d6[1], d6[2], (double *)dv);
+  }
   for (long i = 0; i < N; i++) img[i] -= dv[i];
   double g = g1 + g2 + l1(img, N);
-  if (method != ORC_TV) {
+  if (method != ORC_TV && method != ORC_TVTEMP) {
     orc_symdiv((const double *)(y + 3 * N), W, H, T, d6[0], d6[1], d6[2], (double *)sd);
     for (long i = 0; i < 3 * N; i++) sd[i] += y[i];
     g += l1(sd, N) + l1(sd + N, N) + l1(sd + 2 * N, N);
@@ -681,7 +741,9 @@ double orc_pdgap(int method, const double *xr, const double *yr,
   double gs = orc_gstar(method, xr, yr, zr, data, b1, mask, W, H, C, T,
                         centered, lambda, d6);
   double r;
-  if (method == ORC_TV)
+  if (method == ORC_TVTEMP)
+    r = orc_tvtemp_norm(xr, W, H, T, d6[2]);
+  else if (method == ORC_TV)
     r = orc_tv_norm(xr, W, H, T, d6[0], d6[1], d6[2]);
   else if (method == ORC_TGV2)
     r = orc_tgv2_norm(xr, (const double *)(x + N), W, H, T, alpha0, alpha1,
@@ -715,8 +777,13 @@ static int recon_core(const orc_params *p, const orc_op *op, const double *datar
   long N = (long)W * H * T, CN = op->kn;
   int P = orc_primal_components(method), D = orc_dual_components(method);
   double d6[6] = {1, 1, 1, 1, 1, 1};
-  if (orc_time_space_weights(p->tsw, &d6[0], &d6[2]) != 0) return -1;
-  d6[1] = d6[0];
+  if (method == ORC_TVTEMP) {  /* dt as given: no time/space weights (tv_temp.cpp:98) */
+    if (!(p->dt > 0.0)) return -1;
+    d6[2] = p->dt;
+  } else {
+    if (orc_time_space_weights(p->tsw, &d6[0], &d6[2]) != 0) return -1;
+    d6[1] = d6[0];
+  }
   if (method == ORC_ICTGV2) {
     if (orc_time_space_weights(p->tsw2, &d6[3], &d6[5]) != 0) return -1;
     d6[4] = d6[3];

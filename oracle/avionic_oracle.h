@@ -27,10 +27,10 @@ extern "C" {
 #endif
 
 /* Method identifiers, same numbering as the product C-ABI. */
-enum { ORC_TV = 0, ORC_TGV2 = 1, ORC_ICTGV2 = 2 };
+enum { ORC_TV = 0, ORC_TGV2 = 1, ORC_ICTGV2 = 2, ORC_TVTEMP = 3 };
 
 typedef struct orc_params {
-  int method;            /* ORC_TV / ORC_TGV2 / ORC_ICTGV2 */
+  int method;            /* ORC_TV / ORC_TGV2 / ORC_ICTGV2 / ORC_TVTEMP */
   int maxIt;             /* PDParams::maxIt (pd_recon.h:37) */
   double sigma, tau;     /* initial step sizes (tv.cpp:44-45) */
   double sigmaTauRatio;  /* pd_recon.h:64 */
@@ -44,6 +44,8 @@ typedef struct orc_params {
   int gapEvery;          /* >0: record the normalised PD gap every gapEvery
                             iterations (tv.cpp:180-193, debugstep) */
   int centered;          /* 1: centred FFTs (cartesian_operator.cpp:90-99) */
+  double dt;             /* TVTEMP's temporal step, used as given (tv_temp.cpp
+                            :49, tvtemp.dt in options_parser.cpp:206) */
 } orc_params;
 
 /* ---- FFT (unnormalised, sign -1 = e^{-i...}, +1 = e^{+i...}) ---- */
@@ -77,6 +79,10 @@ void orc_adapt_step(double nKx, double nx, double ratio, double *sigma,
                     double *tau);
 
 /* ---- functionals ---- */
+/* TVTEMP: temporal forward difference / 1-norm of it (tv_temp.cpp:130-147, 217-219) */
+void orc_tgrad(const double *x, int W, int H, int T, double dt, double *out);
+void orc_tgrad_T(const double *y, int W, int H, int T, double dt, double *out);
+double orc_tvtemp_norm(const double *u, int W, int H, int T, double dt);
 double orc_tv_norm(const double *u, int W, int H, int T, double dx, double dy,
                    double dt);
 double orc_tgv2_norm(const double *u, const double *v, int W, int H, int T,

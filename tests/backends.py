@@ -156,7 +156,7 @@ class GpuBackend:
     def _solver(self, method, dims, C, mask, centered, params=None):
         T, H, W = dims
         op = self.A.CartesianOperator(W, H, C, T, self._r(mask), bool(centered))
-        cls = {0: self.A.TV, 1: self.A.TGV2, 2: self.A.ICTGV2}[method]
+        cls = {0: self.A.TV, 1: self.A.TGV2, 2: self.A.ICTGV2, 3: self.A.TVTEMP}[method]
         return cls(W, H, C, T, op, params)
 
     def solver_adapt_step(self, method, delta, b1, mask, centered, d6, ratio, sigma, tau):
@@ -229,6 +229,8 @@ class GpuBackend:
                  timeSpaceWeight2=params["tsw2"], alpha0=params["alpha0"],
                  alpha1=params["alpha1"], alpha=params["alpha"],
                  adaptSchedule=int(params.get("schedule", 0)))
+        if method == 3:
+            p["dt"] = params["dt"]
         p["lambda"] = params["lambda_"]
         if "dualStorage" in params:
             p["dualStorage"] = int(params["dualStorage"])

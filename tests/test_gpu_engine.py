@@ -35,6 +35,7 @@ CASES = [  # method, T, H, W, C, maxIt, dualStorage
     (1, 5, 16, 16, 2, 70, 0),
     (2, 3, 20, 28, 3, 30, 1),
     (2, 2, 32, 32, 5, 55, 0),
+    (3, 4, 24, 24, 3, 50, 0),  # temporal TV, batched with the TV request
 ]
 
 
@@ -51,9 +52,11 @@ def test_engine_mixed_batch_matches_alone_and_oracle(gpu):
     for i, (m, T, H, W, C, it, dual) in enumerate(CASES):
         data, b1, mask = _inputs(T, H, W, C, seed=i)
         p = _params(m, it, dual, mask, W, H, T)
+        op = O.default_params(m)
+        if m == 3:
+            p["dt"] = op["dt"] = 0.8
         reqs.append(A.Request(m, _dev(torch, data), _dev(torch, b1),
                               mask=_dev(torch, mask, np.float32), params=p))
-        op = O.default_params(m)
         op.update(maxIt=it, lambda_=p["lambda"])
         refs.append((O.recon(op, data, b1, mask), dual))
     # alone, eager

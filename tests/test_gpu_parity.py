@@ -85,7 +85,7 @@ def _problem(T=4, H=32, W=32, C=4, seed=3):
     return data, b1, mask
 
 
-@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2"])
+@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2", "tvtemp"])
 @pytest.mark.parametrize("schedule", [0, 1])
 def test_recon_loop_matches_oracle(gpu, method, schedule):
     print(f"[gpu recon] {method} schedule={schedule}", flush=True)
@@ -94,6 +94,8 @@ def test_recon_loop_matches_oracle(gpu, method, schedule):
     T, C, H, W = data.shape
     k, d = O.adapt_lambda_defaults(method)
     p.update(maxIt=200, schedule=schedule, lambda_=O.adapt_lambda_cart(mask, W, H, T, k, d))
+    if method == "tvtemp":
+        p.update(dt=0.7)  # used as given, no time/space weights
     ref = O.recon(p, data, b1, mask, logs=True)
     got = gpu.recon_logs(p, data, b1, mask, None)
     assert rel(got["x"], ref["x"]) < 1e-4
@@ -115,7 +117,7 @@ def test_recon_bf16_duals_runs(gpu):
     assert err < 5e-2
 
 
-@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2"])
+@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2", "tvtemp"])
 def test_recon_q16_duals_match_oracle(gpu, method):
     """16-bit fixed-point duals scaled by the projection radius (dualStorage
     2): 4 bytes per complex dual like bf16, with an absolute resolution of
