@@ -28,7 +28,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libavionic.so")
 
-TV_METHOD, TGV2_METHOD, ICTGV2_METHOD, TVTEMP_METHOD = 0, 1, 2, 3
+TV_METHOD, TGV2_METHOD, ICTGV2_METHOD, TVTEMP_METHOD, ICTV_METHOD = 0, 1, 2, 3, 4
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -396,6 +396,11 @@ class TVTEMP(PDRecon):
     """TVTEMP, temporal TV (CUDA/include/tv_temp.h): parameter dt is used as
     given; dx, dy and timeSpaceWeight do not apply."""
     METHOD = TVTEMP_METHOD
+
+
+class ICTV(PDRecon):
+    """ICTV, infimal convolution of two TV terms (CUDA/include/ictv.h)."""
+    METHOD = ICTV_METHOD
 
 
 def default_params(method) -> dict:

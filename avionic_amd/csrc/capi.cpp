@@ -137,7 +137,7 @@ size_t avionic_operator_kspace_size(const avionic_operator *op) {
 int avionic_default_params(avionic_method method, avionic_params *params) {
   return guard([&] {
     NEED(params);
-    AV_CHECK(method >= 0 && method <= AVIONIC_TVTEMP, "unknown method");
+    AV_CHECK(method >= 0 && method <= AVIONIC_ICTV, "unknown method");
     Recon::defaults(kernel_method((int)method), params);
   });
 }
@@ -255,12 +255,12 @@ int avionic_recon_export(avionic_recon *recon, avionic_export_cb cb, void *user)
       }
       cb(name, tmp.data(), v.size(), user);
     };
-    if (r->method == kICTGV2 && !r->x3_host.empty())
+    if ((r->method == kICTGV2 || r->method == kICTV) && !r->x3_host.empty())
       cb("x3_component", reinterpret_cast<const float *>(r->x3_host.data()), r->x3_host.size(), user);
     if (r->debug) {
       emit("PDGap", r->pdgap_log);
-      if (r->method == kICTGV2) {
-        emit("ICTGVnorm", r->norm_log);
+      if (r->method == kICTGV2 || r->method == kICTV) {  // ictv.cpp:370-388
+        emit(r->method == kICTV ? "ictvnorm" : "ICTGVnorm", r->norm_log);
         emit("DATAfidelity", r->fid_log);
       }
     }

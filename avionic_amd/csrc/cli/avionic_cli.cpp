@@ -4,7 +4,7 @@
 //
 //   avionic [options] -d w:h:d:nRO:nEnc1:nEnc2:c:f <kdata.bin> <mask|traj.bin> <output.bin>
 //
-// Supported: methods TV, TVTEMP, TGV2, ICTGV2; Cartesian and (-n) non-Cartesian data;
+// Supported: methods TV, TVTEMP, TGV2, ICTV, ICTGV2; Cartesian and (-n) non-Cartesian data;
 // coil maps (-s) and initial image (-u); the .cfg parameter file (-p) with
 // the reference's keys; -i maxIt, -j stopPDGap, -a adaptlambda, -w density,
 // -e extra results, -v verbose, -g debugstep, -o data normalisation.  Not in
@@ -122,7 +122,7 @@ Options parse_args(int argc, char **argv) {
     if (a == "-h" || a == "--help") {
       std::printf(
           "Usage: avionic [options] -d w:h:d:nRO:nEnc1:nEnc2:c:f <kdata> <mask/traj> <output>\n"
-          "  -m, --method TV|TVTEMP|TGV2|ICTGV2  -i, --maxIt N  -j, --stopPDGap G\n"
+          "  -m, --method TV|TVTEMP|TGV2|ICTV|ICTGV2  -i, --maxIt N  -j, --stopPDGap G\n"
           "  -p, --params file.cfg         -s, --sens b1.bin -u, --uzero u0.bin\n"
           "  -n, --nonuniform  -w, --dens dcf.bin  -a, --adaptlambda  -e, --extradata\n"
           "  -v, --verbose     -g, --debugstep N   -o, --normalize   --dry-run\n");
@@ -184,7 +184,8 @@ avionic_method method_of(const std::string &m) {
   if (m == "TGV2") return AVIONIC_TGV2;
   if (m == "ICTGV2") return AVIONIC_ICTGV2;
   if (m == "TVTEMP") return AVIONIC_TVTEMP;
-  die("method " + m + " is not part of this build (TV, TVTEMP, TGV2, ICTGV2)", 3);
+  if (m == "ICTV") return AVIONIC_ICTV;
+  die("method " + m + " is not part of this build (TV, TVTEMP, TGV2, ICTV, ICTGV2)", 3);
 }
 
 // Config section keys of one method onto the parameter struct
@@ -383,6 +384,7 @@ int main(int argc, char **argv) {
   const char *sec = method == AVIONIC_TV       ? "tv"
                     : method == AVIONIC_TVTEMP ? "tvtemp"
                     : method == AVIONIC_TGV2   ? "tgv2"
+                    : method == AVIONIC_ICTV   ? "ictv"
                                                : "ictgv2";
   apply_cfg(o, sec, p);
   if (!p.adaptLambda && !(p.lambda > 0.0))

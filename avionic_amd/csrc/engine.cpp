@@ -191,7 +191,7 @@ avionic_params request_params(const avionic_request_desc *d, const Operator &op)
     p = *d->params;
     if (p.adaptLambda) p.lambda = op.adapt_lambda(p.adaptLambdaK, p.adaptLambdaD);
   } else {
-    Recon::defaults((int)d->method, &p);
+    Recon::defaults(kernel_method((int)d->method), &p);
     p.lambda = op.adapt_lambda(p.adaptLambdaK, p.adaptLambdaD);
   }
   double ds, dt;
@@ -204,7 +204,7 @@ avionic_params request_params(const avionic_request_desc *d, const Operator &op)
     p.dx = p.dy = ds;
     p.dt = dt;
   }
-  if (d->method == AVIONIC_ICTGV2) {
+  if (d->method == AVIONIC_ICTGV2 || d->method == AVIONIC_ICTV) {
     AV_CHECK(time_space_weights(p.timeSpaceWeight2, ds, dt), "ComputeTimeSpaceWeights: Invalid ratio.");
     p.dx2 = p.dy2 = ds;
     p.dt2 = dt;
@@ -216,7 +216,7 @@ avionic_params request_params(const avionic_request_desc *d, const Operator &op)
 
 void check_desc(const avionic_request_desc *d) {
   AV_CHECK(d != nullptr, "request descriptor is null");
-  AV_CHECK(d->method >= 0 && d->method <= AVIONIC_TVTEMP, "unknown method");
+  AV_CHECK(d->method >= 0 && d->method <= AVIONIC_ICTV, "unknown method");
   AV_CHECK(d->width > 0 && d->height > 0 && d->coils > 0 && d->frames > 0, "empty request");
   AV_CHECK(d->data != nullptr && d->b1 != nullptr, "request data / b1 missing");
   check_grid(Grid3{(int)d->width, (int)d->height, (int)d->frames,

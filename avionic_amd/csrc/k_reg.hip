@@ -91,6 +91,11 @@ __global__ __launch_bounds__(RT) void k_reg_norm(Grid3 g, RegParams rp, double a
     if (M == kTV) {
       grad3(X, 0, v, g, d1, gr);
       acc = norm3f(gr);
+    } else if (M == kICTV) {  // utils::ICTVNorm (utils.cpp:1221-1243)
+      grad3(DiffAcc{x, x + N}, 0, v, g, d1, gr);
+      const double t1 = norm3f(gr);
+      grad3(X, N, v, g, d2, gr);
+      acc = a1 * (wa * t1 + wb * (double)norm3f(gr));
     } else {
       if (M == kTGV2) {
         grad3(X, 0, v, g, d1, gr);
@@ -132,7 +137,10 @@ __global__ __launch_bounds__(RT) void k_gstar_terms(Grid3 g, RegParams rp,
     PlainAcc Y{y};
     const float2 dv1 = div3(Y, 0, N, 2 * N, v, g, d1);
     acc += sqrt((double)cabs2(csub(khz[i], dv1)));
-    if (M != kTV) {
+    if (M == kICTV) {  // ictv.cpp:178-183: + |div1 y1 - div2 y3|
+      const float2 dv3 = div3(Y, 3 * N, 4 * N, 5 * N, v, g, d2);
+      acc += sqrt((double)cabs2(csub(dv1, dv3)));
+    } else if (M != kTV) {
       float2 sd[3];
       symdiv3(Y, 3 * N, N, v, g, d1, sd);
       for (int c = 0; c < 3; ++c) acc += sqrt((double)cabs2(cadd(sd[c], y[c * N + i])));
@@ -245,7 +253,8 @@ void launch_dual(int method, int ds, const Grid3 &g, const RegParams &rp, const 
   do {                                       \
     if (method == kTV) AV_DUAL(kTV, DT);     \
     else if (method == kTGV2) AV_DUAL(kTGV2, DT); \
-    else AV_DUAL(kICTGV2, DT);               \
+    else if (method == kICTGV2) AV_DUAL(kICTGV2, DT);  \
+    else AV_DUAL(kICTV, DT);                           \
   } while (0)
   if (ds == kDualF32) AV_DUAL_M(float2);
   else if (ds == kDualBF16) AV_DUAL_M(bf16x2);
@@ -265,7 +274,8 @@ void launch_primal(int method, int ds, const Grid3 &g, const RegParams &rp, floa
   do {                                           \
     if (method == kTV) AV_PRIMAL(kTV, DT);       \
     else if (method == kTGV2) AV_PRIMAL(kTGV2, DT); \
-    else AV_PRIMAL(kICTGV2, DT);                 \
+    else if (method == kICTGV2) AV_PRIMAL(kICTGV2, DT);  \
+    else AV_PRIMAL(kICTV, DT);                           \
   } while (0)
   if (ds == kDualF32) AV_PRIMAL_M(float2);
   else if (ds == kDualBF16) AV_PRIMAL_M(bf16x2);
@@ -284,7 +294,8 @@ void launch_step_reg(int method, const Grid3 &g, const RegParams &rp, const floa
   dim3 grid((unsigned)step_blocks(g)), block(RT);
   if (method == kTV) hipLaunchKernelGGL(k_step_reg<kTV>, grid, block, 0, s, g, rp, ext, x, partial);
   else if (method == kTGV2) hipLaunchKernelGGL(k_step_reg<kTGV2>, grid, block, 0, s, g, rp, ext, x, partial);
-  else hipLaunchKernelGGL(k_step_reg<kICTGV2>, grid, block, 0, s, g, rp, ext, x, partial);
+  else if (method == kICTGV2) hipLaunchKernelGGL(k_step_reg<kICTGV2>, grid, block, 0, s, g, rp, ext, x, partial);
+  else hipLaunchKernelGGL(k_step_reg<kICTV>, grid, block, 0, s, g, rp, ext, x, partial);
   AV_LAUNCH_CHECK();
 }
 
@@ -306,14 +317,15 @@ void launch_reg_norm(int method, const Grid3 &g, const RegParams &rp, double alp
   check_grid(g);
   dim3 grid(nblk(g.N)), block(RT);
   double wa = 1.0, wb = 1.0;
-  if (method == kICTGV2) {
+  if (method == kICTGV2 || method == kICTV) {
     const double m = std::min(alpha, 1.0 - alpha);
     wa = alpha / m;
     wb = (1.0 - alpha) / m;
   }
   if (method == kTV) hipLaunchKernelGGL(k_reg_norm<kTV>, grid, block, 0, s, g, rp, alpha0, alpha1, wa, wb, x, partial);
   else if (method == kTGV2) hipLaunchKernelGGL(k_reg_norm<kTGV2>, grid, block, 0, s, g, rp, alpha0, alpha1, wa, wb, x, partial);
-  else hipLaunchKernelGGL(k_reg_norm<kICTGV2>, grid, block, 0, s, g, rp, alpha0, alpha1, wa, wb, x, partial);
+  else if (method == kICTGV2) hipLaunchKernelGGL(k_reg_norm<kICTGV2>, grid, block, 0, s, g, rp, alpha0, alpha1, wa, wb, x, partial);
+  else hipLaunchKernelGGL(k_reg_norm<kICTV>, grid, block, 0, s, g, rp, alpha0, alpha1, wa, wb, x, partial);
   AV_LAUNCH_CHECK();
 }
 
@@ -323,7 +335,8 @@ void launch_gstar_terms(int method, const Grid3 &g, const RegParams &rp, const f
   dim3 grid(nblk(g.N)), block(RT);
   if (method == kTV) hipLaunchKernelGGL(k_gstar_terms<kTV>, grid, block, 0, s, g, rp, khz, y, partial);
   else if (method == kTGV2) hipLaunchKernelGGL(k_gstar_terms<kTGV2>, grid, block, 0, s, g, rp, khz, y, partial);
-  else hipLaunchKernelGGL(k_gstar_terms<kICTGV2>, grid, block, 0, s, g, rp, khz, y, partial);
+  else if (method == kICTGV2) hipLaunchKernelGGL(k_gstar_terms<kICTGV2>, grid, block, 0, s, g, rp, khz, y, partial);
+  else hipLaunchKernelGGL(k_gstar_terms<kICTV>, grid, block, 0, s, g, rp, khz, y, partial);
   AV_LAUNCH_CHECK();
 }
 

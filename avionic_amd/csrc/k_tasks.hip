@@ -307,15 +307,18 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
     if (ds == kDualF32) {                                                            \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
-      else hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);                   \
+      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTV, float2>), __VA_ARGS__);                           \
     } else if (ds == kDualBF16) {                                                    \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
-      else hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);                   \
+      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTV, bf16x2>), __VA_ARGS__);                           \
     } else {                                                                         \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, q16x2>), __VA_ARGS__);          \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, q16x2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);                    \
+      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTV, q16x2>), __VA_ARGS__);                           \
     }                                                                                \
   } while (0)
   switch (id) {
@@ -328,7 +331,8 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
       if (gr.vox) {
         if (method == kTV) hipLaunchKernelGGL(kt_step_reg<kTV>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
         else if (method == kTGV2) hipLaunchKernelGGL(kt_step_reg<kTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
-        else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+        else if (method == kICTGV2) hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+        else hipLaunchKernelGGL(kt_step_reg<kICTV>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
       }
       break;
     case kKColFwdDelta: launch_col_fwd<true>(gr, d_tasks, n, s); break;
@@ -362,15 +366,18 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
     if (ds == kDualF32) {                                                            \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
-      else hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);                   \
+      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTV, float2>), __VA_ARGS__);                           \
     } else if (ds == kDualBF16) {                                                    \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
-      else hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);                   \
+      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTV, bf16x2>), __VA_ARGS__);                           \
     } else {                                                                         \
       if (method == kTV) hipLaunchKernelGGL((KER<kTV, q16x2>), __VA_ARGS__);          \
       else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, q16x2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);                    \
+      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);  \
+      else hipLaunchKernelGGL((KER<kICTV, q16x2>), __VA_ARGS__);                           \
     }                                                                                \
   } while (0)
   // dual update (reads ext, writes y) beside the data-term chain (reads ext,
@@ -405,7 +412,8 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
     if (gr.vox > 0) {
       if (method == kTV) hipLaunchKernelGGL(kt_step_reg<kTV>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
       else if (method == kTGV2) hipLaunchKernelGGL(kt_step_reg<kTGV2>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
-      else hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
+      else if (method == kICTGV2) hipLaunchKernelGGL(kt_step_reg<kICTGV2>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
+      else hipLaunchKernelGGL(kt_step_reg<kICTV>, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
     }
     AV_LAUNCH_CHECK();
     launch_col_fwd<true>(gr, d_tasks, n, s);

@@ -8,16 +8,24 @@
 
 namespace avionic {
 
-enum Method { kTV = 0, kTGV2 = 1, kICTGV2 = 2 };
-// Temporal TV (tv_temp.cpp) runs on the TV kernels with zero spatial weights
-// (1/dx = 1/dy = 0): its spatial dual components then stay exactly zero, so
-// the norm_3 projection, the divergence and the step-rule sums equal the
-// one-component temporal ones of the reference bit for bit.
-constexpr int kTVTEMP = 3;
-inline int kernel_method(int m) { return m == kTVTEMP ? kTV : m; }
+// Kernel methods.  ICTV (ictv.cpp) keeps u, w and the duals y1(3), y3(3).
+enum Method { kTV = 0, kTGV2 = 1, kICTGV2 = 2, kICTV = 3 };
+// Method numbers of the C-ABI (avionic_method) that differ from the kernel
+// method: temporal TV (tv_temp.cpp) runs on the TV kernels with zero spatial
+// weights (1/dx = 1/dy = 0).  Its spatial dual components then stay exactly
+// zero, so the norm_3 projection, the divergence and the step-rule sums equal
+// the one-component temporal ones of the reference bit for bit.
+constexpr int kApiTVTEMP = 3, kApiICTV = 4;
+inline int kernel_method(int api) {
+  return api == kApiTVTEMP ? kTV : api == kApiICTV ? kICTV : api;
+}
 
-inline int primal_components(int m) { return m == kTV ? 1 : m == kTGV2 ? 4 : 8; }
-inline int dual_components(int m) { return m == kTV ? 3 : m == kTGV2 ? 9 : 18; }
+inline int primal_components(int m) {
+  return m == kTV ? 1 : m == kTGV2 ? 4 : m == kICTV ? 2 : 8;
+}
+inline int dual_components(int m) {
+  return m == kTV ? 3 : m == kTGV2 ? 9 : m == kICTV ? 6 : 18;
+}
 
 struct Grid3 {
   int W, H, T;

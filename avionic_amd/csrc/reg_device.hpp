@@ -230,6 +230,13 @@ __device__ inline void dual_voxel(const Grid3 &g, const RegParams &rp, const flo
     ascend(0, 3, gr, rp.r1);
     return;
   }
+  if (M == kICTV) {  // ictv.cpp:268-290: y1 <- grad1(u - w), y3 <- grad2 w
+    grad3(DiffAcc{ext, ext + N}, 0, v, g, d1, gr);
+    ascend(0, 3, gr, rp.r1);
+    grad3(E, N, v, g, d2, gr);
+    ascend(3, 3, gr, rp.r3);
+    return;
+  }
   if (M == kTGV2) {
     grad3(E, 0, v, g, d1, gr);
   } else {
@@ -264,7 +271,11 @@ __device__ inline void primal_voxel(const Grid3 &g, const RegParams &rp, float2 
   const float2 dv1 = div3(Y1, 0, N, 2 * N, v, g, d1);
   upd[0] = csub(gld(x, i), cscale(csub(gld(khz, i), dv1), tau));
   int P = 1;
-  if (M != kTV) {
+  if (M == kICTV) {  // ictv.cpp:301-310: w <- w - tau (div1 y1 - div2 y3)
+    const float2 dv3 = div3(Y3, 3 * N, 4 * N, 5 * N, v, g, d2);
+    upd[1] = csub(gld(x + N, i), cscale(csub(dv1, dv3), tau));
+    P = 2;
+  } else if (M != kTV) {
     float2 sd[3];
     symdiv3(Y2, 3 * N, N, v, g, d1, sd);
 #pragma unroll
@@ -303,8 +314,11 @@ __device__ inline void step_voxel(const Grid3 &g, const RegParams &rp, const flo
   const Inv3 d1{rp.idx, rp.idy, rp.idt}, d2{rp.idx2, rp.idy2, rp.idt2};
   const DiffAcc D{ext, x};
   float2 gr[3], e[6];
-  const int P = (M == kTV) ? 1 : (M == kTGV2 ? 4 : 8);
+  const int P = (M == kTV) ? 1 : (M == kTGV2 ? 4 : (M == kICTV ? 2 : 8));
   for (int c = 0; c < P; ++c) ax += (double)cabs2(D.at(c * N)(i));
+  // ICTV's AdaptStepSize (ictv.cpp:96-131) overwrites its regulariser sum
+  // before use: nKx holds the data term alone, so nothing is added here.
+  if (M == kICTV) return;
   if (M == kTV) {
     grad3(D, 0, v, g, d1, gr);
     for (int c = 0; c < 3; ++c) ak += (double)cabs2(gr[c]);

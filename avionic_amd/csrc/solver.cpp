@@ -125,8 +125,8 @@ bool time_space_weights(double tsw, double &ds, double &dt) {
   return true;
 }
 
-// InitParams: tv.cpp:39-57, tgv2.cpp:36-56, ictgv2.cpp:37-64; TVTEMP
-// (tv_temp.cpp:39-54) sets the TV values it uses (dt = 1).
+// InitParams: tv.cpp:39-57, tgv2.cpp:36-56, ictgv2.cpp:37-64, ictv.cpp:37-62
+// TVTEMP (tv_temp.cpp:39-54) sets the TV values it uses (dt = 1).
 void Recon::defaults(int method, avionic_params *p) {
   *p = avionic_params{};
   p->maxIt = 500;
@@ -153,12 +153,20 @@ void Recon::defaults(int method, avionic_params *p) {
     p->adaptLambdaK = 0.4 / (5.1191 * 0.35);
     p->adaptLambdaD = 10.0 / (5.1191 * 0.35);
   }
+  if (method == kICTV) {
+    p->timeSpaceWeight = 4.0;
+    p->timeSpaceWeight2 = 0.5;
+    p->alpha = 0.5;
+    p->alpha1 = 1.0;
+    p->adaptLambdaK = 0.4 / (5.1191 * 0.35);
+    p->adaptLambdaD = 10.0 / (5.1191 * 0.35);
+  }
 }
 
 Recon::Recon(int method_, int W_, int H_, int C_, int T_, const avionic_params *p, Operator *op_)
-    : method(kernel_method(method_)), temporal(method_ == kTVTEMP), W(W_), H(H_), C(C_), T(T_),
+    : method(kernel_method(method_)), temporal(method_ == kApiTVTEMP), W(W_), H(H_), C(C_), T(T_),
       N((long)W_ * H_ * T_), op(op_) {
-  AV_CHECK(method_ >= 0 && method_ <= kTVTEMP, "unknown method");
+  AV_CHECK(method_ >= 0 && method_ <= kApiICTV, "unknown method");
   AV_CHECK(op != nullptr, "operator is null");
   AV_CHECK(op->W == W && op->H == H && op->C == C && op->T == T,
            "operator dimensions do not match the solver");
@@ -293,7 +301,7 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
     params.dx = params.dy = ds;
     params.dt = dt;
   }
-  if (method == kICTGV2) {
+  if (method == kICTGV2 || method == kICTV) {
     AV_CHECK(time_space_weights(params.timeSpaceWeight2, ds, dt),
              "ComputeTimeSpaceWeights: Invalid ratio.");
     params.dx2 = params.dy2 = ds;
@@ -403,7 +411,7 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
       const double pg = std::fabs(gstar + r) / (double)N;
       pdgap_log.push_back(pg);
       if (verbose) std::printf("Normalized Primal-Dual Gap after %u iterations: %.4e\n", k, pg);
-      if (method == kICTGV2) {
+      if (method == kICTGV2 || method == kICTV) {  // ictv.cpp:347-355 too
         norm_log.push_back(r);
         fid_log.push_back(data_fidelity());
       }
@@ -421,9 +429,10 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
   std::vector<double> hs(std::max(1u, maxIt)), ht(std::max(1u, maxIt));
   AV_HIP(hipMemcpyAsync(hs.data(), slog.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
   AV_HIP(hipMemcpyAsync(ht.data(), tlog.p, sizeof(double) * ht.size(), hipMemcpyDeviceToHost, s));
-  if (method == kICTGV2) {
+  if (method == kICTGV2 || method == kICTV) {  // w: component 4 (ICTGV2) or 1 (ICTV)
     x3_host.resize(N);
-    AV_HIP(hipMemcpyAsync(x3_host.data(), xs + 4 * N, sizeof(float2) * N, hipMemcpyDeviceToHost, s));
+    AV_HIP(hipMemcpyAsync(x3_host.data(), xs + (method == kICTV ? 1 : 4) * N, sizeof(float2) * N,
+                          hipMemcpyDeviceToHost, s));
   }
   AV_HIP(hipStreamSynchronize(s));
   sig_log.assign(ran, 0.0);

@@ -48,8 +48,10 @@ typedef enum avionic_method {
   AVIONIC_TV = 0,     /* TV     (CUDA/include/tv.h)     */
   AVIONIC_TGV2 = 1,   /* TGV2   (CUDA/include/tgv2.h)   */
   AVIONIC_ICTGV2 = 2, /* ICTGV2 (CUDA/include/ictgv2.h) */
-  AVIONIC_TVTEMP = 3  /* TVTEMP, temporal TV (CUDA/include/tv_temp.h): dt is
+  AVIONIC_TVTEMP = 3, /* TVTEMP, temporal TV (CUDA/include/tv_temp.h): dt is
                          used as given, dx, dy and timeSpaceWeight are unused */
+  AVIONIC_ICTV = 4    /* ICTV (CUDA/include/ictv.h): u, w; alpha1, alpha,
+                         timeSpaceWeight and timeSpaceWeight2 apply */
 } avionic_method;
 
 /* Parameters: the union of PDParams (pd_recon.h:34-74), TGV2Params

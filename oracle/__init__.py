@@ -21,8 +21,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-TV, TGV2, ICTGV2, TVTEMP = 0, 1, 2, 3
-METHODS = {"tv": TV, "tgv2": TGV2, "ictgv2": ICTGV2, "tvtemp": TVTEMP}
+TV, TGV2, ICTGV2, TVTEMP, ICTV = 0, 1, 2, 3, 4
+METHODS = {"tv": TV, "tgv2": TGV2, "ictgv2": ICTGV2, "tvtemp": TVTEMP, "ictv": ICTV}
 
 
 class OrcParams(ctypes.Structure):
@@ -92,6 +92,8 @@ def _declare(L):
         getattr(L, f).restype = None
     L.orc_tvtemp_norm.argtypes = [_P, _I, _I, _I, _D]
     L.orc_tvtemp_norm.restype = _D
+    L.orc_ictv_norm.argtypes = [_P, _P, _I, _I, _I, _D, _D, _P]
+    L.orc_ictv_norm.restype = _D
     L.orc_tgv2_norm.argtypes = [_P, _P, _I, _I, _I, _D, _D, _D, _D, _D]
     L.orc_tgv2_norm.restype = _D
     L.orc_ictgv2_norm.argtypes = [_P, _P, _P, _P, _I, _I, _I] + [_D] * 9
@@ -265,6 +267,13 @@ def tvtemp_norm(u, dt=1.0):
     return lib().orc_tvtemp_norm(_ptr(u), W, H, T, float(dt))
 
 
+def ictv_norm(u, w, alpha1, alpha, d6):
+    u, w = _c(u), _c(w)
+    T, H, W = u.shape
+    d = np.ascontiguousarray(d6, np.float64)
+    return lib().orc_ictv_norm(_ptr(u), _ptr(w), W, H, T, float(alpha1), float(alpha), _ptr(d))
+
+
 def tv_norm(u, dx, dy, dt):
     u = _c(u)
     T, H, W = u.shape
@@ -313,12 +322,14 @@ def default_params(method):
              alpha=0.5, schedule=0, gapEvery=0, centered=0, dt=1.0)
     if m == ICTGV2:
         p.update(tsw=6.5, tsw2=2.5, alpha=0.6423)
+    if m == ICTV:  # ictv.cpp:37-62
+        p.update(tsw=4.0, tsw2=0.5, alpha=0.5, alpha1=1.0)
     return p
 
 
 def adapt_lambda_defaults(method):
     m = METHODS[method] if isinstance(method, str) else int(method)
-    if m == ICTGV2:
+    if m in (ICTGV2, ICTV):
         return 0.4 / (5.1191 * 0.35), 10.0 / (5.1191 * 0.35)
     return 0.4 * 0.2991, 10.0 * 0.2991
 

@@ -407,10 +407,17 @@ void orc_adapt_step(double nKx, double nx, double ratio, double *sigma,
 /* ======================================================================== */
 
 int orc_primal_components(int method) {
-  return (method == ORC_TV || method == ORC_TVTEMP) ? 1 : method == ORC_TGV2 ? 4 : 8;
+  return (method == ORC_TV || method == ORC_TVTEMP) ? 1
+         : method == ORC_TGV2                      ? 4
+         : method == ORC_ICTV                      ? 2
+                                                   : 8;
 }
 int orc_dual_components(int method) {
-  return method == ORC_TVTEMP ? 1 : method == ORC_TV ? 3 : method == ORC_TGV2 ? 9 : 18;
+  return method == ORC_TVTEMP ? 1
+         : method == ORC_TV   ? 3
+         : method == ORC_TGV2 ? 9
+         : method == ORC_ICTV ? 6
+                              : 18;
 }
 
 /* Temporal forward difference and its adjoint (TVTEMP: utils::Gradient_temp
@@ -465,6 +472,13 @@ static void reg_K(int method, const dc *x, int W, int H, int T, const double *d6
     tgrad(x, W, H, T, d6[2], out);
     return;
   }
+  if (method == ORC_ICTV) {  /* [grad1(u - w), grad2 w] (ictv.cpp:266-271) */
+    const dc *u = x, *w = x + N;
+    for (long i = 0; i < N; i++) tmp[i] = u[i] - w[i];
+    orc_grad((const double *)tmp, W, H, T, d6[0], d6[1], d6[2], (double *)out);
+    orc_grad((const double *)w, W, H, T, d6[3], d6[4], d6[5], (double *)(out + 3 * N));
+    return;
+  }
   if (method == ORC_TV) {
     orc_grad((const double *)x, W, H, T, d6[0], d6[1], d6[2], (double *)out);
     return;
@@ -498,6 +512,15 @@ static void reg_KT(int method, const dc *y, int W, int H, int T, const double *d
     tgrad_T(y, W, H, T, d6[2], out);
     return;
   }
+  if (method == ORC_ICTV) {  /* [-div1 y1, div1 y1 - div2 y3] (ictv.cpp:301-310) */
+    orc_div((const double *)y, W, H, T, d6[0], d6[1], d6[2], (double *)out);
+    orc_div((const double *)(y + 3 * N), W, H, T, d6[3], d6[4], d6[5], (double *)(out + N));
+    for (long i = 0; i < N; i++) {
+      out[N + i] = out[i] - out[N + i];
+      out[i] = -out[i];
+    }
+    return;
+  }
   orc_div((const double *)y, W, H, T, d6[0], d6[1], d6[2], (double *)out);
   if (method == ORC_TV) {
     for (long i = 0; i < N; i++) out[i] = -out[i];
@@ -522,7 +545,7 @@ static void reg_KT(int method, const dc *y, int W, int H, int T, const double *d
  * utils.cpp:1398-1418): 2 on the off-diagonal symmetric-gradient entries. */
 static double dual_weight(int method, int comp) {
   int c = comp;
-  if (method == ORC_TV || method == ORC_TVTEMP) return 1.0;
+  if (method == ORC_TV || method == ORC_TVTEMP || method == ORC_ICTV) return 1.0;
   if (method == ORC_ICTGV2 && c >= 9) c -= 9;
   return (c >= 6) ? 2.0 : 1.0;
 }
@@ -539,6 +562,12 @@ static void prox_all(int method, dc *y, long N, double alpha0, double alpha1,
     return;
   }
   if (method == ORC_TV) { orc_prox3((double *)y, N, 1.0); return; }
+  if (method == ORC_ICTV) {  /* ictv.cpp:284-290 */
+    double m = alpha < 1.0 - alpha ? alpha : 1.0 - alpha;
+    orc_prox3((double *)y, N, alpha1 * (alpha / m));
+    orc_prox3((double *)(y + 3 * N), N, alpha1 * ((1.0 - alpha) / m));
+    return;
+  }
   if (method == ORC_TGV2) {
     orc_prox3((double *)y, N, alpha1);
     orc_prox6((double *)(y + 3 * N), N, alpha0);
@@ -587,7 +616,9 @@ static void adapt_step_op(int method, const dc *delta, const double *b1, const o
   reg_K(method, delta, W, H, T, d6, ky, tmp);
   op_K(op, (const double *)delta, b1, (double *)kz);
   double s = 0.0, sx = 0.0;
-  for (int k = 0; k < D; k++) {
+  /* ICTV's AdaptStepSize (ictv.cpp:96-131) overwrites its regulariser sum
+   * before use: its nKx is the data term alone. */
+  for (int k = 0; k < D && method != ORC_ICTV; k++) {
     double wk = dual_weight(method, k), acc = 0.0;
     for (long i = 0; i < N; i++) acc += abs2(ky[k * N + i]);
     s += wk * acc;
@@ -637,6 +668,19 @@ double orc_tv_norm(const double *u, int W, int H, int T, double dx, double dy,
   double s = sum_norm3(g, N);
   free(g);
   return s;
+}
+
+double orc_ictv_norm(const double *ur, const double *wr, int W, int H, int T, double alpha1,
+                     double alpha, const double *d6) {
+  long N = (long)W * H * T;
+  const dc *u = (const dc *)ur, *w = (const dc *)wr;
+  dc *d = malloc(sizeof(dc) * N);
+  for (long i = 0; i < N; i++) d[i] = u[i] - w[i];
+  double n1 = orc_tv_norm((const double *)d, W, H, T, d6[0], d6[1], d6[2]);
+  double n2 = orc_tv_norm(wr, W, H, T, d6[3], d6[4], d6[5]);
+  free(d);
+  double m = alpha < 1.0 - alpha ? alpha : 1.0 - alpha;
+  return alpha1 * (alpha / m) * n1 + alpha1 * ((1.0 - alpha) / m) * n2;
 }
 
 /* alpha1 sum|grad u - v|_3 + alpha0 sum|E v|_6.  utils.cpp:1179-1219;
@@ -710,7 +754,13 @@ double orc_gstar(int method, const double *x1r, const double *yr,
   }
   for (long i = 0; i < N; i++) img[i] -= dv[i];
   double g = g1 + g2 + l1(img, N);
-  if (method != ORC_TV && method != ORC_TVTEMP) {
+  if (method == ORC_ICTV) {  /* + |div1 y1 - div2 y3|_1 (ictv.cpp:178-183) */
+    dc *dv3 = malloc(sizeof(dc) * N);
+    orc_div((const double *)(y + 3 * N), W, H, T, d6[3], d6[4], d6[5], (double *)dv3);
+    for (long i = 0; i < N; i++) dv3[i] = dv[i] - dv3[i];
+    g += l1(dv3, N);
+    free(dv3);
+  } else if (method != ORC_TV && method != ORC_TVTEMP) {
     orc_symdiv((const double *)(y + 3 * N), W, H, T, d6[0], d6[1], d6[2], (double *)sd);
     for (long i = 0; i < 3 * N; i++) sd[i] += y[i];
     g += l1(sd, N) + l1(sd + N, N) + l1(sd + 2 * N, N);
@@ -743,6 +793,8 @@ double orc_pdgap(int method, const double *xr, const double *yr,
   double r;
   if (method == ORC_TVTEMP)
     r = orc_tvtemp_norm(xr, W, H, T, d6[2]);
+  else if (method == ORC_ICTV)
+    r = orc_ictv_norm(xr, (const double *)(x + N), W, H, T, alpha1, alpha, d6);
   else if (method == ORC_TV)
     r = orc_tv_norm(xr, W, H, T, d6[0], d6[1], d6[2]);
   else if (method == ORC_TGV2)
@@ -784,7 +836,7 @@ static int recon_core(const orc_params *p, const orc_op *op, const double *datar
     if (orc_time_space_weights(p->tsw, &d6[0], &d6[2]) != 0) return -1;
     d6[1] = d6[0];
   }
-  if (method == ORC_ICTGV2) {
+  if (method == ORC_ICTGV2 || method == ORC_ICTV) {
     if (orc_time_space_weights(p->tsw2, &d6[3], &d6[5]) != 0) return -1;
     d6[4] = d6[3];
   }

@@ -27,10 +27,10 @@ extern "C" {
 #endif
 
 /* Method identifiers, same numbering as the product C-ABI. */
-enum { ORC_TV = 0, ORC_TGV2 = 1, ORC_ICTGV2 = 2, ORC_TVTEMP = 3 };
+enum { ORC_TV = 0, ORC_TGV2 = 1, ORC_ICTGV2 = 2, ORC_TVTEMP = 3, ORC_ICTV = 4 };
 
 typedef struct orc_params {
-  int method;            /* ORC_TV / ORC_TGV2 / ORC_ICTGV2 / ORC_TVTEMP */
+  int method;            /* ORC_TV / ORC_TGV2 / ORC_ICTGV2 / ORC_TVTEMP / ORC_ICTV */
   int maxIt;             /* PDParams::maxIt (pd_recon.h:37) */
   double sigma, tau;     /* initial step sizes (tv.cpp:44-45) */
   double sigmaTauRatio;  /* pd_recon.h:64 */
@@ -83,6 +83,10 @@ void orc_adapt_step(double nKx, double nx, double ratio, double *sigma,
 void orc_tgrad(const double *x, int W, int H, int T, double dt, double *out);
 void orc_tgrad_T(const double *y, int W, int H, int T, double dt, double *out);
 double orc_tvtemp_norm(const double *u, int W, int H, int T, double dt);
+/* ICTV: a1 (alpha/m) TV1(u - w) + a1 ((1-alpha)/m) TV2(w), m = min(alpha,
+ * 1-alpha) (utils.cpp:1221-1243) */
+double orc_ictv_norm(const double *u, const double *w, int W, int H, int T, double alpha1,
+                     double alpha, const double *d6);
 double orc_tv_norm(const double *u, int W, int H, int T, double dx, double dy,
                    double dt);
 double orc_tgv2_norm(const double *u, const double *v, int W, int H, int T,

@@ -156,7 +156,8 @@ class GpuBackend:
     def _solver(self, method, dims, C, mask, centered, params=None):
         T, H, W = dims
         op = self.A.CartesianOperator(W, H, C, T, self._r(mask), bool(centered))
-        cls = {0: self.A.TV, 1: self.A.TGV2, 2: self.A.ICTGV2, 3: self.A.TVTEMP}[method]
+        cls = {0: self.A.TV, 1: self.A.TGV2, 2: self.A.ICTGV2, 3: self.A.TVTEMP,
+               4: self.A.ICTV}[method]
         return cls(W, H, C, T, op, params)
 
     def solver_adapt_step(self, method, delta, b1, mask, centered, d6, ratio, sigma, tau):

@@ -101,7 +101,7 @@ def test_cli_rejects_what_it_cannot_do(tmp_path):
     for flag, code in (("-r", 3), ("--bogus", 2)):
         r = _run([flag, "-d", dims] + files, ok=False)
         assert r.returncode == code, (flag, r.stderr)
-    r = _run(["-m", "ICTV", "-d", dims, "-s", str(tmp_path / "b1.bin"), "-p",
+    r = _run(["-m", "TGV2_3D", "-d", dims, "-s", str(tmp_path / "b1.bin"), "-p",
               str(tmp_path / "p.cfg")] + files + ["--dry-run"], ok=False)
     assert r.returncode == 3 and "not part of this build" in r.stderr
     # no lambda and no adaptation

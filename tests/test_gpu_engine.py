@@ -36,6 +36,7 @@ CASES = [  # method, T, H, W, C, maxIt, dualStorage
     (2, 3, 20, 28, 3, 30, 1),
     (2, 2, 32, 32, 5, 55, 0),
     (3, 4, 24, 24, 3, 50, 0),  # temporal TV, batched with the TV request
+    (4, 3, 24, 32, 2, 50, 2),  # ICTV, q16 duals
 ]
 
 
@@ -55,6 +56,8 @@ def test_engine_mixed_batch_matches_alone_and_oracle(gpu):
         op = O.default_params(m)
         if m == 3:
             p["dt"] = op["dt"] = 0.8
+        if m == 4:
+            p["sigma"] = p["tau"] = op["sigma"] = op["tau"] = 0.1
         reqs.append(A.Request(m, _dev(torch, data), _dev(torch, b1),
                               mask=_dev(torch, mask, np.float32), params=p))
         op.update(maxIt=it, lambda_=p["lambda"])

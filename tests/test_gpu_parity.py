@@ -85,7 +85,7 @@ def _problem(T=4, H=32, W=32, C=4, seed=3):
     return data, b1, mask
 
 
-@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2", "tvtemp"])
+@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2", "tvtemp", "ictv"])
 @pytest.mark.parametrize("schedule", [0, 1])
 def test_recon_loop_matches_oracle(gpu, method, schedule):
     print(f"[gpu recon] {method} schedule={schedule}", flush=True)
@@ -96,6 +96,10 @@ def test_recon_loop_matches_oracle(gpu, method, schedule):
     p.update(maxIt=200, schedule=schedule, lambda_=O.adapt_lambda_cart(mask, W, H, T, k, d))
     if method == "tvtemp":
         p.update(dt=0.7)  # used as given, no time/space weights
+    if method == "ictv":
+        # ICTV's step rule never shrinks the steps (its nKx is the data term
+        # alone, ictv.cpp:96-131): start from steps the loop converges with
+        p.update(sigma=0.1, tau=0.1)
     ref = O.recon(p, data, b1, mask, logs=True)
     got = gpu.recon_logs(p, data, b1, mask, None)
     assert rel(got["x"], ref["x"]) < 1e-4
@@ -117,7 +121,7 @@ def test_recon_bf16_duals_runs(gpu):
     assert err < 5e-2
 
 
-@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2", "tvtemp"])
+@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2", "tvtemp", "ictv"])
 def test_recon_q16_duals_match_oracle(gpu, method):
     """16-bit fixed-point duals scaled by the projection radius (dualStorage
     2): 4 bytes per complex dual like bf16, with an absolute resolution of
@@ -128,6 +132,8 @@ def test_recon_q16_duals_match_oracle(gpu, method):
     T, C, H, W = data.shape
     k, d = O.adapt_lambda_defaults(method)
     p.update(maxIt=200, lambda_=O.adapt_lambda_cart(mask, W, H, T, k, d))
+    if method == "ictv":
+        p.update(sigma=0.1, tau=0.1)
     ref = O.recon(p, data, b1, mask)
     got = gpu.recon_logs(dict(p, dualStorage=2), data, b1, mask, None)
     err = rel(got["x"], ref)
