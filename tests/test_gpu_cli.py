@@ -194,3 +194,41 @@ def test_cli_normalization_radial(gpu, tmp_path):
     assert margin > 1e-4, "fixture too close to the selection threshold"
     print(f"[gpu cli] radial datanorm {fac[0].real:.7g} vs oracle {255 * T / med:.7g}", flush=True)
     assert fac[0].real == pytest.approx(255.0 * T / med, rel=1e-4)
+
+
+@pytest.mark.parametrize("method", ["TVTEMP", "ICTV"])
+def test_cli_other_regularisers_match_library(gpu, tmp_path, method):
+    """-m TVTEMP / ICTV with their [tvtemp] / [ictv] sections
+    (options_parser.cpp:204-210, 240-258) equal the library's solver on the
+    same prepared inputs."""
+    rng = np.random.default_rng(13)
+    W, H, C, T = 24, 20, 2, 4
+    k = (rng.standard_normal((T, C, H, W)) + 1j * rng.standard_normal((T, C, H, W))) * 10
+    mask = (rng.random((T, H, 1)) < 0.5) * np.ones((1, 1, W))
+    b1 = rng.standard_normal((C, H, W)) * 0.2 + 1.0 + 0.1j
+    u0 = rng.standard_normal((H, W)) + 0j
+    for name, v in (("k", k), ("m", mask.astype(float)), ("b1", b1), ("u0", u0)):
+        A.write_bin(tmp_path / f"{name}.bin", v)
+    if method == "TVTEMP":
+        cfg, extra = "[tvtemp]\nlambda = 2.0\ndt = 0.8\n", dict(dt=0.8)
+    else:
+        cfg = ("[ictv]\nlambda = 2.0\nsigma = 0.1\ntau = 0.1\ntimeSpaceWeight = 3\n"
+               "timeSpaceWeight2 = 0.7\nalpha = 0.4\nalpha1 = 1.2\n")
+        extra = dict(sigma=0.1, tau=0.1, timeSpaceWeight=3.0, timeSpaceWeight2=0.7, alpha=0.4,
+                     alpha1=1.2)
+    (tmp_path / "p.cfg").write_text(cfg)
+    _run(["-m", method, "-i", "40", "-p", str(tmp_path / "p.cfg"), "-d",
+          f"{W}:{H}:1:{W}:{H}:1:{C}:{T}", "-s", str(tmp_path / "b1.bin"), "-u",
+          str(tmp_path / "u0.bin"), str(tmp_path / "k.bin"), str(tmp_path / "m.bin"),
+          str(tmp_path / "x.bin")])
+    got = A.read_bin(tmp_path / "x.bin").reshape(T, H, W)
+    op = A.CartesianOperator(W, H, C, T, gpu._r(mask), centered=False)
+    rec = getattr(A, method)(W, H, C, T, op, None)
+    p = rec.GetParams()
+    p.update(maxIt=40, adaptLambda=0, **extra)
+    p["lambda"] = 2.0
+    rec.SetParams(p)
+    x = gpu._t(np.repeat(u0[None], T, 0))
+    rec.IterativeReconstruction(gpu._t(k * mask[:, None]), x, gpu._t(b1))
+    ref = gpu._n(x)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
