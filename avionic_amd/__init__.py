@@ -28,7 +28,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libavionic.so")
 
-TV_METHOD, TGV2_METHOD, ICTGV2_METHOD, TVTEMP_METHOD, ICTV_METHOD = 0, 1, 2, 3, 4
+TV_METHOD, TGV2_METHOD, ICTGV2_METHOD, TVTEMP_METHOD, ICTV_METHOD, TGV2_3D_METHOD = 0, 1, 2, 3, 4, 5
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -102,6 +102,7 @@ def _declare(L):
         "avionic_bin_write": (_I, [ctypes.c_char_p, _P, _P, _SZ]),
         "avionic_normalization_factor": (_I, [_P, _SZ, ctypes.POINTER(ctypes.c_float)]),
         "avionic_cartesian_operator_create": (_I, [_U, _U, _U, _U, _P, _I, ctypes.POINTER(_P)]),
+        "avionic_cartesian3d_operator_create": (_I, [_U, _U, _U, _U, _P, ctypes.POINTER(_P)]),
         "avionic_noncartesian_operator_create": (_I, [_U, _U, _U, _U, _U, _U, _P, _P, _I, _D,
                                                       ctypes.POINTER(_P)]),
         "avionic_operator_destroy": (_I, [_P]),
@@ -249,6 +250,22 @@ class CartesianOperator(BaseOperator):
                                                        int(bool(centered)), ctypes.byref(h)))
         super().__init__(h, width, height, coils, frames)
         self.centered = bool(centered)
+
+
+class CartesianOperator3D(BaseOperator):
+    """CartesianOperator3D(width, height, depth, coils[, mask])
+    (CUDA/include/cartesian_operator3d.h): unitary 3D transforms; k-space and
+    coil maps are [coils][depth][height][width], the image [depth][height][width]
+    (``frames`` holds the depth)."""
+
+    def __init__(self, width, height, depth, coils, mask=None):
+        h = ctypes.c_void_p()
+        _check(lib().avionic_cartesian3d_operator_create(width, height, depth, coils, _f32(mask),
+                                                         ctypes.byref(h)))
+        super().__init__(h, width, height, coils, depth)
+
+    def kspace_shape(self):
+        return (self.coils, self.frames, self.height, self.width)
 
 
 class NoncartesianOperator(BaseOperator):
@@ -401,6 +418,12 @@ class TVTEMP(PDRecon):
 class ICTV(PDRecon):
     """ICTV, infimal convolution of two TV terms (CUDA/include/ictv.h)."""
     METHOD = ICTV_METHOD
+
+
+class TGV2_3D(PDRecon):
+    """TGV2_3D (CUDA/include/tgv2_3d.h): TGV2 on a volume with a
+    CartesianOperator3D; ``frames`` is the depth, dx, dy, dz apply."""
+    METHOD = TGV2_3D_METHOD
 
 
 def default_params(method) -> dict:

@@ -12,6 +12,7 @@ namespace avionic {
 double ellipke_host(double m);
 bool time_space_weights(double tsw, double &ds, double &dt);
 class NoncartesianOperator;
+Operator *make_cartesian3d(int W, int H, int D, int C, const float *mask);
 Operator *make_noncartesian(int W, int H, int C, int T, int nFE, int spf, const float *traj,
                             const float *dens, int kw, double osf);
 }  // namespace avionic
@@ -60,6 +61,22 @@ extern "C" {
 
 const char *avionic_last_error(void) { return g_err.c_str(); }
 const char *avionic_version(void) { return "avionic-mi355x 0.1 (gfx950)"; }
+
+int avionic_cartesian3d_operator_create(unsigned width, unsigned height, unsigned depth,
+                                        unsigned coils, const float *mask, avionic_operator **op) {
+  return guard([&] {
+    NEED(op);
+    *op = nullptr;
+    auto *o = new avionic_operator{nullptr};
+    try {
+      o->op = make_cartesian3d((int)width, (int)height, (int)depth, (int)coils, mask);
+    } catch (...) {
+      delete o;
+      throw;
+    }
+    *op = o;
+  });
+}
 
 int avionic_cartesian_operator_create(unsigned width, unsigned height, unsigned coils,
                                       unsigned frames, const float *mask, int centered,
@@ -137,8 +154,9 @@ size_t avionic_operator_kspace_size(const avionic_operator *op) {
 int avionic_default_params(avionic_method method, avionic_params *params) {
   return guard([&] {
     NEED(params);
-    AV_CHECK(method >= 0 && method <= AVIONIC_ICTV, "unknown method");
+    AV_CHECK(method >= 0 && method <= AVIONIC_TGV2_3D, "unknown method");
     Recon::defaults(kernel_method((int)method), params);
+    if (method == AVIONIC_TGV2_3D) params->alpha0 = std::sqrt(3.0);  // tgv2_3d.cpp InitParams
   });
 }
 

@@ -101,5 +101,8 @@ void launch_compact_gather(const CartGeom &g, const float2 *dense, float2 *compa
 // Standalone batched 2D FFT (in place).
 void launch_fft2(float2 *data, const FftPlan &pw, const FftPlan &ph, int H, int W, int batch,
                  int sign, int centered, float scale, hipStream_t s);
+// In-place transform along the slowest axis of `batch` arrays [L][plane].
+void launch_fft_axis0(float2 *data, const FftPlan &p, int L, int plane, int batch, int sign,
+                      float scale, hipStream_t s);
 
 }  // namespace avionic

@@ -290,4 +290,14 @@ void launch_fft2(float2 *data, const FftPlan &pw, const FftPlan &ph, int H, int 
   AV_HIP(hipFreeAsync(tmp, s));
 }
 
+// In-place transform along the slowest axis of `batch` arrays [L][plane]:
+// one length-L transform per column of the plane (the depth axis of a volume).
+void launch_fft_axis0(float2 *data, const FftPlan &p, int L, int plane, int batch, int sign,
+                      float scale, hipStream_t s) {
+  const int XB = pick_xb(L, plane);
+  hipLaunchKernelGGL(k_fft_cols, dim3((plane + XB - 1) / XB, batch), dim3(NT), 0, s, data, p, L,
+                     plane, XB, sign, 0, scale);
+  AV_LAUNCH_CHECK();
+}
+
 }  // namespace avionic

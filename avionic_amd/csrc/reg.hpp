@@ -15,9 +15,10 @@ enum Method { kTV = 0, kTGV2 = 1, kICTGV2 = 2, kICTV = 3 };
 // weights (1/dx = 1/dy = 0).  Its spatial dual components then stay exactly
 // zero, so the norm_3 projection, the divergence and the step-rule sums equal
 // the one-component temporal ones of the reference bit for bit.
-constexpr int kApiTVTEMP = 3, kApiICTV = 4;
+// TGV2_3D (tgv2_3d.cpp) is TGV2 on a (W, H, depth) grid with (dx, dy, dz).
+constexpr int kApiTVTEMP = 3, kApiICTV = 4, kApiTGV2_3D = 5;
 inline int kernel_method(int api) {
-  return api == kApiTVTEMP ? kTV : api == kApiICTV ? kICTV : api;
+  return api == kApiTVTEMP ? kTV : api == kApiICTV ? kICTV : api == kApiTGV2_3D ? kTGV2 : api;
 }
 
 inline int primal_components(int m) {

@@ -164,9 +164,10 @@ void Recon::defaults(int method, avionic_params *p) {
 }
 
 Recon::Recon(int method_, int W_, int H_, int C_, int T_, const avionic_params *p, Operator *op_)
-    : method(kernel_method(method_)), temporal(method_ == kApiTVTEMP), W(W_), H(H_), C(C_), T(T_),
-      N((long)W_ * H_ * T_), op(op_) {
-  AV_CHECK(method_ >= 0 && method_ <= kApiICTV, "unknown method");
+    : method(kernel_method(method_)), temporal(method_ == kApiTVTEMP),
+      volumetric(method_ == kApiTGV2_3D), W(W_), H(H_), C(C_), T(T_), N((long)W_ * H_ * T_),
+      op(op_) {
+  AV_CHECK(method_ >= 0 && method_ <= kApiTGV2_3D, "unknown method");
   AV_CHECK(op != nullptr, "operator is null");
   AV_CHECK(op->W == W && op->H == H && op->C == C && op->T == T,
            "operator dimensions do not match the solver");
@@ -175,6 +176,7 @@ Recon::Recon(int method_, int W_, int H_, int C_, int T_, const avionic_params *
     if (params.adaptLambda) params.lambda = op->adapt_lambda(params.adaptLambdaK, params.adaptLambdaD);
   } else {
     defaults(method, &params);
+    if (volumetric) params.alpha0 = std::sqrt(3.0);  // tgv2_3d.cpp InitParams
     params.lambda = op->adapt_lambda(params.adaptLambdaK, params.adaptLambdaD);  // InitLambda(true)
   }
 }
@@ -182,6 +184,7 @@ Recon::Recon(int method_, int W_, int H_, int C_, int T_, const avionic_params *
 void Recon::d6(double out[6]) const {
   out[0] = params.dx; out[1] = params.dy; out[2] = params.dt;
   if (temporal) out[0] = out[1] = INFINITY;  // 1/dx = 1/dy = 0
+  if (volumetric) out[2] = params.dz;         // third axis: depth
   out[3] = params.dx2; out[4] = params.dy2; out[5] = params.dt2;
 }
 
@@ -295,6 +298,9 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
   if (temporal) {
     // TVTEMP uses dt as configured, no time/space weights (tv_temp.cpp:98)
     AV_CHECK(params.dt > 0.0, "dt must be positive");
+  } else if (volumetric) {
+    // TGV2_3D uses dx, dy, dz as configured (tgv2_3d.cpp loop)
+    AV_CHECK(params.dx > 0.0 && params.dy > 0.0 && params.dz > 0.0, "dx, dy, dz must be positive");
   } else {
     AV_CHECK(time_space_weights(params.timeSpaceWeight, ds, dt),
              "ComputeTimeSpaceWeights: Invalid ratio.");

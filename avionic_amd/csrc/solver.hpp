@@ -103,6 +103,7 @@ class Recon {
   Recon(int method, int W, int H, int C, int T, const avionic_params *params, Operator *op);
   int method;            // kernel method (kTV for temporal TV)
   bool temporal = false;  // TVTEMP: spatial weights zero, dt as given
+  bool volumetric = false;  // TGV2_3D: (dx, dy, dz) as given, third axis = depth
   int W, H, C, T;
   long N;
   Operator *op;

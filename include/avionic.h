@@ -50,8 +50,11 @@ typedef enum avionic_method {
   AVIONIC_ICTGV2 = 2, /* ICTGV2 (CUDA/include/ictgv2.h) */
   AVIONIC_TVTEMP = 3, /* TVTEMP, temporal TV (CUDA/include/tv_temp.h): dt is
                          used as given, dx, dy and timeSpaceWeight are unused */
-  AVIONIC_ICTV = 4    /* ICTV (CUDA/include/ictv.h): u, w; alpha1, alpha,
+  AVIONIC_ICTV = 4,   /* ICTV (CUDA/include/ictv.h): u, w; alpha1, alpha,
                          timeSpaceWeight and timeSpaceWeight2 apply */
+  AVIONIC_TGV2_3D = 5 /* TGV2_3D (CUDA/include/tgv2_3d.h): TGV2 on a volume with the
+                         operator of avionic_cartesian3d_operator_create, "frames" =
+                         depth; dx, dy, dz apply; eager solver only (not the engine) */
 } avionic_method;
 
 /* Parameters: the union of PDParams (pd_recon.h:34-74), TGV2Params
@@ -104,6 +107,15 @@ int avionic_cartesian_operator_create(unsigned width, unsigned height,
                                       unsigned coils, unsigned frames,
                                       const float *mask, int centered,
                                       avionic_operator **op);
+
+/* CartesianOperator3D(width, height, depth, coils, mask)
+ * (cartesian_operator3d.cpp:1-345): unitary 3D transforms over
+ * [depth][height][width]; k-space and coil maps are coil-major
+ * [coils][depth][height][width]; mask [depth][height][width] or NULL.  K^H
+ * (avionic_operator_forward) masks a copy of its input, not the input. */
+int avionic_cartesian3d_operator_create(unsigned width, unsigned height, unsigned depth,
+                                        unsigned coils, const float *mask,
+                                        avionic_operator **op);
 
 /* NoncartesianOperator(width, height, coils, frames, nSpokes, nFE,
  * spokesPerFrame, kTraj, dens, sens) (noncartesian_operator.h,

@@ -21,8 +21,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-TV, TGV2, ICTGV2, TVTEMP, ICTV = 0, 1, 2, 3, 4
-METHODS = {"tv": TV, "tgv2": TGV2, "ictgv2": ICTGV2, "tvtemp": TVTEMP, "ictv": ICTV}
+TV, TGV2, ICTGV2, TVTEMP, ICTV, TGV2_3D = 0, 1, 2, 3, 4, 5
+METHODS = {"tv": TV, "tgv2": TGV2, "ictgv2": ICTGV2, "tvtemp": TVTEMP, "ictv": ICTV,
+           "tgv2_3d": TGV2_3D}
 
 
 class OrcParams(ctypes.Structure):
@@ -42,6 +43,9 @@ class OrcParams(ctypes.Structure):
         ("gapEvery", ctypes.c_int),
         ("centered", ctypes.c_int),
         ("dt", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("dy", ctypes.c_double),
+        ("dz", ctypes.c_double),
     ]
 
 
@@ -94,6 +98,13 @@ def _declare(L):
     L.orc_tvtemp_norm.restype = _D
     L.orc_ictv_norm.argtypes = [_P, _P, _I, _I, _I, _D, _D, _P]
     L.orc_ictv_norm.restype = _D
+    L.orc_fft3.argtypes = [_P, _I, _I, _I, _I]
+    L.orc_fft3.restype = None
+    for f in ("orc_cart3_K", "orc_cart3_KH"):
+        getattr(L, f).argtypes = [_P, _P, _P, _I, _I, _I, _I, _P]
+        getattr(L, f).restype = None
+    L.orc_recon3.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I]
+    L.orc_recon3.restype = _I
     L.orc_tgv2_norm.argtypes = [_P, _P, _I, _I, _I, _D, _D, _D, _D, _D]
     L.orc_tgv2_norm.restype = _D
     L.orc_ictgv2_norm.argtypes = [_P, _P, _P, _P, _I, _I, _I] + [_D] * 9
@@ -319,11 +330,13 @@ def default_params(method):
     m = METHODS[method] if isinstance(method, str) else int(method)
     p = dict(method=m, maxIt=500, sigma=1.0 / 3.0, tau=1.0 / 3.0, sigmaTauRatio=1.0,
              tsw=5.0, tsw2=1.0, lambda_=1.0, alpha0=float(np.sqrt(2.0)), alpha1=1.0,
-             alpha=0.5, schedule=0, gapEvery=0, centered=0, dt=1.0)
+             alpha=0.5, schedule=0, gapEvery=0, centered=0, dt=1.0, dx=1.0, dy=1.0, dz=1.0)
     if m == ICTGV2:
         p.update(tsw=6.5, tsw2=2.5, alpha=0.6423)
     if m == ICTV:  # ictv.cpp:37-62
         p.update(tsw=4.0, tsw2=0.5, alpha=0.5, alpha1=1.0)
+    if m == TGV2_3D:  # tgv2_3d.cpp InitParams
+        p.update(alpha0=float(np.sqrt(3.0)))
     return p
 
 
@@ -358,6 +371,52 @@ def recon(params: dict, data, b1, mask=None, x0=None, nthreads=0, logs=False):
         return x
     return dict(x=x, primal=xall, sigma=sig[: pr.maxIt], tau=tau[: pr.maxIt],
                 gap=gaps[: ng.value])
+
+
+def fft3(a, sign):
+    """Unnormalised 3D DFT of a [D][H][W] array (sign -1: e^{-i})."""
+    a = _c(a).copy()
+    D, H, W = a.shape
+    lib().orc_fft3(_ptr(a), D, H, W, int(sign))
+    return a
+
+
+def cart3_K(x, b1, mask=None):
+    """Volumetric K: x [D][H][W], b1 [C][D][H][W] -> [C][D][H][W]."""
+    x, b1 = _c(x), _c(b1)
+    D, H, W = x.shape
+    C = b1.shape[0]
+    m = None if mask is None else _c(mask, np.float64)
+    out = np.zeros((C, D, H, W), np.complex128)
+    lib().orc_cart3_K(_ptr(x), _ptr(b1), _ptr(m), W, H, D, C, _ptr(out))
+    return out
+
+
+def cart3_KH(z, b1, mask=None):
+    z, b1 = _c(z), _c(b1)
+    C, D, H, W = z.shape
+    m = None if mask is None else _c(mask, np.float64)
+    out = np.zeros((D, H, W), np.complex128)
+    lib().orc_cart3_KH(_ptr(z), _ptr(b1), _ptr(m), W, H, D, C, _ptr(out))
+    return out
+
+
+def recon3(params: dict, data, b1, mask=None, x0=None, nthreads=0):
+    """TGV2_3D loop on a volume: data, b1 [C][D][H][W]; returns x [D][H][W]
+    and the sigma/tau logs."""
+    pr = OrcParams(**params)
+    data, b1 = _c(data), _c(b1)
+    C, D, H, W = data.shape
+    m = None if mask is None else _c(mask, np.float64)
+    x = np.zeros((D, H, W), np.complex128) if x0 is None else _c(x0).copy()
+    xall = np.zeros((lib().orc_primal_components(TGV2), D, H, W), np.complex128)
+    sig = np.zeros(max(pr.maxIt, 1))
+    tau = np.zeros(max(pr.maxIt, 1))
+    rc = lib().orc_recon3(ctypes.byref(pr), W, H, D, C, _ptr(data), _ptr(b1), _ptr(m), _ptr(x),
+                          _ptr(xall), _ptr(sig), _ptr(tau), int(nthreads))
+    if rc < 0:
+        raise ValueError("invalid parameters")
+    return dict(x=x, sigma=sig[: pr.maxIt], tau=tau[: pr.maxIt])
 
 
 def primal_components(method):

@@ -590,16 +590,69 @@ typedef struct {
   int nc, M, J;
   double osf;
   const double *traj, *dcf;
+  int vol;  /* volumetric operator: T is the depth */
 } orc_op;
 
+void orc_fft3(double *a, int D, int H, int W, int sign) {
+  long P = (long)H * W;
+  for (int d = 0; d < D; d++) orc_fft2(a + 2 * d * P, H, W, sign, 0);
+  double *col = malloc(sizeof(double) * 2 * D);
+  for (long p = 0; p < P; p++) {
+    for (int d = 0; d < D; d++) {
+      col[2 * d] = a[2 * (d * P + p)];
+      col[2 * d + 1] = a[2 * (d * P + p) + 1];
+    }
+    orc_fft1(col, D, sign);
+    for (int d = 0; d < D; d++) {
+      a[2 * (d * P + p)] = col[2 * d];
+      a[2 * (d * P + p) + 1] = col[2 * d + 1];
+    }
+  }
+  free(col);
+}
+
+/* cartesian_operator3d.cpp:216-280 (K) and :64-133 (K^H) */
+void orc_cart3_K(const double *xr, const double *b1r, const double *mask, int W, int H, int D,
+                 int C, double *zr) {
+  long N = (long)W * H * D;
+  const dc *x = (const dc *)xr, *b1 = (const dc *)b1r;
+  dc *z = (dc *)zr;
+  double s = 1.0 / sqrt((double)N);
+  for (int c = 0; c < C; c++) {
+    dc *zc = z + c * N;
+    for (long i = 0; i < N; i++) zc[i] = b1[c * N + i] * x[i];
+    orc_fft3((double *)zc, D, H, W, +1);
+    for (long i = 0; i < N; i++) zc[i] *= s * (mask ? mask[i] : 1.0);
+  }
+}
+
+void orc_cart3_KH(const double *zr, const double *b1r, const double *mask, int W, int H, int D,
+                  int C, double *xr) {
+  long N = (long)W * H * D;
+  const dc *z = (const dc *)zr, *b1 = (const dc *)b1r;
+  dc *x = (dc *)xr, *t = malloc(sizeof(dc) * N);
+  double s = 1.0 / sqrt((double)N);
+  for (long i = 0; i < N; i++) x[i] = 0.0;
+  for (int c = 0; c < C; c++) {
+    for (long i = 0; i < N; i++) t[i] = z[c * N + i] * (mask ? mask[i] : 1.0);
+    orc_fft3((double *)t, D, H, W, -1);
+    for (long i = 0; i < N; i++) x[i] += conj(b1[c * N + i]) * t[i] * s;
+  }
+  free(t);
+}
+
 static void op_K(const orc_op *op, const double *x, const double *b1, double *z) {
-  if (op->nc)
+  if (op->vol)
+    orc_cart3_K(x, b1, op->mask, op->W, op->H, op->T, op->C, z);
+  else if (op->nc)
     orc_nufft_K(x, b1, op->traj, op->dcf, op->W, op->H, op->C, op->T, op->M, op->J, op->osf, z);
   else
     orc_cart_K(x, b1, op->mask, op->W, op->H, op->C, op->T, op->centered, z);
 }
 static void op_KH(const orc_op *op, const double *z, const double *b1, double *x) {
-  if (op->nc)
+  if (op->vol)
+    orc_cart3_KH(z, b1, op->mask, op->W, op->H, op->T, op->C, x);
+  else if (op->nc)
     orc_nufft_KH(z, b1, op->traj, op->dcf, op->W, op->H, op->C, op->T, op->M, op->J, op->osf, x);
   else
     orc_cart_KH(z, b1, op->mask, op->W, op->H, op->C, op->T, op->centered, x);
@@ -825,13 +878,18 @@ static int recon_core(const orc_params *p, const orc_op *op, const double *datar
 #else
   (void)nthreads;
 #endif
-  int method = p->method;
+  int method = p->method == ORC_TGV2_3D ? ORC_TGV2 : p->method;  /* TGV2 on a volume */
   long N = (long)W * H * T, CN = op->kn;
   int P = orc_primal_components(method), D = orc_dual_components(method);
   double d6[6] = {1, 1, 1, 1, 1, 1};
   if (method == ORC_TVTEMP) {  /* dt as given: no time/space weights (tv_temp.cpp:98) */
     if (!(p->dt > 0.0)) return -1;
     d6[2] = p->dt;
+  } else if (p->method == ORC_TGV2_3D) {  /* dx, dy, dz as given */
+    if (!(p->dx > 0.0 && p->dy > 0.0 && p->dz > 0.0)) return -1;
+    d6[0] = p->dx;
+    d6[1] = p->dy;
+    d6[2] = p->dz;
   } else {
     if (orc_time_space_weights(p->tsw, &d6[0], &d6[2]) != 0) return -1;
     d6[1] = d6[0];
@@ -878,7 +936,7 @@ static int recon_core(const orc_params *p, const orc_op *op, const double *datar
     }
     if (sig_log) sig_log[k] = sigma;
     if (tau_log) tau_log[k] = tau;
-    if (p->gapEvery > 0 && k % p->gapEvery == 0 && gap_log && !op->nc) {
+    if (p->gapEvery > 0 && k % p->gapEvery == 0 && gap_log && !op->nc && !op->vol) {
       double g = orc_pdgap(method, (double *)x, (double *)y, (double *)z, datar,
                            b1, mask, W, H, C, T, p->centered, lambda, p->alpha0,
                            p->alpha1, p->alpha, d6);
@@ -899,6 +957,14 @@ int orc_recon(const orc_params *p, int W, int H, int C, int T,
               double *tau_log, double *gap_log, int *n_gap, int nthreads) {
   orc_op op = {W, H, C, T, (long)C * W * H * T, p->centered, mask, 0, 0, 0, 0.0, NULL, NULL};
   return recon_core(p, &op, datar, b1, x_inout, x_out_all, sig_log, tau_log, gap_log, n_gap,
+                    nthreads);
+}
+
+int orc_recon3(const orc_params *p, int W, int H, int D, int C, const double *data,
+               const double *b1, const double *mask, double *x_inout, double *x_out_all,
+               double *sig_log, double *tau_log, int nthreads) {
+  orc_op op = {W, H, C, D, (long)C * W * H * D, 0, mask, 0, 0, 0, 0.0, NULL, NULL, 1};
+  return recon_core(p, &op, data, b1, x_inout, x_out_all, sig_log, tau_log, NULL, NULL,
                     nthreads);
 }
 

@@ -27,7 +27,7 @@ extern "C" {
 #endif
 
 /* Method identifiers, same numbering as the product C-ABI. */
-enum { ORC_TV = 0, ORC_TGV2 = 1, ORC_ICTGV2 = 2, ORC_TVTEMP = 3, ORC_ICTV = 4 };
+enum { ORC_TV = 0, ORC_TGV2 = 1, ORC_ICTGV2 = 2, ORC_TVTEMP = 3, ORC_ICTV = 4, ORC_TGV2_3D = 5 };
 
 typedef struct orc_params {
   int method;            /* ORC_TV / ORC_TGV2 / ORC_ICTGV2 / ORC_TVTEMP / ORC_ICTV */
@@ -46,11 +46,25 @@ typedef struct orc_params {
   int centered;          /* 1: centred FFTs (cartesian_operator.cpp:90-99) */
   double dt;             /* TVTEMP's temporal step, used as given (tv_temp.cpp
                             :49, tvtemp.dt in options_parser.cpp:206) */
+  double dx, dy, dz;     /* TGV2_3D's steps, used as given (tgv2_3d.cpp
+                            InitParams) */
 } orc_params;
 
 /* ---- FFT (unnormalised, sign -1 = e^{-i...}, +1 = e^{+i...}) ---- */
 void orc_fft1(double *a, int n, int sign);
 void orc_fft2(double *a, int H, int W, int sign, int centered);
+
+/* ---- 3D transform and volumetric operator pair (cartesian_operator3d.cpp):
+ * arrays [D][H][W]; k-space and b1 [C][D][H][W]; K = M F3^{+}(b1 x)/sqrt(N),
+ * K^H = sum_c conj(b1_c) F3^{-}(M z_c)/sqrt(N); unnormalised orc_fft3 ---- */
+void orc_fft3(double *a, int D, int H, int W, int sign);
+void orc_cart3_K(const double *x, const double *b1, const double *mask, int W, int H, int D,
+                 int C, double *z);
+void orc_cart3_KH(const double *z, const double *b1, const double *mask, int W, int H, int D,
+                  int C, double *x);
+int orc_recon3(const orc_params *p, int W, int H, int D, int C, const double *data,
+               const double *b1, const double *mask, double *x_inout, double *x_out_all,
+               double *sig_log, double *tau_log, int nthreads);
 
 /* ---- Cartesian operator pair ---- */
 void orc_cart_K(const double *x, const double *b1, const double *mask, int W,
