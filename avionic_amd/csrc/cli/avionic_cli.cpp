@@ -4,7 +4,8 @@
 //
 //   avionic [options] -d w:h:d:nRO:nEnc1:nEnc2:c:f <kdata.bin> <mask|traj.bin> <output.bin>
 //
-// Supported: methods TV, TVTEMP, TGV2, ICTV, ICTGV2; Cartesian and (-n) non-Cartesian data;
+// Supported: methods TV, TVTEMP, TGV2, ICTV, ICTGV2 on Cartesian and (-n)
+// non-Cartesian 2D+t data, and TGV2_3D on one Cartesian volume (depth > 1);
 // coil maps (-s) and initial image (-u); the .cfg parameter file (-p) with
 // the reference's keys; -i maxIt, -j stopPDGap, -a adaptlambda, -w density,
 // -e extra results, -v verbose, -g debugstep, -o data normalisation.  Not in
@@ -122,7 +123,7 @@ Options parse_args(int argc, char **argv) {
     if (a == "-h" || a == "--help") {
       std::printf(
           "Usage: avionic [options] -d w:h:d:nRO:nEnc1:nEnc2:c:f <kdata> <mask/traj> <output>\n"
-          "  -m, --method TV|TVTEMP|TGV2|ICTV|ICTGV2  -i, --maxIt N  -j, --stopPDGap G\n"
+          "  -m, --method TV|TVTEMP|TGV2|TGV2_3D|ICTV|ICTGV2  -i, --maxIt N  -j, --stopPDGap G\n"
           "  -p, --params file.cfg         -s, --sens b1.bin -u, --uzero u0.bin\n"
           "  -n, --nonuniform  -w, --dens dcf.bin  -a, --adaptlambda  -e, --extradata\n"
           "  -v, --verbose     -g, --debugstep N   -o, --normalize   --dry-run\n");
@@ -185,7 +186,8 @@ avionic_method method_of(const std::string &m) {
   if (m == "ICTGV2") return AVIONIC_ICTGV2;
   if (m == "TVTEMP") return AVIONIC_TVTEMP;
   if (m == "ICTV") return AVIONIC_ICTV;
-  die("method " + m + " is not part of this build (TV, TVTEMP, TGV2, ICTV, ICTGV2)", 3);
+  if (m == "TGV2_3D") return AVIONIC_TGV2_3D;
+  die("method " + m + " is not part of this build (TV, TVTEMP, TGV2, TGV2_3D, ICTV, ICTGV2)", 3);
 }
 
 // Config section keys of one method onto the parameter struct
@@ -195,7 +197,7 @@ void apply_cfg(const Options &o, const std::string &sec, avionic_params &p) {
     auto it = o.cfg.find(sec + "." + k);
     if (it != o.cfg.end()) dst = std::atof(it->second.c_str());
   };
-  get("dx", p.dx); get("dy", p.dy); get("dt", p.dt);
+  get("dx", p.dx); get("dy", p.dy); get("dt", p.dt); get("dz", p.dz);
   get("sigma", p.sigma); get("tau", p.tau); get("sigmaTauRatio", p.sigmaTauRatio);
   get("timeSpaceWeight", p.timeSpaceWeight); get("lambda", p.lambda);
   get("alpha0", p.alpha0); get("alpha1", p.alpha1); get("alpha", p.alpha);
@@ -371,9 +373,15 @@ void export_cb(const char *name, const float *data, size_t count, void *user) {
 int main(int argc, char **argv) {
   Options o = parse_args(argc, argv);
   const Dims &d = o.dims;
-  if (d.depth > 1) die("3D data (depth > 1) is not part of this build", 3);
   const avionic_method method = method_of(o.method);
-  const unsigned W = d.width, H = d.height, C = d.coils, T = d.frames;
+  // TGV2_3D reconstructs one volume: the third axis is the depth, k-space and
+  // coil maps are coil-major [c][depth][h][w] (main.cpp:403-406, 466-474)
+  const bool vol = method == AVIONIC_TGV2_3D;
+  if (!vol && d.depth > 1) die("3D data (depth > 1) needs -m TGV2_3D", 3);
+  if (vol && o.nonuniform) die("non-Cartesian 3D data is not part of this build", 3);
+  if (vol && o.normalize) die("data normalisation (-o) of 3D data is not part of this build", 3);
+  if (vol && d.frames != 1) die("TGV2_3D takes one frame");
+  const unsigned W = d.width, H = d.height, C = d.coils, T = vol ? d.depth : d.frames;
   if (!W || !H || !C || !T) die("width, height, coils and frames must be positive");
   const size_t HW = (size_t)W * H, N = HW * T;
   const size_t M = o.nonuniform ? (size_t)d.readouts * d.enc1 : 0;
@@ -385,6 +393,7 @@ int main(int argc, char **argv) {
                     : method == AVIONIC_TVTEMP ? "tvtemp"
                     : method == AVIONIC_TGV2   ? "tgv2"
                     : method == AVIONIC_ICTV   ? "ictv"
+                    : method == AVIONIC_TGV2_3D ? "tgv2_3D"
                                                : "ictgv2";
   apply_cfg(o, sec, p);
   if (!p.adaptLambda && !(p.lambda > 0.0))
@@ -407,10 +416,13 @@ int main(int argc, char **argv) {
   if (!have_b1)
     die("coil maps (-s) are required: the coil-map estimation is not part of this build", 3);
   b1 = load(o.sens, "coil maps");
-  if (b1.n != HW * C) die("coil maps have " + std::to_string(b1.n) + " values, dims imply " + std::to_string(HW * C));
+  const size_t B1N = vol ? N * C : HW * C;
+  if (b1.n != B1N) die("coil maps have " + std::to_string(b1.n) + " values, dims imply " + std::to_string(B1N));
   if (!o.u0.empty()) {
     u0 = load(o.u0, "initial image");
-    if (u0.n != HW && u0.n != N) die("initial image must hold w*h or w*h*frames values");
+    if (vol ? u0.n != N : (u0.n != HW && u0.n != N))
+      die(vol ? "initial image must hold w*h*depth values"
+              : "initial image must hold w*h or w*h*frames values");
   }
   if (o.nonuniform) {
     if (o.dens.empty()) die("non-Cartesian data needs the density compensation (-w)");
@@ -440,7 +452,14 @@ int main(int argc, char **argv) {
   std::vector<float> maskh(mk.n);
   for (size_t i = 0; i < mk.n; ++i) maskh[i] = (float)mk.re[i];
   std::vector<float> wh;
-  if (!o.nonuniform) {
+  if (vol) {
+    // zero the unsampled k-space of every coil volume (main.cpp:466-474)
+    for (size_t c = 0; c < C; ++c)
+      for (size_t i = 0; i < N; ++i) {
+        kdh[2 * (c * N + i)] *= maskh[i];
+        kdh[2 * (c * N + i) + 1] *= maskh[i];
+      }
+  } else if (!o.nonuniform) {
     // zero the unsampled k-space of every frame and coil (main.cpp:509-519)
     for (size_t t = 0; t < T; ++t)
       for (size_t c = 0; c < C; ++c)
@@ -474,7 +493,7 @@ int main(int argc, char **argv) {
   }
 
   Dev dk(sizeof(float) * kdh.size()), dmask(sizeof(float) * maskh.size()),
-      db1(sizeof(float) * 2 * HW * C), dx(sizeof(float) * 2 * N), dw(sizeof(float) * (wh.size() + 1));
+      db1(sizeof(float) * 2 * B1N), dx(sizeof(float) * 2 * N), dw(sizeof(float) * (wh.size() + 1));
   upload(dk, kdh);
   upload(dmask, maskh);
   upload(db1, to_c64(b1));
@@ -485,6 +504,8 @@ int main(int argc, char **argv) {
     check(avionic_noncartesian_operator_create(W, H, C, T, d.readouts, d.enc1, dmask.f(), dw.f(), kw,
                                                osf, &op),
           "non-Cartesian operator");
+  else if (vol)
+    check(avionic_cartesian3d_operator_create(W, H, T, C, dmask.f(), &op), "3D Cartesian operator");
   else
     check(avionic_cartesian_operator_create(W, H, C, T, dmask.f(), 0, &op), "Cartesian operator");
 
@@ -492,7 +513,10 @@ int main(int argc, char **argv) {
   // for Cartesian data the time-averaged zero-filled image
   // u0 = sum_c conj(b1_c) F(sum_t d_t / max(1, sum_t M_t)) (get_sos_crec_opt.m:7-15)
   std::vector<float> x0(2 * N, 0.f);
-  if (!o.u0.empty()) {
+  if (vol) {
+    // the volume from the file (main.cpp:770-773), or zero
+    if (!o.u0.empty()) x0 = to_c64(u0);
+  } else if (!o.u0.empty()) {
     const std::vector<float> u = to_c64(u0);
     for (size_t t = 0; t < T; ++t)
       for (size_t i = 0; i < 2 * HW; ++i) x0[2 * t * HW + i] = u[(u0.n == N ? 2 * t * HW : 0) + i];

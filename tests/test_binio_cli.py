@@ -98,10 +98,14 @@ def test_cli_rejects_what_it_cannot_do(tmp_path):
     r = _run(["-d", "8:6:1:8:6:1:3:3", "-s", str(tmp_path / "b1.bin"), "-p",
               str(tmp_path / "p.cfg")] + files + ["--dry-run"], ok=False)
     assert r.returncode == 2 and "k-space data has" in r.stderr
+    # a volume needs -m TGV2_3D
+    r = _run(["-d", "8:6:3:8:6:1:3:1", "-s", str(tmp_path / "b1.bin"), "-p",
+              str(tmp_path / "p.cfg")] + files + ["--dry-run"], ok=False)
+    assert r.returncode == 3 and "TGV2_3D" in r.stderr
     for flag, code in (("-r", 3), ("--bogus", 2)):
         r = _run([flag, "-d", dims] + files, ok=False)
         assert r.returncode == code, (flag, r.stderr)
-    r = _run(["-m", "TGV2_3D", "-d", dims, "-s", str(tmp_path / "b1.bin"), "-p",
+    r = _run(["-m", "ASLTGV2RECON4D", "-d", dims, "-s", str(tmp_path / "b1.bin"), "-p",
               str(tmp_path / "p.cfg")] + files + ["--dry-run"], ok=False)
     assert r.returncode == 3 and "not part of this build" in r.stderr
     # no lambda and no adaptation

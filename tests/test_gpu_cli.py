@@ -232,3 +232,33 @@ def test_cli_other_regularisers_match_library(gpu, tmp_path, method):
     rec.IterativeReconstruction(gpu._t(k * mask[:, None]), x, gpu._t(b1))
     ref = gpu._n(x)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def test_cli_tgv2_3d_matches_library(gpu, tmp_path):
+    """-m TGV2_3D on one volume (-d w:h:depth:...:c:1) with the [tgv2_3D]
+    section (options_parser.cpp:227-237): coil-major k-space masked per coil
+    (main.cpp:466-474), equal to the library's volumetric solver."""
+    rng = np.random.default_rng(14)
+    W, H, D, C = 16, 12, 6, 3
+    k = (rng.standard_normal((C, D, H, W)) + 1j * rng.standard_normal((C, D, H, W))) * 10
+    mask = (rng.random((D, H, W)) < 0.5).astype(float)
+    b1 = rng.standard_normal((C, D, H, W)) * 0.2 + 1.0 + 0.1j
+    u0 = rng.standard_normal((D, H, W)) + 0j
+    for name, v in (("k", k), ("m", mask), ("b1", b1), ("u0", u0)):
+        A.write_bin(tmp_path / f"{name}.bin", v)
+    (tmp_path / "p.cfg").write_text("[tgv2_3D]\nlambda = 3.0\ndz = 2.0\n")
+    _run(["-m", "TGV2_3D", "-i", "40", "-p", str(tmp_path / "p.cfg"), "-d",
+          f"{W}:{H}:{D}:{W}:{H}:{D}:{C}:1", "-s", str(tmp_path / "b1.bin"), "-u",
+          str(tmp_path / "u0.bin"), str(tmp_path / "k.bin"), str(tmp_path / "m.bin"),
+          str(tmp_path / "x.bin")])
+    got = A.read_bin(tmp_path / "x.bin").reshape(D, H, W)
+    op = A.CartesianOperator3D(W, H, D, C, gpu._r(mask))
+    rec = A.TGV2_3D(W, H, C, D, op, None)
+    p = rec.GetParams()
+    p.update(maxIt=40, adaptLambda=0, dz=2.0)
+    p["lambda"] = 3.0
+    rec.SetParams(p)
+    x = gpu._t(u0)
+    rec.IterativeReconstruction(gpu._t(k * mask[None]), x, gpu._t(b1))
+    ref = gpu._n(x)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
