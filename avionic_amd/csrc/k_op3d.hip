@@ -173,6 +173,7 @@ class CartesianOperator3D : public Operator {
     return nb;
   }
   int max_partials() const override { return blocks_for(dense_elems()); }
+  bool volumetric() const override { return true; }
 
  private:
   DevBuf mask_, twx_, twy_, twz_, tmp_, ku_;

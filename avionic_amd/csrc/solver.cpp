@@ -171,6 +171,8 @@ Recon::Recon(int method_, int W_, int H_, int C_, int T_, const avionic_params *
   AV_CHECK(op != nullptr, "operator is null");
   AV_CHECK(op->W == W && op->H == H && op->C == C && op->T == T,
            "operator dimensions do not match the solver");
+  AV_CHECK(op->volumetric() == volumetric,
+           "TGV2_3D runs on the volumetric operator, and that operator only with TGV2_3D");
   if (p) {
     params = *p;
     if (params.adaptLambda) params.lambda = op->adapt_lambda(params.adaptLambdaK, params.adaptLambdaD);

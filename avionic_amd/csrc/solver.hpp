@@ -70,6 +70,8 @@ class Operator {
                         hipStream_t s) = 0;
   // Upper bound of the partial-sum counts above.
   virtual int max_partials() const = 0;
+  // True for the volumetric operator (third axis = depth).
+  virtual bool volumetric() const { return false; }
 };
 
 class CartesianOperator : public Operator {

@@ -65,3 +65,13 @@ def test_tgv2_3d_recon_matches_oracle(gpu):
     print(f"[gpu tgv2_3d] NRMSE vs oracle {err:.3e}", flush=True)
     assert err < 1e-4
     np.testing.assert_allclose(rec.sigma_log(), ref["sigma"], rtol=1e-5)
+
+
+def test_method_and_operator_must_match(gpu):
+    import avionic_amd as A
+    op3 = A.CartesianOperator3D(8, 8, 4, 2, None)
+    with pytest.raises(Exception, match="volumetric"):
+        A.TGV2(8, 8, 2, 4, op3, None)
+    op2 = A.CartesianOperator(8, 8, 2, 4, None, centered=False)
+    with pytest.raises(Exception, match="volumetric"):
+        A.TGV2_3D(8, 8, 2, 4, op2, None)
