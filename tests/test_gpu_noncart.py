@@ -119,13 +119,14 @@ def _recon_gpu(gpu, method, p, data, b1, tr, dcf, n_fe, spf, J, osf):
     T, C, M = data.shape
     H, W = b1.shape[1:]
     op = make_op(gpu, W, H, C, T, n_fe, spf, tr, dcf, J, osf)
-    cls = {0: gpu.A.TV, 1: gpu.A.TGV2, 2: gpu.A.ICTGV2}[method]
+    cls = {0: gpu.A.TV, 1: gpu.A.TGV2, 2: gpu.A.ICTGV2, 3: gpu.A.TVTEMP, 4: gpu.A.ICTV}[method]
     s = cls(W, H, C, T, op, None)
     q = s.GetParams()
     q.update(maxIt=int(p["maxIt"]), sigma=p["sigma"], tau=p["tau"],
              sigmaTauRatio=p["sigmaTauRatio"], timeSpaceWeight=p["tsw"],
              timeSpaceWeight2=p["tsw2"], alpha0=p["alpha0"], alpha1=p["alpha1"],
-             alpha=p["alpha"], adaptSchedule=int(p.get("schedule", 0)), adaptLambda=0)
+             alpha=p["alpha"], adaptSchedule=int(p.get("schedule", 0)), adaptLambda=0,
+             dt=p["dt"] if method == 3 else q["dt"])
     q["lambda"] = p["lambda_"]
     s.SetParams(q)
     x = gpu._t(np.zeros((T, H, W)))
@@ -133,7 +134,7 @@ def _recon_gpu(gpu, method, p, data, b1, tr, dcf, n_fe, spf, J, osf):
     return gpu._n(x), np.array(s.sigma_log()), np.array(s.tau_log())
 
 
-@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2"])
+@pytest.mark.parametrize("method", ["tv", "tgv2", "ictgv2", "tvtemp", "ictv"])
 def test_recon_loop_matches_oracle(gpu, method):
     W = H = 32
     C, T, nfe, spf, J, osf = 3, 4, 64, 9, 3, 1.5
@@ -150,8 +151,12 @@ def test_recon_loop_matches_oracle(gpu, method):
     p = O.default_params(method)
     k, d = O.adapt_lambda_defaults(method)
     p.update(maxIt=200, lambda_=W / (spf * np.pi / 2) * k + d)  # noncartesian_operator.cpp:88-95
+    if method == "tvtemp":
+        p.update(dt=0.8)
+    if method == "ictv":
+        p.update(sigma=0.1, tau=0.1)  # its step rule never shrinks the steps
     ref = O.recon_nc(p, data, b1, tr, dcf, J=J, osf=osf, logs=True)
-    m = {"tv": 0, "tgv2": 1, "ictgv2": 2}[method]
+    m = O.METHODS[method]
     x, sig, tau = _recon_gpu(gpu, m, p, data, b1, tr, dcf, nfe, spf, J, osf)
     assert np.isfinite(x).all()
     assert rel(x, ref["x"]) < 1e-4
