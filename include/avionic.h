@@ -111,8 +111,9 @@ int avionic_cartesian_operator_create(unsigned width, unsigned height,
 /* CartesianOperator3D(width, height, depth, coils, mask)
  * (cartesian_operator3d.cpp:1-345): unitary 3D transforms over
  * [depth][height][width]; k-space and coil maps are coil-major
- * [coils][depth][height][width]; mask [depth][height][width] or NULL.  K^H
- * (avionic_operator_forward) masks a copy of its input, not the input. */
+ * [coils][depth][height][width]; mask: device pointer [depth][height][width]
+ * or NULL (fully sampled), copied at creation.  K^H (avionic_operator_forward)
+ * masks a copy of its input, not the input. */
 int avionic_cartesian3d_operator_create(unsigned width, unsigned height, unsigned depth,
                                         unsigned coils, const float *mask,
                                         avionic_operator **op);
