@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace avionic {
 
@@ -105,6 +106,20 @@ __device__ __forceinline__ void gst(T *base, unsigned i, const T &v) {
   base[i] = v;
 #endif
 }
+// Streaming (non-temporal) store of a 4- or 8-byte value, for outputs that
+// are not read again before they would have left the caches.
+template <class T>
+__device__ __forceinline__ void gst_stream(T *base, unsigned i, const T &v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  typedef __attribute__((address_space(1))) char gchar;
+  typedef __attribute__((address_space(1))) U gU;
+  __builtin_nontemporal_store(__builtin_bit_cast(U, v), (gU *)((gchar *)base + i * (unsigned)sizeof(T)));
+#else
+  base[i] = v;
+#endif
+}
 
 // Dual storage kinds (avionic_params::dualStorage).
 enum DualStorage { kDualF32 = 0, kDualBF16 = 1, kDualQ16 = 2 };
@@ -117,7 +132,7 @@ struct DualIO;
 template <>
 struct DualIO<float2> {
   __device__ static inline float2 load(const float2 *p, unsigned i, float) { return gld(p, i); }
-  __device__ static inline void store(float2 *p, unsigned i, float2 v, float) { gst(p, i, v); }
+  __device__ static inline void store(float2 *p, unsigned i, float2 v, float) { gst_stream(p, i, v); }
 };
 template <>
 struct DualIO<bf16x2> {
@@ -127,7 +142,7 @@ struct DualIO<bf16x2> {
   }
   __device__ static inline void store(bf16x2 *p, unsigned i, float2 v, float) {
     const uint32_t w = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
-    gst(reinterpret_cast<uint32_t *>(p), i, w);
+    gst_stream(reinterpret_cast<uint32_t *>(p), i, w);
   }
 };
 template <>
@@ -144,7 +159,7 @@ struct DualIO<q16x2> {
     const float inv = kQ16Max * __builtin_amdgcn_rcpf(r);
     const uint32_t w = (uint32_t)(uint16_t)f32_to_q16(v.x, inv) |
                        ((uint32_t)(uint16_t)f32_to_q16(v.y, inv) << 16);
-    gst(reinterpret_cast<uint32_t *>(p), i, w);
+    gst_stream(reinterpret_cast<uint32_t *>(p), i, w);
   }
 };
 

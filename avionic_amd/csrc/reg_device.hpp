@@ -297,8 +297,8 @@ __device__ inline void primal_voxel(const Grid3 &g, const RegParams &rp, float2 
     if (c < P) {
       float2 *xc = x + c * N, *ec = ext + c * N;
       const float2 xo = gld(xc, i);
-      gst(ec, i, csub(cscale(upd[c], 2.f), xo));
-      gst(xc, i, upd[c]);
+      gst_stream(ec, i, csub(cscale(upd[c], 2.f), xo));
+      gst_stream(xc, i, upd[c]);
     }
   }
 }
