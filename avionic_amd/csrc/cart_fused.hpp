@@ -407,7 +407,7 @@ struct RowDualStore {
     float2 zz = gld(z, gi);
     zz = cadd(zz, cscale(val, sigma));
     zz = cscale(csub(zz, cscale(gld(d, gi), sigma)), inv);
-    gst(z, gi, zz);
+    gst_stream(z, gi, zz);
     const int xp = centered ? wrap_add(kx, wc, W) : kx;
     tile2[b * W + xp] = cscale(zz, mk);
   }
@@ -609,7 +609,7 @@ __device__ inline double row_fused_ct(const CartGeom &g, int blk, const float2 *
       const float2 val = cscale(w[c][r], scale * mk[c][r]);
       float2 zn = cadd(zz[c][r], cscale(val, sigma));
       zn = cscale(csub(zn, cscale(dd[c][r], sigma)), inv);
-      if (b < nr) gst(zb, (unsigned)(b * L + kx), zn);
+      if (b < nr) gst_stream(zb, (unsigned)(b * L + kx), zn);
       w[c][r] = cscale(zn, mk[c][r]);
     }
   }
