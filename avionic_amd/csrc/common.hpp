@@ -106,6 +106,21 @@ __device__ __forceinline__ void gst(T *base, unsigned i, const T &v) {
   base[i] = v;
 #endif
 }
+// Streaming (non-temporal) load of a 4- or 8-byte value that no other lane
+// reads.
+template <class T>
+__device__ __forceinline__ T gld_stream(const T *base, unsigned i) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const U gU;
+  return __builtin_bit_cast(T, __builtin_nontemporal_load((gU *)((gchar *)base + i * (unsigned)sizeof(T))));
+#else
+  return base[i];
+#endif
+}
+
 // Streaming (non-temporal) store of a 4- or 8-byte value, for outputs that
 // are not read again before they would have left the caches.
 template <class T>
@@ -132,12 +147,20 @@ struct DualIO;
 template <>
 struct DualIO<float2> {
   __device__ static inline float2 load(const float2 *p, unsigned i, float) { return gld(p, i); }
+  __device__ static inline float2 load_stream(const float2 *p, unsigned i, float) {
+    return gld_stream(p, i);
+  }
   __device__ static inline void store(float2 *p, unsigned i, float2 v, float) { gst_stream(p, i, v); }
 };
 template <>
 struct DualIO<bf16x2> {
   __device__ static inline float2 load(const bf16x2 *p, unsigned i, float) {
-    const uint32_t w = gld(reinterpret_cast<const uint32_t *>(p), i);
+    return decode(gld(reinterpret_cast<const uint32_t *>(p), i));
+  }
+  __device__ static inline float2 load_stream(const bf16x2 *p, unsigned i, float) {
+    return decode(gld_stream(reinterpret_cast<const uint32_t *>(p), i));
+  }
+  __device__ static inline float2 decode(uint32_t w) {
     return make_float2(u2f(w << 16), u2f(w & 0xffff0000u));
   }
   __device__ static inline void store(bf16x2 *p, unsigned i, float2 v, float) {
@@ -148,7 +171,12 @@ struct DualIO<bf16x2> {
 template <>
 struct DualIO<q16x2> {
   __device__ static inline float2 load(const q16x2 *p, unsigned i, float r) {
-    const uint32_t w = gld(reinterpret_cast<const uint32_t *>(p), i);
+    return decode(gld(reinterpret_cast<const uint32_t *>(p), i), r);
+  }
+  __device__ static inline float2 load_stream(const q16x2 *p, unsigned i, float r) {
+    return decode(gld_stream(reinterpret_cast<const uint32_t *>(p), i), r);
+  }
+  __device__ static inline float2 decode(uint32_t w, float r) {
     const float s = r * (1.f / kQ16Max);
     return make_float2((float)(int16_t)(w & 0xffffu) * s, (float)(int16_t)(w >> 16) * s);
   }

@@ -219,7 +219,7 @@ __device__ inline void dual_voxel(const Grid3 &g, const RegParams &rp, const flo
   auto ascend = [&](int c0, int n, const float2 *k, float r) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-      if (c < n) p[c] = cadd(DualIO<DT>::load(y + (c0 + c) * N, i, r), cscale(k[c], sig));
+      if (c < n) p[c] = cadd(DualIO<DT>::load_stream(y + (c0 + c) * N, i, r), cscale(k[c], sig));
     if (n == 3) proj3(p, r); else proj6(p, r);
 #pragma unroll
     for (int c = 0; c < 6; ++c)
