@@ -251,14 +251,16 @@ const char *kernel_name(int id) {
   return (id >= 0 && id < kNumKernels) ? names[id] : "?";
 }
 
-// Column-pass workgroups resident at once (4 per CU: the LDS tile).
+// Column-pass workgroups up to which the column pass is submitted before the
+// dual: 16 per CU, i.e. up to four waves of the 4 resident per CU (the LDS
+// tile).
 static int col_resident_blocks() {
   static int n = [] {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 256;
-    return 4 * cus;
+    return 16 * cus;
   }();
   return n;
 }
@@ -383,13 +385,14 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   // dual update (reads ext, writes y) beside the data-term chain (reads ext,
   // writes kcol / z / zhat / khz); the primal update needs both
   if (two && gr.col < col_resident_blocks()) {
-    // A column pass that cannot fill the GPU alone (one reconstruction of
-    // 256x256x30x16: 480 workgroups) is submitted first, so that all of its
-    // workgroups are resident and the dual fills the rest of the CUs; the
-    // dual still depends only on what precedes the column pass.  With more
-    // column workgroups than fit at once the dual goes first (measured on
-    // MI355X, ICTGV2 256x256x30x16: 1 reconstruction 2507 -> 2724 it/s with
-    // this order, 8 reconstructions 3219 -> 3172 it/s).
+    // The column pass of up to a few waves of workgroups (one reconstruction
+    // of 256x256x30x16: 480 workgroups; eight: 3840) is submitted first, so
+    // that its workgroups start and the dual fills the rest of the CUs; the
+    // dual still depends only on what precedes the column pass.  Larger
+    // column passes go after the dual (measured on MI355X, ICTGV2
+    // 256x256x30x16: 1 reconstruction 2507 -> 2724 it/s with this order;
+    // 8 reconstructions 3268 -> 3283 it/s once the dual reads and writes y
+    // with non-temporal accesses, 3219 -> 3172 before).
     AV_HIP(hipEventRecord(st.fork, s));
     launch_col_fwd<false>(gr, d_tasks, n, s);
     AV_HIP(hipStreamWaitEvent(s2, st.fork, 0));
