@@ -329,3 +329,19 @@ def kat_noncart_adapt_lambda(be, s):
 
 
 NAMES = [k for k in KATS if not k.startswith("_") and k != "eps"]
+
+
+def kat_time_averaged_reconstruction(be, s):
+    """TimeAveragedReconstruction (cartesian_coil_construction.cpp:23-88), the
+    first step of the -o normalisation: crec_c = K^H of the frame-averaged
+    data of coil c (one coil, b1 = 1), u = sqrt(sum_c |crec_c|^2) with the
+    phase of sum_c crec_c."""
+    W, H, T, C = s["dims"]
+    k = AR[:W * H * C * T].reshape(T, C, H, W).copy()
+    avg = k.sum(0) / float(T)  # all-ones mask: max(1, sum_t M_t) = T
+    ones = np.ones((1, H, W), np.complex128)
+    crec = np.stack([be.cart_KH(avg[c][None, None], ones, None, False)[0] for c in range(C)])
+    u = np.sqrt((np.abs(crec) ** 2).sum(0)) * np.exp(1j * np.angle(crec.sum(0)))
+    out = [(f"Re u{c[:2]}", u[c[1], c[0]].real, c[2], tol_for(c[2])) for c in s["u"]]
+    out += [(f"Re crec{c[:3]}", crec[c[2], c[1], c[0]].real, c[3], tol_for(c[3])) for c in s["crec"]]
+    return out
