@@ -566,7 +566,8 @@ class _EngineConfig(ctypes.Structure):
 class _ReqDesc(ctypes.Structure):
     _fields_ = [("method", _I), ("width", _U), ("height", _U), ("coils", _U), ("frames", _U),
                 ("params", ctypes.POINTER(_Params)), ("mask", _P), ("centered", _I),
-                ("data", _P), ("b1", _P), ("x0", _P), ("stream", _P)]
+                ("data", _P), ("b1", _P), ("x0", _P), ("stream", _P), ("verbose", _I),
+                ("debug", _I), ("debugstep", _I)]
 
 
 _ENGINE_SIGS = {
@@ -579,6 +580,7 @@ _ENGINE_SIGS = {
     "avionic_engine_cancel": (_I, [_P, ctypes.c_longlong]),
     "avionic_engine_stats": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_SZ)]),
     "avionic_engine_get_step_log": (_I, [_P, ctypes.c_longlong, _P, _P, ctypes.POINTER(_SZ)]),
+    "avionic_engine_get_gap_log": (_I, [_P, ctypes.c_longlong, _P, _P, _P, ctypes.POINTER(_SZ)]),
 }
 
 AVIONIC_E_CAPACITY = -5
@@ -599,8 +601,10 @@ def _engine_lib():
 class Request:
     """Inputs of one reconstruction request (device tensors)."""
 
-    def __init__(self, method, data, b1, mask=None, x0=None, centered=False, params=None):
+    def __init__(self, method, data, b1, mask=None, x0=None, centered=False, params=None,
+                 verbose=False, debug=False, debugstep=1):
         self.method = int(method)
+        self.verbose, self.debug, self.debugstep = bool(verbose), bool(debug), int(debugstep)
         T, C, H, W = data.shape
         self.shape = (W, H, C, T)
         self.data, self.b1, self.mask, self.x0 = data, b1, mask, x0
@@ -623,6 +627,7 @@ class Request:
         d.b1 = _c64(self.b1)
         d.x0 = _c64(self.x0)
         d.stream = _stream()
+        d.verbose, d.debug, d.debugstep = int(self.verbose), int(self.debug), self.debugstep
         return d
 
 
@@ -691,6 +696,19 @@ class Engine:
         _check(_engine_lib().avionic_engine_get_step_log(self._h, int(rid), s.ctypes.data_as(_P),
                                                          t.ctypes.data_as(_P), ctypes.byref(m)))
         return s[: m.value], t[: m.value]
+
+    def gap_log(self, rid):
+        """(gap, regulariser norm, data fidelity) arrays of the in-loop PD gap
+        evaluations so far (SetVerbose / SetDebug / stopPDGap)."""
+        L = _engine_lib()
+        n = ctypes.c_size_t(1 << 30)
+        _check(L.avionic_engine_get_gap_log(self._h, int(rid), None, None, None, ctypes.byref(n)))
+        g, r, f = (np.zeros(max(1, n.value)) for _ in range(3))
+        m = ctypes.c_size_t(n.value)
+        _check(L.avionic_engine_get_gap_log(self._h, int(rid), g.ctypes.data_as(_P),
+                                            r.ctypes.data_as(_P), f.ctypes.data_as(_P),
+                                            ctypes.byref(m)))
+        return g[: m.value], r[: m.value], f[: m.value]
 
 
 def engine_time_kernels(engine: Engine, reps=20):

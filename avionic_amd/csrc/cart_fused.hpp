@@ -20,7 +20,8 @@ namespace avionic {
 
 constexpr int RTILE = kRowTileElems;
 constexpr int NTC = AVIONIC_COL_THREADS;  // threads of a column-pass block
-constexpr int kColSlots = max_slots(kColTileElems, NTC);
+// Outputs per lane of a column-pass stage (narrow / wide plans).
+constexpr int col_slots(bool wide) { return max_slots(kColTileElems, NTC, wide); }
 
 // (i + n/2) mod n and (a + b) mod n for 0 <= i, a < n, 0 <= b <= n, without a
 // division.
@@ -41,12 +42,17 @@ __device__ __forceinline__ int wrap_add(int a, int b, int n) {
 // runtime plan evaluates per element fold into shifts and masks: measured on
 // MI355X, the runtime-plan passes spend more VALU instructions on that index
 // arithmetic than on the transform.
-struct ColPlanRt {
+template <bool W>
+struct ColPlanRtT {
   static constexpr int H = 0, XB = 0;
+  static constexpr bool WIDE = W;
 };
+using ColPlanRt = ColPlanRtT<false>;
+using ColPlanRtWide = ColPlanRtT<true>;
 template <int H_, int XB_, int R0_, int R1_, int R2_ = 1>
 struct ColPlanCt {
   static constexpr int H = H_, XB = XB_, R0 = R0_, R1 = R1_, R2 = R2_;
+  static constexpr bool WIDE = false;
   static_assert(H_ * XB_ <= kColTileElems, "column tile exceeds the LDS tile");
   static_assert(R0_ * R1_ * R2_ == H_, "radices must multiply to the length");
   static_assert(H_ <= 256, "compile-time plans are at most 256 long");
@@ -64,7 +70,8 @@ template <class P, class LD, class ST>
 __device__ __forceinline__ void col_fft(float2 *tile, const CartGeom &g, float sign, int tid,
                                         const float2 *tw, LD &ld, ST &st) {
   if constexpr (P::H == 0) {
-    fft_io<NTC, kColTileElems, true, true, true>(tile, g.ph, g.XBF, 1, g.XBF, sign, tid, tw, ld, st);
+    fft_io<NTC, kColTileElems, true, true, true, P::WIDE>(tile, g.ph, g.XBF, 1, g.XBF, sign, tid, tw,
+                                                          ld, st);
   } else {
     constexpr int L = P::H, B = P::XB, E = L * B;
     NoStore ns;
@@ -309,10 +316,10 @@ struct ColAdjLoad {
 };
 // The coil sum stays in registers, one accumulator per output of the lane.
 // PRE: the coil map values are loaded by prefetch() ahead of the last stage.
-template <int HS, bool PRE>
+template <int HS, bool PRE, int SLOTS>
 struct ColAdjStore {
-  float2 acc[kColSlots];
-  float2 bpre[PRE ? kColSlots : 1];
+  float2 acc[SLOTS];
+  float2 bpre[PRE ? SLOTS : 1];
   const float2 *bc;
   float2 *out;  // image frame t
   int H, W, xb0, centered;
@@ -356,9 +363,10 @@ __device__ inline void col_adjoint_fused(const CartGeom &g, int xbi, int t, cons
   ld.W = g.W;
   ld.xb0 = xbi * XB;
   ld.centered = g.centered;
-  ColAdjStore<P::H, P::H != 0> st;
+  constexpr int SLOTS = col_slots(P::WIDE);
+  ColAdjStore<P::H, P::H != 0, SLOTS> st;
 #pragma unroll
-  for (int k = 0; k < kColSlots; ++k) st.acc[k] = czero();
+  for (int k = 0; k < SLOTS; ++k) st.acc[k] = czero();
   st.out = out + t * HW;
   st.H = g.H;
   st.W = g.W;
@@ -433,6 +441,7 @@ struct RowNorm2Store {
   }
 };
 
+template <bool WIDE>
 __device__ inline void row_dual_fused(const CartGeom &g, int blk, const float2 *kcol, float2 *z,
                                       const float2 *d, float2 *zhat, const DevScalars *sc,
                                       float2 *tile, float2 *tile2, float2 *twl, int *mrl) {
@@ -456,11 +465,11 @@ __device__ inline void row_dual_fused(const CartGeom &g, int blk, const float2 *
   st.scale = g.scale;
   st.sigma = (float)sc->sigma;
   st.inv = (float)(1.0 / (1.0 + sc->sigma / sc->lambda));
-  fft_io<NT, RTILE, false, true, true>(tile, g.pw, nr, W, 1, +1.f, tid, tw, ld, st);
+  fft_io<NT, RTILE, false, true, true, WIDE>(tile, g.pw, nr, W, 1, +1.f, tid, tw, ld, st);
   __syncthreads();  // tile2 holds M z for every row of the block
   RowAdjStore st2{zhat + g0 * W, W, g.centered};
   NoLoad nl;
-  fft_io<NT, RTILE, false, false, true>(tile2, g.pw, nr, W, 1, -1.f, tid, tw, nl, st2);
+  fft_io<NT, RTILE, false, false, true, WIDE>(tile2, g.pw, nr, W, 1, -1.f, tid, tw, nl, st2);
 }
 
 // ------------------------------------------------- row pass, compile-time
@@ -663,6 +672,7 @@ __device__ inline double row_fused_ct(const CartGeom &g, int blk, const float2 *
   return 0.0;
 }
 
+template <bool WIDE>
 __device__ inline double row_norm2_fused(const CartGeom &g, int blk, const float2 *kcol,
                                          float2 *tile, float2 *twl, int *mrl) {
   const int tid = threadIdx.x;
@@ -674,7 +684,7 @@ __device__ inline double row_norm2_fused(const CartGeom &g, int blk, const float
   __syncthreads();
   RowLoad ld{kcol + g0 * W, W};
   RowNorm2Store st{g.mrows, mrow, W, g.centered, g.scale, 0.0};
-  fft_io<NT, RTILE, false, true, true>(tile, g.pw, nr, W, 1, +1.f, tid, tw, ld, st);
+  fft_io<NT, RTILE, false, true, true, WIDE>(tile, g.pw, nr, W, 1, +1.f, tid, tw, ld, st);
   return st.acc;
 }
 

@@ -29,6 +29,7 @@ struct CartGeom {
   int cplan;      // compile-time plan of the fused column passes (ColPlanId)
   int rplan;      // compile-time plan of the fused row pass (RowPlanId)
   int RBF;        // rows per block of the fused row pass
+  int wide_col, wide_row;  // column / row plan is wide (plan_is_wide)
   long nrows;          // sampled rows over all frames (row_off[T])
   long nrows_c;        // C * nrows
   const int *row_off;  // [T+1]
@@ -44,11 +45,19 @@ struct CartGeom {
 // other geometry runs the runtime plan.
 enum ColPlanId { kColPlanRt = 0, kColPlan256, kColPlan128, kColPlan160, kColPlan192, kColPlan224 };
 // Kind of a geometry's fused column passes: 0 runtime plan, 1 compile-time
-// plan (one kernel per kind).
-constexpr int kColKinds = 2;
-__host__ __device__ inline int col_kind(const CartGeom &g) { return g.cplan == kColPlanRt ? 0 : 1; }
+// plan, 2 wide runtime plan (one kernel per kind).
+constexpr int kColKinds = 3;
+__host__ __device__ inline int col_kind(const CartGeom &g) {
+  return g.cplan != kColPlanRt ? 1 : (g.wide_col ? 2 : 0);
+}
 // Compile-time row plans (row length W); rows per block 16 and 32.
 enum RowPlanId { kRowPlanRt = 0, kRowPlan256, kRowPlan128 };
+// Kind of the fused row pass: 0 compile-time plan, 1 runtime plan, 2 wide
+// runtime plan (one kernel per kind).
+constexpr int kRowKinds = 3;
+__host__ __device__ inline int row_kind(const CartGeom &g) {
+  return g.rplan != kRowPlanRt ? 0 : (g.wide_row ? 2 : 1);
+}
 
 // Row-pass modes.
 enum RowMode {

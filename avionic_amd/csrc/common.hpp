@@ -198,7 +198,13 @@ constexpr int kFftTileElems = 4096;  // 32 KiB of float2 (column passes)
 constexpr int kRowTileElems = 2048;  // 16 KiB of float2 (row passes)
 constexpr int kRowMaxRows = 256;     // rows of one row-pass tile at most
 constexpr int kMaxStages = 12;
-constexpr int kMaxFftLen = 512;  // longest 1D transform (twiddles staged in LDS)
+constexpr int kMaxFftLen = 1024;  // longest 1D transform (twiddles staged in LDS)
+// Longest transform of a "narrow" plan.  Narrow plans (length <= 512, radices
+// 16, 8, 4, 2, 3, 5, 7) run in kernels whose LDS twiddle tables and register
+// arrays are sized for them; longer plans and plans with a radix-11 or -13
+// stage ("wide") run in separately compiled kernel variants, so that those
+// stages' larger butterflies and tables cost nothing where they are not used.
+constexpr int kMaxFftLenNarrow = 512;
 // LDS tile of the hot loop's fused column passes (complex elements).
 #ifndef AVIONIC_COL_EMAX
 #define AVIONIC_COL_EMAX 4096
@@ -213,6 +219,12 @@ struct FftPlan {
   int ns[kMaxStages];
   const float2 *tw;  // tw[m] = exp(-2 pi i m / L), m < L (device)
 };
+__host__ __device__ inline bool plan_is_wide(const FftPlan &p) {
+  if (p.L > kMaxFftLenNarrow) return true;
+  for (int s = 0; s < p.nst; ++s)
+    if (p.radix[s] == 11 || p.radix[s] == 13) return true;
+  return false;
+}
 
 // ----------------------------------------------------- reductions / state
 constexpr int kRedThreads = 256;

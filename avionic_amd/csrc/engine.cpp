@@ -9,6 +9,8 @@
 // depend on what else is resident.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
+#include <cstdio>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -37,12 +39,23 @@ struct Slot {
   avionic_params params{};
   int maxit = 0, k_host = 0;
   bool done = false;
+  // in-loop PD gap (avionic_request_desc::verbose / debug, params.stopPDGap)
+  bool verbose = false, debug = false;
+  int debugstep = 1;
+  DevBuf gap_a, gap_b;
+  std::vector<double> gap_log, norm_log, fid_log;
   size_t bytes = 0;
   Task task{};
 
   bool adapts(int k) const {
     return params.adaptSchedule == 1 || k < 10 || (k % 50) == 0;
   }
+  // ictgv2.cpp:462-465
+  bool wants_gap(int k) const {
+    return (verbose && (k < 10 || k % 50 == 0)) || (debug && k % debugstep == 0) ||
+           (params.stopPDGap > 0 && k % 20 == 0);
+  }
+  bool gap_enabled() const { return verbose || debug || params.stopPDGap > 0; }
 };
 
 size_t slot_bytes(int method, int ds, long N, long K, long CHW, int maxit) {
@@ -108,21 +121,20 @@ struct avionic_engine {
     g.drop_graphs();
     // tasks ordered by column kind, then row kind: each kind's kernel is
     // launched over its own span of blocks only
-    auto row_kind = [](const Slot *s) { return s->task.cg.rplan != kRowPlanRt ? 0 : 1; };
     std::stable_sort(g.slots.begin(), g.slots.end(), [&](const Slot *a, const Slot *b) {
       const int ka = col_kind(a->task.cg), kb = col_kind(b->task.cg);
-      return ka != kb ? ka < kb : row_kind(a) < row_kind(b);
+      return ka != kb ? ka < kb : row_kind(a->task.cg) < row_kind(b->task.cg);
     });
     std::vector<Task> tasks;
     int bv = 0, bc = 0, br = 0;
     g.grid = TaskGrid{};
-    bool col_seen[kColKinds] = {}, row_seen[2] = {};
+    bool col_seen[kColKinds] = {}, row_seen[kRowKinds] = {};
     for (Slot *s : g.slots) {
       Task t = s->task;
       t.b_vox = bv;
       t.b_col = bc;
       t.b_row = br;
-      const int ck = col_kind(t.cg), rk = row_kind(s);
+      const int ck = col_kind(t.cg), rk = row_kind(t.cg);
       if (!col_seen[ck]) g.grid.col_lo[ck] = bc;
       if (!row_seen[rk]) g.grid.row_lo[rk] = br;
       col_seen[ck] = row_seen[rk] = true;
@@ -142,6 +154,44 @@ struct avionic_engine {
                             hipMemcpyHostToDevice, stream));
     }
     g.dirty = false;
+  }
+
+  // PD gap after iteration k of one request (its state is final for k once
+  // the engine stream reaches this point); finishes the request early when
+  // the gap falls below stopPDGap: its device iteration limit drops to k + 1,
+  // so the replayed iterations leave it idle.
+  void evaluate_gap(Slot &s, int k) {
+    GapState gs;
+    gs.method = s.method;
+    gs.ds = s.ds;
+    gs.g = s.task.g;
+    gs.rp = s.task.rp;
+    gs.alpha0 = s.params.alpha0;
+    gs.alpha1 = s.params.alpha1;
+    gs.alpha = s.params.alpha;
+    gs.lambda = s.params.lambda;
+    gs.op = s.op.get();
+    gs.x = s.x.as<float2>();
+    gs.y = s.y.p;
+    gs.z = s.z.as<float2>();
+    gs.d = s.d.as<float2>();
+    gs.khz = s.khz.as<float2>();
+    gs.b1 = s.b1.as<float2>();
+    gs.part_a = s.gap_a.as<double>();
+    gs.part_b = s.gap_b.as<double>();
+    const GapValues gv = compact_gap(gs, stream);
+    s.gap_log.push_back(gv.gap);
+    s.norm_log.push_back(gv.norm);
+    s.fid_log.push_back(gv.fidelity);
+    if (s.verbose)
+      std::printf("request %lld: Normalized Primal-Dual Gap after %d iterations: %.4e\n", s.id, k,
+                  gv.gap);
+    if (s.params.stopPDGap > 0 && gv.gap < s.params.stopPDGap) {
+      s.maxit = k + 1;
+      AV_HIP(hipMemcpyAsync(reinterpret_cast<char *>(s.sc.p) + offsetof(DevScalars, maxit),
+                            &s.maxit, sizeof(int), hipMemcpyHostToDevice, stream));
+      AV_HIP(hipStreamSynchronize(stream));  // s.maxit is read by the copy
+    }
   }
 
   void run_iteration(Group &g, bool with_step) {
@@ -285,6 +335,10 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->params = request_params(d, *s->op);
     s->ds = s->params.dualStorage;
     s->maxit = (int)s->params.maxIt;
+    s->verbose = d->verbose != 0;
+    s->debug = d->debug != 0;
+    s->debugstep = d->debugstep > 0 ? d->debugstep : 1;
+    AV_CHECK(s->params.stopPDGap >= 0.0, "stopPDGap must not be negative");
     s->bytes = slot_bytes(s->method, s->ds, s->N, s->K, (long)s->C * s->H * s->W, s->maxit);
     if (e->cfg.hbm_budget_bytes && e->used + s->bytes > e->cfg.hbm_budget_bytes)
       throw Error(AVIONIC_E_CAPACITY, "request does not fit the HBM budget now");
@@ -315,6 +369,11 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     const int nrb = s->op->plan().fused_row_blocks();
     s->preg.ensure(sizeof(double) * 2 * nvb);
     s->pk.ensure(sizeof(double) * std::max(1, nrb));
+    if (s->gap_enabled()) {
+      const size_t np = gap_partials(g, *s->op);
+      s->gap_a.ensure(sizeof(double) * np);
+      s->gap_b.ensure(sizeof(double) * np);
+    }
     AV_HIP(hipMemsetAsync(s->x.p, 0, s->x.bytes, st));
     AV_HIP(hipMemsetAsync(s->y.p, 0, s->y.bytes, st));
     AV_HIP(hipMemsetAsync(s->z.p, 0, s->z.bytes, st));
@@ -401,7 +460,10 @@ int avionic_engine_step(avionic_engine *e, void *stream) {
         if (!any) break;
         e->run_iteration(g, step);
         for (Slot *s : g.slots)
-          if (s->k_host < s->maxit) s->k_host++;
+          if (s->k_host < s->maxit) {
+            const int k = s->k_host++;
+            if (s->wants_gap(k)) e->evaluate_gap(*s, k);
+          }
       }
       for (Slot *s : g.slots)
         if (s->k_host >= s->maxit) s->done = true;
@@ -541,6 +603,24 @@ int avionic_engine_get_step_log(const avionic_engine *e, long long id, double *s
       }
       if (sigma) sigma[k] = cs;
       if (tau) tau[k] = ct;
+    }
+    *n = m;
+  });
+}
+
+int avionic_engine_get_gap_log(const avionic_engine *e, long long id, double *gap, double *norm,
+                               double *fidelity, size_t *n) {
+  return eguard([&] {
+    AV_CHECK(e && n, "null argument");
+    std::lock_guard<std::mutex> lock(e->mu);
+    auto it = e->slots.find(id);
+    if (it == e->slots.end()) throw Error(AVIONIC_E_NOTFOUND, "unknown request id");
+    const Slot *s = it->second.get();
+    const size_t m = std::min(*n, s->gap_log.size());
+    for (size_t k = 0; k < m; ++k) {
+      if (gap) gap[k] = s->gap_log[k];
+      if (norm) norm[k] = s->norm_log[k];
+      if (fidelity) fidelity[k] = s->fid_log[k];
     }
     *n = m;
   });

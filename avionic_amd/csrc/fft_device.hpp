@@ -26,6 +26,14 @@ template <> struct RootTable<7> {
   static constexpr float c[7] = {1.000000000e+00f, 6.234898019e-01f, -2.225209340e-01f, -9.009688679e-01f, -9.009688679e-01f, -2.225209340e-01f, 6.234898019e-01f};
   static constexpr float s[7] = {0.000000000e+00f, 7.818314825e-01f, 9.749279122e-01f, 4.338837391e-01f, -4.338837391e-01f, -9.749279122e-01f, -7.818314825e-01f};
 };
+template <> struct RootTable<11> {
+  static constexpr float c[11] = {1.000000000e+00f, 8.412535328e-01f, 4.154150130e-01f, -1.423148383e-01f, -6.548607339e-01f, -9.594929736e-01f, -9.594929736e-01f, -6.548607339e-01f, -1.423148383e-01f, 4.154150130e-01f, 8.412535328e-01f};
+  static constexpr float s[11] = {0.000000000e+00f, 5.406408175e-01f, 9.096319954e-01f, 9.898214419e-01f, 7.557495744e-01f, 2.817325568e-01f, -2.817325568e-01f, -7.557495744e-01f, -9.898214419e-01f, -9.096319954e-01f, -5.406408175e-01f};
+};
+template <> struct RootTable<13> {
+  static constexpr float c[13] = {1.000000000e+00f, 8.854560257e-01f, 5.680647467e-01f, 1.205366803e-01f, -3.546048870e-01f, -7.485107482e-01f, -9.709418174e-01f, -9.709418174e-01f, -7.485107482e-01f, -3.546048870e-01f, 1.205366803e-01f, 5.680647467e-01f, 8.854560257e-01f};
+  static constexpr float s[13] = {0.000000000e+00f, 4.647231720e-01f, 8.229838659e-01f, 9.927088741e-01f, 9.350162427e-01f, 6.631226582e-01f, 2.393156643e-01f, -2.393156643e-01f, -6.631226582e-01f, -9.350162427e-01f, -9.927088741e-01f, -8.229838659e-01f, -4.647231720e-01f};
+};
 // exp(sign * i * theta) applied as (c, sign * s).
 __device__ inline float2 crot(float2 a, float c, float s, float sign) {
   float ss = sign * s;
@@ -193,7 +201,8 @@ __device__ inline void fft_stage(float2 *buf, int L, int B, int bstride, int ist
 }
 
 // Full transform of B batched length-L sequences in LDS (unnormalised).
-template <int NT, int EMAX, bool BFAST>
+// Radices 11 and 13 are compiled in only with WIDE (see plan_is_wide).
+template <int NT, int EMAX, bool BFAST, bool WIDE = true>
 __device__ inline void fft_lds(float2 *buf, const FftPlan &p, int B, int bstride, int istride,
                                float sign, int tid) {
   for (int s = 0; s < p.nst; ++s) {
@@ -206,6 +215,12 @@ __device__ inline void fft_lds(float2 *buf, const FftPlan &p, int B, int bstride
       case 3: fft_stage<3, NT, EMAX, BFAST>(buf, p.L, B, bstride, istride, Ns, sign, p.tw, tid); break;
       case 5: fft_stage<5, NT, EMAX, BFAST>(buf, p.L, B, bstride, istride, Ns, sign, p.tw, tid); break;
       case 7: fft_stage<7, NT, EMAX, BFAST>(buf, p.L, B, bstride, istride, Ns, sign, p.tw, tid); break;
+      case 11:
+        if constexpr (WIDE) fft_stage<11, NT, EMAX, BFAST>(buf, p.L, B, bstride, istride, Ns, sign, p.tw, tid);
+        break;
+      case 13:
+        if constexpr (WIDE) fft_stage<13, NT, EMAX, BFAST>(buf, p.L, B, bstride, istride, Ns, sign, p.tw, tid);
+        break;
       default: break;
     }
   }
@@ -284,7 +299,7 @@ __device__ __forceinline__ void fft_stage_io(float2 *buf, int L, int B, int bstr
   if (!TOG) __syncthreads();
 }
 
-template <int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, class LD, class ST>
+template <int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, bool WIDE, class LD, class ST>
 __device__ inline void fft_stage_io_dispatch(int R, float2 *buf, int L, int B, int bstride,
                                              int istride, int Ns, float sign, const float2 *tw,
                                              int tid, LD &ld, ST &st) {
@@ -296,6 +311,12 @@ __device__ inline void fft_stage_io_dispatch(int R, float2 *buf, int L, int B, i
     case 3: fft_stage_io<3, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
     case 5: fft_stage_io<5, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
     case 7: fft_stage_io<7, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st); break;
+    case 11:
+      if constexpr (WIDE) fft_stage_io<11, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st);
+      break;
+    case 13:
+      if constexpr (WIDE) fft_stage_io<13, NT, EMAX, BFAST, FROMG, TOG>(buf, L, B, bstride, istride, Ns, sign, tw, tid, ld, st);
+      break;
     default: break;
   }
 }
@@ -307,11 +328,13 @@ struct NoStore {
   __device__ void operator()(int, int, float2, int) {}
 };
 
-// Outputs per lane of one stage: max over radices of ceil(EMAX/R/NT) * R.
-constexpr int max_slots(int emax, int nt) {
+// Outputs per lane of one stage: max over radices of ceil(EMAX/R/NT) * R
+// (radices 11 and 13 only in wide kernels).
+constexpr int max_slots(int emax, int nt, bool wide) {
   int m = 0;
-  const int rs[] = {16, 8, 4, 2, 3, 5, 7};
+  const int rs[] = {16, 8, 4, 2, 3, 5, 7, 11, 13};
   for (int r : rs) {
+    if (!wide && r > 8 && r != 16) continue;
     const int v = ((emax / r + nt - 1) / nt) * r;
     m = v > m ? v : m;
   }
@@ -323,7 +346,7 @@ constexpr int max_slots(int emax, int nt) {
 // before re-using a tile.  With TOG the LDS tile is free once this returns
 // (the last stage read it before the function's final store loop; lanes may
 // still be reading it, so callers sync before writing the tile again).
-template <int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, class LD, class ST>
+template <int NT, int EMAX, bool BFAST, bool FROMG, bool TOG, bool WIDE, class LD, class ST>
 __device__ inline void fft_io(float2 *buf, const FftPlan &p, int B, int bstride, int istride,
                               float sign, int tid, const float2 *tw, LD &ld, ST &st) {
   const int n = p.nst;
@@ -336,18 +359,18 @@ __device__ inline void fft_io(float2 *buf, const FftPlan &p, int B, int bstride,
     return;
   }
   if (n == 1) {
-    fft_stage_io_dispatch<NT, EMAX, BFAST, FROMG, TOG>(p.radix[0], buf, p.L, B, bstride, istride,
+    fft_stage_io_dispatch<NT, EMAX, BFAST, FROMG, TOG, WIDE>(p.radix[0], buf, p.L, B, bstride, istride,
                                                        p.ns[0], sign, tw, tid, ld, st);
     return;
   }
   NoStore ns;
   NoLoad nl;
-  fft_stage_io_dispatch<NT, EMAX, BFAST, FROMG, false>(p.radix[0], buf, p.L, B, bstride, istride,
+  fft_stage_io_dispatch<NT, EMAX, BFAST, FROMG, false, WIDE>(p.radix[0], buf, p.L, B, bstride, istride,
                                                        p.ns[0], sign, tw, tid, ld, ns);
   for (int s = 1; s < n - 1; ++s)
-    fft_stage_io_dispatch<NT, EMAX, BFAST, false, false>(p.radix[s], buf, p.L, B, bstride, istride,
+    fft_stage_io_dispatch<NT, EMAX, BFAST, false, false, WIDE>(p.radix[s], buf, p.L, B, bstride, istride,
                                                          p.ns[s], sign, tw, tid, nl, ns);
-  fft_stage_io_dispatch<NT, EMAX, BFAST, false, TOG>(p.radix[n - 1], buf, p.L, B, bstride, istride,
+  fft_stage_io_dispatch<NT, EMAX, BFAST, false, TOG, WIDE>(p.radix[n - 1], buf, p.L, B, bstride, istride,
                                                      p.ns[n - 1], sign, tw, tid, nl, st);
 }
 

@@ -125,11 +125,11 @@ static int pick_col_plan(int H, int XBF) {
 }
 
 void make_fft_plan(int L, FftPlan &p, void **d_tw) {
-  AV_CHECK(L >= 1 && L <= kMaxFftLen, "FFT length out of range (1..512)");
+  AV_CHECK(L >= 1 && L <= kMaxFftLen, "FFT length out of range (1..1024)");
   p = FftPlan{};
   p.L = L;
   int m = L, ns = 1;
-  const int order[] = {16, 8, 4, 2, 3, 5, 7};
+  const int order[] = {16, 8, 4, 2, 3, 5, 7, 11, 13};
   for (int r : order) {
     while (m % r == 0) {
       AV_CHECK(p.nst < kMaxStages, "FFT plan too long");
@@ -140,7 +140,7 @@ void make_fft_plan(int L, FftPlan &p, void **d_tw) {
       p.nst++;
     }
   }
-  AV_CHECK(m == 1, "FFT length must factor into 2, 3, 5 and 7");
+  AV_CHECK(m == 1, "FFT length must factor into 2, 3, 5, 7, 11 and 13");
   std::vector<float2> tw(L);
   for (int k = 0; k < L; ++k) {
     const double a = -2.0 * M_PI * (double)k / (double)L;
@@ -159,7 +159,7 @@ CartPlan::~CartPlan() {
 
 void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered, hipStream_t s) {
   AV_CHECK(W >= 1 && H >= 1 && C >= 1 && T >= 1, "invalid operator dimensions");
-  AV_CHECK(H <= kMaxFftLen && W <= kMaxFftLen, "image dimensions exceed the longest FFT (512)");
+  AV_CHECK(H <= kMaxFftLen && W <= kMaxFftLen, "image dimensions exceed the longest FFT (1024)");
   g = CartGeom{};
   g.W = W; g.H = H; g.C = C; g.T = T;
   g.centered = centered ? 1 : 0;
@@ -228,6 +228,8 @@ void CartPlan::init(int W, int H, int C, int T, const float *mask, int centered,
   g.rowmap = (const int *)d_rowmap;
   make_fft_plan(W, g.pw, &d_twx);
   make_fft_plan(H, g.ph, &d_twy);
+  g.wide_col = plan_is_wide(g.ph) ? 1 : 0;
+  g.wide_row = plan_is_wide(g.pw) ? 1 : 0;
   // The tables above come from pageable host memory: such a copy (on the
   // null stream) may return before its data has landed, and the kernels that
   // index with them may run on non-blocking streams, which do not wait for

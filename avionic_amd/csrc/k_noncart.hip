@@ -18,8 +18,10 @@ constexpr int NCF = AVIONIC_NC_COLF_THREADS;  // threads of a forward column-pas
 constexpr int NCC = AVIONIC_NC_COLA_THREADS;  // threads of an adjoint column-pass block
 constexpr int NCR = 256;   // threads of a row-pass / sample-pass block
 constexpr int NCE = 4096;  // complex elements of an LDS transform tile
-constexpr int kColSlotsNc = max_slots(NCE, NCF);
-constexpr int kRowSlotsNc = max_slots(NCE, NCR);
+// Twiddle-table entries and per-lane outputs of the narrow / wide kernel
+// variants (plan_is_wide: a grid longer than 512 or a radix-11 / -13 stage).
+constexpr int nc_table_len(bool wide) { return wide ? kMaxFftLen : kMaxFftLenNarrow; }
+constexpr int row_slots_nc(bool wide) { return max_slots(NCE, NCR, wide); }
 
 __device__ inline int wrap_up(int v, int G) { return v >= G ? v - G : v; }
 
@@ -48,12 +50,13 @@ struct NcColFwdStore {
   }
 };
 
+template <bool WIDE>
 __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__restrict__ u,
                                                     const float2 *__restrict__ usub,
                                                     const float2 *__restrict__ b1,
                                                     float2 *__restrict__ tmp) {
   __shared__ float2 tile[NCE];
-  __shared__ float2 twl[kMaxFftLen];
+  __shared__ float2 twl[nc_table_len(WIDE)];
   const int tid = threadIdx.x;
   const int t = blockIdx.y;
   const long HW = (long)g.H * g.W;
@@ -76,7 +79,7 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
   for (int c = c0; c < c1; ++c) {
     ld.bc = b1 + c * HW;
     st.tc = tmp + ((long)t * g.C + c) * g.Gy * g.W;
-    fft_io<NCF, NCE, true, true, true>(tile, g.py, g.XBf, 1, g.XBf, -1.f, tid, tw, ld, st);
+    fft_io<NCF, NCE, true, true, true, WIDE>(tile, g.py, g.XBf, 1, g.XBf, -1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
@@ -91,10 +94,11 @@ struct NcRowFwdLoad {
   }
 };
 
+template <bool WIDE>
 __global__ __launch_bounds__(NCR) void k_nc_row_fwd(NcGeom g, const float2 *__restrict__ tmp,
                                                     float2 *__restrict__ part) {
   __shared__ float2 tile[NCE];
-  __shared__ float2 twl[kMaxFftLen];
+  __shared__ float2 twl[nc_table_len(WIDE)];
   const int tid = threadIdx.x;
   const int plane = blockIdx.y;
   const int t = plane / g.C;
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(NCR) void k_nc_row_fwd(NcGeom g, const float2 *__re
   __syncthreads();
   NcRowFwdLoad ld{tmp + ((long)plane * g.Gy + gy0) * g.W, g.W, Gx};
   NoStore ns;
-  fft_io<NCR, NCE, false, true, false>(tile, g.px, nr, Gx, 1, -1.f, tid, tw, ld, ns);
+  fft_io<NCR, NCE, false, true, false, WIDE>(tile, g.px, nr, Gx, 1, -1.f, tid, tw, ld, ns);
   // tile[b * Gx + i] holds grid row gy0 + b in natural order.
   const int *rp = g.rptr + (long)t * (g.Gy + 1);
   const int e0 = rp[gy0], e1 = rp[gy0 + nr];
@@ -218,10 +222,11 @@ struct NcColAdjStore {
   }
 };
 
+template <bool WIDE>
 __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__restrict__ z,
                                                     float2 *__restrict__ tmp) {
   __shared__ float2 tile[NCE];
-  __shared__ float2 twl[kMaxFftLen];
+  __shared__ float2 twl[nc_table_len(WIDE)];
   extern __shared__ int cps[];  // (XBa + 1) * Gy ints, sized at launch
   const int tid = threadIdx.x;
   const int plane = blockIdx.y;
@@ -247,7 +252,7 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
   for (int k = tid; k < ncell; k += NCC) tile[k] = ld(k % XB, k / XB, 0);
   __syncthreads();
   NoLoad nl;
-  fft_io<NCC, NCE, true, false, true>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
+  fft_io<NCC, NCE, true, false, true, WIDE>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
 }
 
 // ------------------------------------------------------------ K^H, row pass
@@ -256,8 +261,9 @@ struct NcRowAdjLoad {
   int Gx;
   __device__ float2 operator()(int b, int i, int) const { return tr[(unsigned)(b * Gx + i)]; }
 };
+template <int SLOTS>
 struct NcRowAdjStore {
-  float2 acc[kRowSlotsNc];
+  float2 acc[SLOTS];
   const float2 *bc;
   float2 *img;  // frame t
   const float *dxi, *dyi;
@@ -276,11 +282,12 @@ struct NcRowAdjStore {
   }
 };
 
+template <bool WIDE>
 __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__restrict__ tmp,
                                                     const float2 *__restrict__ b1,
                                                     float2 *__restrict__ img) {
   __shared__ float2 tile[NCE];
-  __shared__ float2 twl[kMaxFftLen];
+  __shared__ float2 twl[nc_table_len(WIDE)];
   const int tid = threadIdx.x;
   const int t = blockIdx.y;
   const int y0 = blockIdx.x * g.RBa;
@@ -289,9 +296,9 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
   const float2 *tw = stage_twiddles(g.px, twl, tid, NCR);
   __syncthreads();
   NcRowAdjLoad ld{nullptr, g.Gx};
-  NcRowAdjStore st;
+  NcRowAdjStore<row_slots_nc(WIDE)> st;
 #pragma unroll
-  for (int k = 0; k < kRowSlotsNc; ++k) st.acc[k] = czero();
+  for (int k = 0; k < row_slots_nc(WIDE); ++k) st.acc[k] = czero();
   st.img = img + t * HW;
   st.dxi = g.dxi;
   st.dyi = g.dyi;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
     ld.tr = tmp + (((long)t * g.C + c) * g.H + y0) * g.Gx;
     st.bc = b1 + c * HW;
     st.last = (c == g.C - 1);
-    fft_io<NCR, NCE, false, true, true>(tile, g.px, nr, g.Gx, 1, +1.f, tid, tw, ld, st);
+    fft_io<NCR, NCE, false, true, true, WIDE>(tile, g.px, nr, g.Gx, 1, +1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
@@ -314,11 +321,16 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
 
 void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub,
                             const float2 *b1, float2 *tmp, float2 *part, hipStream_t s) {
-  hipLaunchKernelGGL(k_nc_col_fwd, dim3((g.W + g.XBf - 1) / g.XBf, g.T, g.CG), dim3(NCF), 0, s, g,
-                     u, usub, b1, tmp);
+  const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG), gr((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C);
+  if (g.wide)
+    hipLaunchKernelGGL(k_nc_col_fwd<true>, gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
+  else
+    hipLaunchKernelGGL(k_nc_col_fwd<false>, gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
   AV_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_nc_row_fwd, dim3((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C), dim3(NCR), 0, s, g,
-                     (const float2 *)tmp, part);
+  if (g.wide)
+    hipLaunchKernelGGL(k_nc_row_fwd<true>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+  else
+    hipLaunchKernelGGL(k_nc_row_fwd<false>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
   AV_LAUNCH_CHECK();
 }
 
@@ -344,11 +356,16 @@ int launch_nc_samples(int mode, const NcGeom &g, const float2 *part, float2 *out
 void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float2 *tmp,
                        float2 *img, hipStream_t s) {
   const size_t cps_bytes = sizeof(int) * (size_t)(g.XBa + 1) * g.Gy;
-  hipLaunchKernelGGL(k_nc_col_adj, dim3((g.Gx + g.XBa - 1) / g.XBa, g.T * g.C), dim3(NCC),
-                     cps_bytes, s, g, z, tmp);
+  const dim3 gc((g.Gx + g.XBa - 1) / g.XBa, g.T * g.C), gr((g.H + g.RBa - 1) / g.RBa, g.T);
+  if (g.wide)
+    hipLaunchKernelGGL(k_nc_col_adj<true>, gc, dim3(NCC), cps_bytes, s, g, z, tmp);
+  else
+    hipLaunchKernelGGL(k_nc_col_adj<false>, gc, dim3(NCC), cps_bytes, s, g, z, tmp);
   AV_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_nc_row_adj, dim3((g.H + g.RBa - 1) / g.RBa, g.T), dim3(NCR), 0, s, g,
-                     (const float2 *)tmp, b1, img);
+  if (g.wide)
+    hipLaunchKernelGGL(k_nc_row_adj<true>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
+  else
+    hipLaunchKernelGGL(k_nc_row_adj<false>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
   AV_LAUNCH_CHECK();
 }
 

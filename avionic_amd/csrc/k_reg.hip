@@ -122,11 +122,13 @@ __global__ __launch_bounds__(RT) void k_reg_norm(Grid3 g, RegParams rp, double a
 
 // G* dual-residual l1 terms (tv.cpp:216-221, tgv2.cpp:146-161,
 // ictgv2.cpp:195-227): |khz - div1 y1| (+ sum_c |divsym1 y2 + y1|_c
-// + |div1 y1 - div2 y3| + sum_c |divsym2 y4 + y3|_c).
-template <int M>
+// + |div1 y1 - div2 y3| + sum_c |divsym2 y4 + y3|_c).  The duals are read in
+// their storage format, each component group decoded with its own projection
+// radius (y1: r1, y2: r2, y3: r3, y4: r4; ICTV keeps y1 and y3 only).
+template <int M, typename DT>
 __global__ __launch_bounds__(RT) void k_gstar_terms(Grid3 g, RegParams rp,
                                                     const float2 *__restrict__ khz,
-                                                    const float2 *__restrict__ y,
+                                                    const DT *__restrict__ y,
                                                     double *__restrict__ partial) {
   const long i = (long)blockIdx.x * RT + threadIdx.x;
   double acc = 0.0;
@@ -134,21 +136,23 @@ __global__ __launch_bounds__(RT) void k_gstar_terms(Grid3 g, RegParams rp,
     const Vox v = vox_of(g, i);
     const long N = g.N;
     const Inv3 d1{rp.idx, rp.idy, rp.idt}, d2{rp.idx2, rp.idy2, rp.idt2};
-    PlainAcc Y{y};
-    const float2 dv1 = div3(Y, 0, N, 2 * N, v, g, d1);
+    const DualAcc<DT> Y1{y, rp.r1}, Y2{y, rp.r2}, Y3{y, rp.r3}, Y4{y, rp.r4};
+    const unsigned ui = (unsigned)i;
+    const float2 dv1 = div3(Y1, 0, N, 2 * N, v, g, d1);
     acc += sqrt((double)cabs2(csub(khz[i], dv1)));
     if (M == kICTV) {  // ictv.cpp:178-183: + |div1 y1 - div2 y3|
-      const float2 dv3 = div3(Y, 3 * N, 4 * N, 5 * N, v, g, d2);
+      const float2 dv3 = div3(Y3, 3 * N, 4 * N, 5 * N, v, g, d2);
       acc += sqrt((double)cabs2(csub(dv1, dv3)));
     } else if (M != kTV) {
       float2 sd[3];
-      symdiv3(Y, 3 * N, N, v, g, d1, sd);
-      for (int c = 0; c < 3; ++c) acc += sqrt((double)cabs2(cadd(sd[c], y[c * N + i])));
+      symdiv3(Y2, 3 * N, N, v, g, d1, sd);
+      for (int c = 0; c < 3; ++c) acc += sqrt((double)cabs2(cadd(sd[c], Y1.at(c * N)(ui))));
       if (M == kICTGV2) {
-        const float2 dv3 = div3(Y, 9 * N, 10 * N, 11 * N, v, g, d2);
+        const float2 dv3 = div3(Y3, 9 * N, 10 * N, 11 * N, v, g, d2);
         acc += sqrt((double)cabs2(csub(dv1, dv3)));
-        symdiv3(Y, 12 * N, N, v, g, d2, sd);
-        for (int c = 0; c < 3; ++c) acc += sqrt((double)cabs2(cadd(sd[c], y[(9 + c) * N + i])));
+        symdiv3(Y4, 12 * N, N, v, g, d2, sd);
+        for (int c = 0; c < 3; ++c)
+          acc += sqrt((double)cabs2(cadd(sd[c], Y3.at((9 + c) * N)(ui))));
       }
     }
   }
@@ -329,14 +333,24 @@ void launch_reg_norm(int method, const Grid3 &g, const RegParams &rp, double alp
   AV_LAUNCH_CHECK();
 }
 
-void launch_gstar_terms(int method, const Grid3 &g, const RegParams &rp, const float2 *khz,
-                        const float2 *y, double *partial, hipStream_t s) {
+void launch_gstar_terms(int method, int ds, const Grid3 &g, const RegParams &rp,
+                        const float2 *khz, const void *y, double *partial, hipStream_t s) {
   check_grid(g);
   dim3 grid(nblk(g.N)), block(RT);
-  if (method == kTV) hipLaunchKernelGGL(k_gstar_terms<kTV>, grid, block, 0, s, g, rp, khz, y, partial);
-  else if (method == kTGV2) hipLaunchKernelGGL(k_gstar_terms<kTGV2>, grid, block, 0, s, g, rp, khz, y, partial);
-  else if (method == kICTGV2) hipLaunchKernelGGL(k_gstar_terms<kICTGV2>, grid, block, 0, s, g, rp, khz, y, partial);
-  else hipLaunchKernelGGL(k_gstar_terms<kICTV>, grid, block, 0, s, g, rp, khz, y, partial);
+#define AV_GS(M, DT) \
+  hipLaunchKernelGGL((k_gstar_terms<M, DT>), grid, block, 0, s, g, rp, khz, (const DT *)y, partial)
+#define AV_GS_M(DT)                                  \
+  do {                                               \
+    if (method == kTV) AV_GS(kTV, DT);               \
+    else if (method == kTGV2) AV_GS(kTGV2, DT);      \
+    else if (method == kICTGV2) AV_GS(kICTGV2, DT);  \
+    else AV_GS(kICTV, DT);                           \
+  } while (0)
+  if (ds == kDualF32) AV_GS_M(float2);
+  else if (ds == kDualBF16) AV_GS_M(bf16x2);
+  else AV_GS_M(q16x2);
+#undef AV_GS_M
+#undef AV_GS
   AV_LAUNCH_CHECK();
 }
 

@@ -17,15 +17,17 @@ __device__ inline int first_block(const Task &t) {
 }
 
 // Last task whose first block does not exceed b (tasks are in block order;
-// a task with no blocks in this family shares its start with the next one).
+// a task with no blocks in this family shares its start with the next one):
+// a binary search over the non-decreasing first blocks.
 template <int F>
 __device__ inline int find_task(const Task *__restrict__ tasks, int n, int b) {
-  int lo = 0;
-  for (int i = 1; i < n; ++i) {
-    if (first_block<F>(tasks[i]) <= b)
-      lo = i;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first_block<F>(tasks[mid]) <= b)
+      lo = mid;
     else
-      break;
+      hi = mid - 1;
   }
   return lo;
 }
@@ -102,22 +104,29 @@ __device__ __forceinline__ void col_adjoint_task(const Task &T, int lb, float2 *
 
 // DELTA = false: K of the extragradient image (dual update);
 // DELTA = true : K of ext - x (step rule, adapting tasks only).
+// LDS entries of a column kind's twiddle and rowmap tables: compile-time
+// plans are at most 256 long, narrow runtime plans 512, wide ones 1024.
+__host__ __device__ constexpr int col_table_len(int kind) {
+  return kind == 1 ? kColPlanMaxLen : (kind == 2 ? kMaxFftLen : kMaxFftLenNarrow);
+}
+
 template <bool DELTA, int KIND>
 __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ tasks, int n,
                                                       int blk0) {
-  constexpr bool CT = KIND != 0;
-  // compile-time plans: transforms of at most 256
+  constexpr bool CT = KIND == 1;
   __shared__ float2 tile[kColTileElems];
-  __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
-  __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
+  __shared__ float2 twl[col_table_len(KIND)];
+  __shared__ int rml[col_table_len(KIND)];
   const int blk = blk0 + (int)blockIdx.x;
   const Task &T = tasks[find_task<kCol>(tasks, n, blk)];
   if (col_kind(T.cg) != KIND) return;
   if (!task_active(T.sc)) return;
   if (DELTA && !task_adapt(T.sc)) return;
   const int lb = blk - T.b_col;
-  if constexpr (!CT) {
+  if constexpr (KIND == 0) {
     col_forward_task<DELTA, ColPlanRt>(T, lb, tile, twl, rml);
+  } else if constexpr (KIND == 2) {
+    col_forward_task<DELTA, ColPlanRtWide>(T, lb, tile, twl, rml);
   } else {
     switch (T.cg.cplan) {
       case kColPlan256: col_forward_task<DELTA, ColPlan256>(T, lb, tile, twl, rml); break;
@@ -130,18 +139,20 @@ __global__ __launch_bounds__(NTC) void kt_col_forward(const Task *__restrict__ t
   }
 }
 
-// Row pass.  CT = true: tasks whose row length has a compile-time plan
-// (16 or 32 rows per block), CT = false: the others (RB rows per block).
-template <int MODE, bool CT>
+// Row pass, one kernel per row kind (row_kind): 0 compile-time plans (16 or
+// 32 rows per block), 1 narrow runtime plans, 2 wide runtime plans (RB rows
+// per block).
+template <int MODE, int RK>
 __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int n, int blk0) {
+  constexpr bool CT = RK == 0;
   __shared__ float2 tile[CT ? kFftTileElems : RTILE];
   __shared__ float2 tile2[(MODE == kRowDualUpdate && !CT) ? RTILE : 1];
-  __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
+  __shared__ float2 twl[CT ? kColPlanMaxLen : (RK == 2 ? kMaxFftLen : kMaxFftLenNarrow)];
   __shared__ double red[NT];
   __shared__ int mrl[CT ? kRowPlanMaxRows : kRowMaxRows];
   const int blk = blk0 + (int)blockIdx.x;
   const Task &T = tasks[find_task<kRow>(tasks, n, blk)];
-  if ((T.cg.rplan != kRowPlanRt) != CT) return;
+  if (row_kind(T.cg) != RK) return;
   if (!task_active(T.sc)) return;
   if (MODE == kRowNorm2 && !task_adapt(T.sc)) return;
   const int lb = blk - T.b_row;
@@ -153,9 +164,9 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
       a = row_fused_ct<MODE, RowPlan128>(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, twl, mrl);
   } else {
     if (MODE == kRowDualUpdate)
-      row_dual_fused(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, tile2, twl, mrl);
+      row_dual_fused<RK == 2>(T.cg, lb, T.kcol, T.z, T.d, T.zhat, T.sc, tile, tile2, twl, mrl);
     else
-      a = row_norm2_fused(T.cg, lb, T.kcol, tile, twl, mrl);
+      a = row_norm2_fused<RK == 2>(T.cg, lb, T.kcol, tile, twl, mrl);
   }
   if (MODE == kRowNorm2) {
     const int tid = threadIdx.x;
@@ -177,17 +188,18 @@ __global__ __launch_bounds__(NT) void kt_row(const Task *__restrict__ tasks, int
 // for 4 spills).
 template <int KIND>
 __device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks, int n, int blk0) {
-  constexpr bool CT = KIND != 0;
   __shared__ float2 tile[kColTileElems];
-  __shared__ float2 twl[CT ? kColPlanMaxLen : kMaxFftLen];
-  __shared__ int rml[CT ? kColPlanMaxLen : kMaxFftLen];
+  __shared__ float2 twl[col_table_len(KIND)];
+  __shared__ int rml[col_table_len(KIND)];
   const int blk = blk0 + (int)blockIdx.x;
   const Task &T = tasks[find_task<kCol>(tasks, n, blk)];
   if (col_kind(T.cg) != KIND) return;
   if (!task_active(T.sc)) return;
   const int lb = blk - T.b_col;
-  if constexpr (!CT) {
+  if constexpr (KIND == 0) {
     col_adjoint_task<ColPlanRt>(T, lb, tile, twl, rml);
+  } else if constexpr (KIND == 2) {
+    col_adjoint_task<ColPlanRtWide>(T, lb, tile, twl, rml);
   } else {
     switch (T.cg.cplan) {
       case kColPlan256: col_adjoint_task<ColPlan256>(T, lb, tile, twl, rml); break;
@@ -199,9 +211,10 @@ __device__ __forceinline__ void col_adjoint_block(const Task *__restrict__ tasks
     }
   }
 }
+template <int KIND>
 __global__ __launch_bounds__(NTC) __attribute__((amdgpu_waves_per_eu(4)))
 void kt_col_adjoint_rt(const Task *__restrict__ tasks, int n, int blk0) {
-  col_adjoint_block<0>(tasks, n, blk0);
+  col_adjoint_block<KIND>(tasks, n, blk0);
 }
 template <int KIND>
 __global__ __launch_bounds__(NTC) void kt_col_adjoint_ct(const Task *__restrict__ tasks, int n,
@@ -281,8 +294,9 @@ static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipSt
   if constexpr (K < kColKinds) {
     const int nb = gr.col_hi[K] - gr.col_lo[K];
     if (nb > 0) {
-      if constexpr (K == 0)
-        hipLaunchKernelGGL(kt_col_adjoint_rt, dim3(nb), dim3(NTC), 0, s, d_tasks, n, gr.col_lo[K]);
+      if constexpr (K != 1)
+        hipLaunchKernelGGL((kt_col_adjoint_rt<K>), dim3(nb), dim3(NTC), 0, s, d_tasks, n,
+                           gr.col_lo[K]);
       else
         hipLaunchKernelGGL((kt_col_adjoint_ct<K>), dim3(nb), dim3(NTC), 0, s, d_tasks, n,
                            gr.col_lo[K]);
@@ -291,13 +305,14 @@ static void launch_col_adj(const TaskGrid &gr, const Task *d_tasks, int n, hipSt
   }
 }
 
-template <int MODE>
+template <int MODE, int RK = 0>
 static void launch_row(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
-  const int nct = gr.row_hi[0] - gr.row_lo[0], nrt = gr.row_hi[1] - gr.row_lo[1];
-  if (nct > 0)
-    hipLaunchKernelGGL((kt_row<MODE, true>), dim3(nct), dim3(NT), 0, s, d_tasks, n, gr.row_lo[0]);
-  if (nrt > 0)
-    hipLaunchKernelGGL((kt_row<MODE, false>), dim3(nrt), dim3(NT), 0, s, d_tasks, n, gr.row_lo[1]);
+  if constexpr (RK < kRowKinds) {
+    const int nb = gr.row_hi[RK] - gr.row_lo[RK];
+    if (nb > 0)
+      hipLaunchKernelGGL((kt_row<MODE, RK>), dim3(nb), dim3(NT), 0, s, d_tasks, n, gr.row_lo[RK]);
+    launch_row<MODE, RK + 1>(gr, d_tasks, n, s);
+  }
 }
 
 void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,

@@ -42,6 +42,7 @@ struct NcGeom {
   int RBf;            // grid rows per forward row-pass block
   int RBa;            // image rows per adjoint row-pass block
   int CG;             // coil groups of the forward column pass (blocks per tile)
+  int wide;           // a grid plan is wide (plan_is_wide): the wide kernel variants
   float scale;        // 1 / sqrt(HW)
   const float *dxi, *dyi;  // deapodisation 1 / psi_hat per image column / row
   const float *sw;         // [T][M] sqrt(density) / sqrt(HW)

@@ -151,7 +151,7 @@ class NoncartesianOperator : public Operator {
     const int Gx = (int)std::lround(osf * W), Gy = (int)std::lround(osf * H);
     AV_CHECK(Gx >= 4 && Gy >= 4, "oversampled grid too small (at least 4 x 4)");
     AV_CHECK(Gx <= kMaxFftLen && Gy <= kMaxFftLen,
-             "oversampled grid exceeds the longest FFT (512)");
+             "oversampled grid exceeds the longest FFT (1024)");
     const long G2 = (long)Gx * Gy;
     AV_CHECK((long)T * M * kNcTaps * kNcTaps < (1L << 31), "trajectory tables exceed 2^31 entries");
 
@@ -279,6 +279,7 @@ class NoncartesianOperator : public Operator {
 
     g_.W = W; g_.H = H; g_.C = C; g_.T = T; g_.M = (int)M;
     g_.Gx = Gx; g_.Gy = Gy;
+    g_.wide = (plan_is_wide(g_.px) || plan_is_wide(g_.py)) ? 1 : 0;
     g_.XBf = pick_cols(Gy, W);
     g_.XBa = pick_cols(Gy, Gx);
     g_.RBf = std::max(1, 4096 / Gx);

@@ -6,8 +6,11 @@
 // equal to the maximum and keeps that element and every later element
 // >= 0.9 * max.  Elements above the threshold that come before the first
 // maximum are not kept; this function keeps that behaviour, since it decides
-// which values the median is taken over.  The comparison is the reference's
-// float value against the double 0.9 * max; the median of an even count is
+// which values the median is taken over.  The threshold is the double product
+// 0.9 * max rounded to float, as the reference's comparator stores it
+// (isLargerThan holds a `const RType &` with RType = float,
+// raw_data_preparation.cpp:649-658), and the comparison is in float; the
+// median of an even count is
 // the mean of the two middle values (their float sum halved).  A median that
 // is not positive (all magnitudes zero) is an error here; the reference would
 // go on with an infinite factor.
@@ -38,11 +41,11 @@ extern "C" int avionic_normalization_factor(const float *magnitude, size_t n, fl
     avionic_set_error("avionic_normalization_factor: non-finite maximum");
     return AVIONIC_E_INVALID;
   }
-  const double thr = (double)mx * 0.9;
+  const float thr = (float)((double)mx * 0.9);
   std::vector<float> keep;
   keep.push_back(mx);
   for (size_t i = imax + 1; i < n; ++i)
-    if ((double)magnitude[i] >= thr) keep.push_back(magnitude[i]);
+    if (magnitude[i] >= thr) keep.push_back(magnitude[i]);
   const size_t mid = keep.size() / 2;
   std::nth_element(keep.begin(), keep.begin() + mid, keep.end());
   const float hi = keep[mid];

@@ -77,8 +77,9 @@ void launch_iter_bump(DevScalars *sc, hipStream_t s);
 void launch_reg_norm(int method, const Grid3 &g, const RegParams &rp, double alpha0, double alpha1,
                      double alpha, const float2 *x, double *partial, hipStream_t s);
 // G* dual residual terms: l1 norms of (khz - div1 y1), (y1 + divsym1 y2), ...
-void launch_gstar_terms(int method, const Grid3 &g, const RegParams &rp, const float2 *khz,
-                        const float2 *y, double *partial, hipStream_t s);
+// y in dual storage ds (DualStorage), decoded on the device.
+void launch_gstar_terms(int method, int ds, const Grid3 &g, const RegParams &rp,
+                        const float2 *khz, const void *y, double *partial, hipStream_t s);
 // sum over compact k-space of Re(conj(d) z) and |z|^2 (two partial arrays).
 void launch_kspace_dots(const float2 *d, const float2 *z, long n, double *partial, hipStream_t s);
 int reduce_blocks(long n);

@@ -87,6 +87,45 @@ int CartesianOperator::residual2(const float2 *u, const float2 *b1, const float2
   return plan_.row_blocks();
 }
 
+// ================================================================== gap
+
+size_t gap_partials(const Grid3 &g, const Operator &op) {
+  const int n = std::max({step_blocks(g), norm_blocks(g), op.max_partials(),
+                          reduce_blocks(op.compact_elems())});
+  return 2 * (size_t)n;
+}
+
+double regularizer_norm_of(const GapState &st, hipStream_t s) {
+  launch_reg_norm(st.method, st.g, st.rp, st.alpha0, st.alpha1, st.alpha, st.x, st.part_a, s);
+  return reduce_host(st.part_a, norm_blocks(st.g), 2, 0, s);
+}
+
+double data_fidelity_of(const GapState &st, hipStream_t s) {
+  const int nk = st.op->residual2(st.x, st.b1, st.d, st.part_b, s);
+  return st.lambda / 2.0 * std::sqrt(reduce_host(st.part_b, nk, 1, 0, s));
+}
+
+GapValues compact_gap(const GapState &st, hipStream_t s) {
+  // G* = lambda/2 ||K x - d||^2 + Re<d, z> + ||z||^2 / (2 lambda) + dual-residual l1 terms
+  const int nk = st.op->residual2(st.x, st.b1, st.d, st.part_b, s);
+  const double r2 = reduce_host(st.part_b, nk, 1, 0, s);
+  const long K = st.op->compact_elems();
+  launch_kspace_dots(st.d, st.z, K, st.part_a, s);
+  const int nb = reduce_blocks(K);
+  const double re_dz = reduce_host(st.part_a, nb, 2, 0, s);
+  const double zz = reduce_host(st.part_a, nb, 2, 1, s);
+  launch_gstar_terms(st.method, st.ds, st.g, st.rp, st.khz, st.y, st.part_a, s);
+  const double l1 = reduce_host(st.part_a, norm_blocks(st.g), 2, 0, s);
+  const double lam = st.lambda;
+  const double gstar = 0.5 * lam * r2 + re_dz + zz / (2.0 * lam) + l1;
+  const double r = regularizer_norm_of(st, s);
+  GapValues v;
+  v.gap = std::fabs(gstar + r) / (double)st.g.N;
+  v.norm = r;
+  v.fidelity = lam / 2.0 * std::sqrt(r2);
+  return v;
+}
+
 // ================================================================== Recon
 
 static double ellipke(double m) {
@@ -192,10 +231,9 @@ void Recon::d6(double out[6]) const {
 
 void Recon::ensure_partials() {
   const Grid3 g{W, H, T, N};
-  const int n = std::max({step_blocks(g), op->max_partials(), reduce_blocks(op->compact_elems()),
-                          reduce_blocks((long)C * N)});
-  part_a_.ensure(sizeof(double) * 2 * (size_t)n);
-  part_b_.ensure(sizeof(double) * 2 * (size_t)n);
+  const size_t n = std::max(gap_partials(g, *op), (size_t)2 * reduce_blocks((long)C * N));
+  part_a_.ensure(sizeof(double) * n);
+  part_b_.ensure(sizeof(double) * n);
   scal_.ensure(sizeof(DevScalars));
 }
 
@@ -281,7 +319,7 @@ double Recon::gstar_dense(const float2 *x1, const float2 *y, const float2 *z, co
   d6(dd);
   const RegParams rp = make_reg_params(method, dd, params.alpha0, params.alpha1, params.alpha);
   const Grid3 gr{W, H, T, N};
-  launch_gstar_terms(method, gr, rp, img.as<float2>(), y, part_b_.as<double>(), s);
+  launch_gstar_terms(method, kDualF32, gr, rp, img.as<float2>(), y, part_b_.as<double>(), s);
   g += reduce_pair(part_b_.as<double>(), norm_blocks(gr), 0, s);
   return g;
 }
@@ -350,13 +388,27 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
   norm_log.clear();
   fid_log.clear();
 
-  auto data_fidelity = [&]() {  // ictgv2.cpp:163-175: lambda/2 * ||K x - d||_2
-    const int nk = op->residual2(xs, b1, d.as<float2>(), part_b_.as<double>(), s);
-    return params.lambda / 2.0 * std::sqrt(reduce_host(part_b_.as<double>(), nk, 1, 0, s));
-  };
+  GapState gs;
+  gs.method = method;
+  gs.ds = dstore;
+  gs.g = g;
+  gs.rp = rp;
+  gs.alpha0 = params.alpha0;
+  gs.alpha1 = params.alpha1;
+  gs.alpha = params.alpha;
+  gs.lambda = params.lambda;
+  gs.op = op;
+  gs.x = xs;
+  gs.y = y.p;
+  gs.z = z.as<float2>();
+  gs.d = d.as<float2>();
+  gs.khz = khz.as<float2>();
+  gs.b1 = b1;
+  gs.part_a = part_a_.as<double>();
+  gs.part_b = part_b_.as<double>();
   if (method == kICTGV2) {
-    fid_log.push_back(data_fidelity());
-    norm_log.push_back(regularizer_norm(xs, s));
+    fid_log.push_back(data_fidelity_of(gs, s));
+    norm_log.push_back(regularizer_norm_of(gs, s));
   }
 
   unsigned k = 0;
@@ -378,52 +430,15 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
     const bool gap = (verbose && (k < 10 || k % 50 == 0)) || (debug && k % std::max(1, debugstep) == 0) ||
                      (params.stopPDGap > 0 && k % 20 == 0);
     if (gap) {
-      // gap on the compact state: G* + R(x), normalised by N
-      DevScalars h{};
-      AV_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
-      const int nk = op->residual2(xs, b1, d.as<float2>(), part_b_.as<double>(), s);
-      const double r2 = reduce_host(part_b_.as<double>(), nk, 1, 0, s);
-      launch_kspace_dots(d.as<float2>(), z.as<float2>(), K, part_a_.as<double>(), s);
-      const int nb = reduce_blocks(K);
-      const double re_dz = reduce_pair(part_a_.as<double>(), nb, 0, s);
-      const double zz = reduce_pair(part_a_.as<double>(), nb, 1, s);
-      const float2 *yf = y.as<float2>();
-      DevBuf yconv;
-      if (dstore != kDualF32) {
-        // decode the 4-byte duals on the host (diagnostic path only)
-        std::vector<uint16_t> hb((size_t)2 * D * N);
-        AV_HIP(hipMemcpyAsync(hb.data(), y.p, (size_t)4 * D * N, hipMemcpyDeviceToHost, s));
-        AV_HIP(hipStreamSynchronize(s));
-        std::vector<float2> hf((size_t)D * N);
-        const float radii[4] = {rp.r1, rp.r2, rp.r3, rp.r4};
-        for (size_t i = 0; i < hf.size(); ++i) {
-          if (dstore == kDualBF16) {
-            hf[i] = make_float2(bf16_to_f32(hb[2 * i]), bf16_to_f32(hb[2 * i + 1]));
-          } else {
-            // component groups: 3 (r1), 6 (r2), 3 (r3), 6 (r4)
-            const int c = (int)(i / (size_t)N);
-            const int grp = c < 3 ? 0 : c < 9 ? 1 : c < 12 ? 2 : 3;
-            const float sc16 = radii[grp] / kQ16Max;
-            hf[i] = make_float2((float)(int16_t)hb[2 * i] * sc16, (float)(int16_t)hb[2 * i + 1] * sc16);
-          }
-        }
-        yconv.ensure(sizeof(float2) * hf.size());
-        AV_HIP(hipMemcpyAsync(yconv.p, hf.data(), yconv.bytes, hipMemcpyHostToDevice, s));
-        yf = yconv.as<float2>();
-      }
-      launch_gstar_terms(method, g, rp, khz.as<float2>(), yf, part_a_.as<double>(), s);
-      const double l1 = reduce_pair(part_a_.as<double>(), norm_blocks(g), 0, s);
-      const double lam = h.lambda;
-      const double gstar = 0.5 * lam * r2 + re_dz + zz / (2.0 * lam) + l1;
-      const double r = regularizer_norm(xs, s);
-      const double pg = std::fabs(gstar + r) / (double)N;
-      pdgap_log.push_back(pg);
-      if (verbose) std::printf("Normalized Primal-Dual Gap after %u iterations: %.4e\n", k, pg);
+      gs.x = xs;
+      const GapValues gv = compact_gap(gs, s);
+      pdgap_log.push_back(gv.gap);
+      if (verbose) std::printf("Normalized Primal-Dual Gap after %u iterations: %.4e\n", k, gv.gap);
       if (method == kICTGV2 || method == kICTV) {  // ictv.cpp:347-355 too
-        norm_log.push_back(r);
-        fid_log.push_back(data_fidelity());
+        norm_log.push_back(gv.norm);
+        fid_log.push_back(gv.fidelity);
       }
-      if (params.stopPDGap > 0 && pg < params.stopPDGap) {
+      if (params.stopPDGap > 0 && gv.gap < params.stopPDGap) {
         ++k;
         break;
       }

@@ -99,6 +99,33 @@ class CartesianOperator : public Operator {
   void scratch();
 };
 
+// PD gap of a running reconstruction on its compact state: |G*(z, y) + R(x)|
+// normalised by the voxel count (tv.cpp:202-235, tgv2.cpp:130-176,
+// ictgv2.cpp:178-249 and the pdGap / N of ictgv2.cpp:466-467), with the
+// regulariser norm R(x) and the data fidelity lambda/2 ||K x - d||
+// (ictgv2.cpp:163-175) that the debug exports record beside it.  The duals
+// are read in their storage format on the device.
+struct GapValues {
+  double gap, norm, fidelity;
+};
+struct GapState {
+  int method = 0, ds = 0;
+  Grid3 g{};
+  RegParams rp{};
+  double alpha0 = 0, alpha1 = 0, alpha = 0, lambda = 0;
+  Operator *op = nullptr;
+  const float2 *x = nullptr;  // primal, P components
+  const void *y = nullptr;    // dual, D components in storage ds
+  const float2 *z = nullptr, *d = nullptr, *khz = nullptr, *b1 = nullptr;  // compact z, d; K^H z
+  double *part_a = nullptr, *part_b = nullptr;  // gap_partials() doubles each
+};
+size_t gap_partials(const Grid3 &g, const Operator &op);
+GapValues compact_gap(const GapState &st, hipStream_t s);
+// Regulariser norm R(x) (utils::TVNorm / TGV2Norm / ICTGV2Norm).
+double regularizer_norm_of(const GapState &st, hipStream_t s);
+// lambda/2 ||K x - d||_2 on the compact data (ictgv2.cpp:163-175).
+double data_fidelity_of(const GapState &st, hipStream_t s);
+
 // Primal-dual solver (PDRecon + the three regularisers).
 class Recon {
  public:

@@ -45,10 +45,10 @@ struct Task {
 struct TaskGrid {
   int vox = 0, col = 0, row = 0;
   // Blocks [lo, hi) spanning the tasks of each column kind (col_kind) and of
-  // each row kind (0: compile-time row plan, 1: runtime); the engine orders
-  // its tasks by kind, so each span holds only that kind's blocks.
+  // each row kind (row_kind); the engine orders its tasks by kind, so each
+  // span holds only that kind's blocks.
   int col_lo[kColKinds] = {}, col_hi[kColKinds] = {};
-  int row_lo[2] = {}, row_hi[2] = {};
+  int row_lo[kRowKinds] = {}, row_hi[kRowKinds] = {};
 };
 
 // Streams of one iteration: the regulariser kernels that do not depend on

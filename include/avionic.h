@@ -322,6 +322,16 @@ typedef struct avionic_request_desc {
   const float *x0;               /* device, [frames][height][width] or NULL */
   void *stream;                  /* stream that produced the inputs; submit
                                     returns once they are copied */
+  /* In-loop PD gap, as PDRecon's SetVerbose / SetDebug / stopPDGap
+   * (ictgv2.cpp:462-485): the gap is evaluated after iteration k when
+   * verbose and (k < 10 || k % 50 == 0), when debug and k % debugstep == 0,
+   * or when params->stopPDGap > 0 and k % 20 == 0; verbose prints it; the
+   * request finishes early once a gap falls below stopPDGap.  The gaps
+   * (with the regulariser norm and data fidelity) are kept for
+   * avionic_engine_get_gap_log.  Each evaluation synchronises the engine. */
+  int verbose;
+  int debug;
+  int debugstep;                 /* <= 0 reads as 1 */
 } avionic_request_desc;
 
 int avionic_engine_create(const avionic_engine_config *cfg,
@@ -355,6 +365,14 @@ int avionic_engine_time_kernels(avionic_engine *engine, int reps, double *ms,
  * available until the request is polled or cancelled. */
 int avionic_engine_get_step_log(const avionic_engine *engine, long long id,
                                 double *sigma, double *tau, size_t *n);
+/* PD gaps evaluated so far (see avionic_request_desc), each normalised by the
+ * voxel count, with the regulariser norm and the data fidelity
+ * lambda/2 ||K x - d|| at the same iterations (the reference's PDGap,
+ * ICTGVnorm and DATAfidelity exports, ictgv2.cpp:494-516); any of the
+ * arrays may be NULL.  n: capacity in, count out; available until the
+ * request is polled or cancelled. */
+int avionic_engine_get_gap_log(const avionic_engine *engine, long long id, double *gap,
+                               double *norm, double *fidelity, size_t *n);
 
 #ifdef __cplusplus
 }

@@ -24,9 +24,12 @@ def _run(args):
     return r
 
 
-def test_cli_cartesian_matches_library(gpu, tmp_path):
+@pytest.mark.parametrize("W,H,C,T", [(32, 24, 3, 4),
+                                     # the reference's cine geometry (demo_avionic_cine.sh:73-74:
+                                     # nX = 168 encodings, nY = 416 readouts): radix-13 columns
+                                     (168, 416, 3, 2)])
+def test_cli_cartesian_matches_library(gpu, tmp_path, W, H, C, T):
     rng = np.random.default_rng(5)
-    W, H, C, T = 32, 24, 3, 4
     k = (rng.standard_normal((T, C, H, W)) + 1j * rng.standard_normal((T, C, H, W))) * 10
     mask = (rng.random((T, H, 1)) < 0.4) * np.ones((1, 1, W))
     b1 = rng.standard_normal((C, H, W)) * 0.2 + 1.0 + 0.1j
@@ -50,9 +53,15 @@ def test_cli_cartesian_matches_library(gpu, tmp_path):
     rec.IterativeReconstruction(gpu._t(k * mask[:, None]), x, gpu._t(b1))
     ref = gpu._n(x)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
-    # -e writes the solver's extra results next to the output
+    # -e writes the solver's extra results next to the output: x3 (the w
+    # component) equals the library's export of the same run
     assert os.path.exists(tmp_path / "x3_component")
-    assert A.read_bin(tmp_path / "x3_component").size == W * H * T
+    x3 = A.read_bin(tmp_path / "x3_component")
+    assert x3.size == W * H * T
+    ex = {}
+    rec.ExportAdditionalResults("", lambda _d, name, arr: ex.__setitem__(name, arr.copy()))
+    ref3 = ex["x3_component"].astype(np.complex128).ravel()
+    np.testing.assert_allclose(x3.ravel(), ref3, rtol=0, atol=1e-5 * np.abs(ref3).max())
 
 
 def test_cli_radial_matches_library(gpu, tmp_path):

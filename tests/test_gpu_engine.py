@@ -105,6 +105,11 @@ PLAN_CASES = [  # method, T, H, W, C, maxIt, centered
     (2, 2, 96, 40, 2, 12, False),
     (2, 2, 64, 256, 3, 12, True),    # compile-time row plans (W 256 and 128)
     (0, 3, 40, 128, 2, 12, False),
+    # wide runtime plans beside the narrow ones: radix 13 columns (the reference's cine
+    # geometry, 416 rows x 168 columns), 768-long rows, radix 11 / 13 in both passes
+    (1, 2, 416, 168, 3, 12, False),
+    (0, 2, 96, 768, 2, 12, True),
+    (2, 2, 143, 176, 2, 12, False),
 ]
 
 

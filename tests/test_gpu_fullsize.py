@@ -1,0 +1,192 @@
+"""The benchmark workloads at their full sizes on the GPU.
+
+* NRMSE (matlab/myerror.m:68-70) of a whole reconstruction against the FP64
+  oracle's, stored by tests/golden/make_fullsize_fixtures.py: TV 128x128x12x8
+  (500 iterations), TGV2 256x256x25x12 and ICTGV2 256x256x30x16 (1000
+  iterations) through the batched engine with hipGraph replay, and radial TGV2
+  256x256x30x12 (500 iterations) through the PDRecon API; bar 1e-4.
+* The first three iterations (all step-rule iterations) of TV, TGV2 and radial
+  TGV2 at full size against the oracle run live, with the sigma / tau logs.
+* A window of the mixed serving stream (Poisson arrivals, TV / TGV2 / ICTGV2,
+  128..256, 8..30 frames, 8..20 coils) through the engine with continuous
+  admission: every result bitwise equal to the same request run alone, and the
+  TV / TGV2 requests of the window against the oracle.
+"""
+import numpy as np
+import pytest
+
+import fullsize as FS
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NRMSE_BAR = 1e-4
+
+
+def engine_run(gpu, case, ds, graphs=True, x0=True, maxit=None):
+    A = gpu.A
+    p = dict(case["product_params"], dualStorage=ds)
+    if maxit is not None:
+        p["maxIt"] = maxit
+    eng = A.Engine(use_graphs=graphs)
+    rid = eng.submit(A.Request(case["method"], gpu._t(case["data"]), gpu._t(case["b1"]),
+                               mask=gpu._r(case["mask"]), x0=gpu._t(case["x0"]) if x0 else None,
+                               params=p))
+    x = gpu._t(np.zeros(case["x0"].shape))
+    while not eng.poll(rid, x):
+        eng.step()
+    return gpu._n(x), eng
+
+
+@pytest.mark.parametrize("name,ds", [("tv_128x12x8", 0), ("tv_128x12x8", 2),
+                                     ("tgv2_256x25x12", 0), ("tgv2_256x25x12", 2),
+                                     ("ictgv2_256x30x16", 0), ("ictgv2_256x30x16", 2)])
+def test_cartesian_nrmse_vs_fp64_fixture(gpu, name, ds):
+    print(f"[gpu fullsize] {name} dualStorage={ds}", flush=True)
+    fx = FS.load_fixture(name)
+    case = FS.cartesian_case(name)
+    assert case["digest"] == fx["digest"], "inputs differ from the fixture's"
+    assert fx["iterations"] == case["iterations"]
+    x, _ = engine_run(gpu, case, ds)
+    e = FS.nrmse(x, fx["x"])
+    print(f"  NRMSE {e:.3e}", flush=True)
+    assert e <= NRMSE_BAR
+
+
+def radial_run(gpu, case, maxit):
+    A = gpu.A
+    T, H, W = case["x0"].shape
+    C = case["b1"].shape[0]
+    op = A.NoncartesianOperator(W, H, C, T, case["n_fe"], case["spokes"], gpu._r(case["traj"]),
+                                gpu._r(case["dcf"]), kernel_width=case["J"], osf=case["osf"])
+    rec = A.TGV2(W, H, C, T, op, None)
+    p = rec.GetParams()
+    p.update(case["product_params"])
+    p["maxIt"] = maxit
+    rec.SetParams(p)
+    x = gpu._t(case["x0"])
+    rec.IterativeReconstruction(gpu._t(case["data"]), x, gpu._t(case["b1"]))
+    return gpu._n(x), rec
+
+
+@pytest.fixture(scope="module")
+def radial_case():
+    return FS.radial_case(O)
+
+
+def test_radial_nrmse_vs_fp64_fixture(gpu, radial_case):
+    print(f"[gpu fullsize] {FS.RADIAL}", flush=True)
+    fx = FS.load_fixture(FS.RADIAL)
+    assert radial_case["digest"] == fx["digest"], "inputs differ from the fixture's"
+    x, _ = radial_run(gpu, radial_case, fx["iterations"])
+    e = FS.nrmse(x, fx["x"])
+    print(f"  NRMSE {e:.3e}", flush=True)
+    assert e <= NRMSE_BAR
+
+
+@pytest.mark.parametrize("name", ["tv_128x12x8", "tgv2_256x25x12"])
+def test_first_iterations_full_size_vs_oracle(gpu, name):
+    print(f"[gpu fullsize] 3 iterations of {name}", flush=True)
+    case = FS.cartesian_case(name)
+    ref = O.recon(FS.oracle_params(O, case, iterations=3), case["data"], case["b1"],
+                  case["mask"], case["x0"], logs=True)
+    x, _ = engine_run(gpu, case, 0, graphs=False, maxit=3)
+    assert FS.nrmse(x, ref["x"]) < 1e-5
+    A = gpu.A
+    eng = A.Engine(use_graphs=False)
+    rid = eng.submit(A.Request(case["method"], gpu._t(case["data"]), gpu._t(case["b1"]),
+                               mask=gpu._r(case["mask"]), x0=gpu._t(case["x0"]),
+                               params=dict(case["product_params"], maxIt=3)))
+    for _ in range(3):
+        eng.step()
+    s, t = eng.step_log(rid)
+    np.testing.assert_allclose(s, ref["sigma"], rtol=1e-5)
+    np.testing.assert_allclose(t, ref["tau"], rtol=1e-5)
+
+
+def test_first_iterations_radial_full_size_vs_oracle(gpu, radial_case):
+    print("[gpu fullsize] 3 iterations of radial TGV2", flush=True)
+    c = radial_case
+    p = FS.oracle_params(O, c, iterations=3)
+    ref = O.recon_nc(p, c["data"], c["b1"], c["traj"], c["dcf"], c["x0"], J=c["J"], osf=c["osf"],
+                     logs=True)
+    x, rec = radial_run(gpu, c, 3)
+    assert FS.nrmse(x, ref["x"]) < 1e-5
+    np.testing.assert_allclose(rec.sigma_log(), ref["sigma"], rtol=1e-5)
+    np.testing.assert_allclose(rec.tau_log(), ref["tau"], rtol=1e-5)
+
+
+def serving_window(seconds=0.2, iterations=60):
+    """Requests of the first `seconds` of the 50/s stream, inputs on the host
+    (rounded to what the GPU receives)."""
+    from avionic_amd import synth
+    stream = synth.serving_stream(50.0, seconds)
+    pool = synth.serving_pool()
+    rng = np.random.default_rng(17)
+    reqs = []
+    for (t_arr, method, n, T, C) in stream:
+        P = pool[n]
+        rho, b1 = P["rho"][:T], P["b1"][:C]
+        mask = P["masks"][method][:T]
+        data = synth.kspace(rho, b1, mask)
+        sig = 0.01 * np.abs(data).max()
+        data = data + sig * mask[:, None] * (rng.standard_normal(data.shape)
+                                             + 1j * rng.standard_normal(data.shape))
+        x0 = np.repeat(synth.initial_image(data, b1, mask)[None], T, axis=0)
+        c64 = lambda a: np.asarray(a, np.complex64).astype(np.complex128)
+        reqs.append(dict(t=t_arr, method=method, n=n, T=T, C=C, data=c64(data), b1=c64(b1),
+                         mask=mask.astype(np.float32).astype(np.float64), x0=c64(x0),
+                         iterations=iterations))
+    return reqs
+
+
+def test_serving_window_alone_equals_mix_and_oracle(gpu):
+    A = gpu.A
+    reqs = serving_window()
+    assert len(reqs) >= 8, len(reqs)
+    mcode = {"tv": 0, "tgv2": 1, "ictgv2": 2}
+
+    def request(r):
+        return A.Request(mcode[r["method"]], gpu._t(r["data"]), gpu._t(r["b1"]),
+                         mask=gpu._r(r["mask"]), x0=gpu._t(r["x0"]),
+                         params={"maxIt": r["iterations"]})
+
+    # continuous admission: request i arrives before engine step 4 i
+    eng = A.Engine(use_graphs=True)
+    ids, out, step = {}, {}, 0
+    pending = list(range(len(reqs)))
+    while pending or ids:
+        while pending and 4 * pending[0] <= step:
+            i = pending.pop(0)
+            print(f"[gpu serving] admit {i}: {reqs[i]['method']} n={reqs[i]['n']} "
+                  f"T={reqs[i]['T']} C={reqs[i]['C']}", flush=True)
+            rid = eng.submit(request(reqs[i]))
+            assert rid is not None
+            ids[rid] = i
+        eng.step()
+        step += 1
+        for rid, i in list(ids.items()):
+            x = gpu._t(np.zeros(reqs[i]["x0"].shape))
+            if eng.poll(rid, x):
+                out[i] = gpu._n(x)
+                del ids[rid]
+    assert sorted(out) == list(range(len(reqs)))
+    checked = 0
+    for i, r in enumerate(reqs):
+        alone = A.Engine(use_graphs=False, max_batch=1)
+        rid = alone.submit(request(r))
+        x = gpu._t(np.zeros(r["x0"].shape))
+        while not alone.poll(rid, x):
+            alone.step()
+        assert np.array_equal(gpu._n(x), out[i]), f"request {i} differs alone and in the mix"
+        if r["method"] != "ictgv2" and checked < 2 and r["n"] <= 192:
+            p = O.default_params(r["method"])
+            k, d = O.adapt_lambda_defaults(r["method"])
+            lam = O.adapt_lambda_cart(r["mask"], r["n"], r["n"], r["T"], k, d)
+            p.update(maxIt=r["iterations"], lambda_=lam)
+            ref = O.recon(p, r["data"], r["b1"], r["mask"], r["x0"])
+            e = FS.nrmse(out[i], ref)
+            print(f"  request {i} ({r['method']} {r['n']}) NRMSE vs oracle {e:.2e}", flush=True)
+            assert e < 1e-4
+            checked += 1
+    assert checked >= 1

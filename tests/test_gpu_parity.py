@@ -28,7 +28,11 @@ def line_mask(T, H, W, frac=0.3, seed=0):
 
 SIZES = [(5, 5, 2, 2), (6, 6, 2, 3), (7, 12, 3, 2), (32, 32, 3, 4), (128, 128, 2, 3),
          (160, 160, 2, 2), (192, 192, 1, 2), (224, 224, 1, 2), (256, 256, 2, 2), (384, 384, 1, 1),
-         (96, 80, 2, 3)]
+         (96, 80, 2, 3),
+         # radix-11 / -13 stages and lengths above 512 (the wide kernel variants): the
+         # reference's cine geometry 168 x 416 (cartesian_operator.cc:152-153,
+         # demo_avionic_cine.sh:73-74), 11 x 13 products, 768 / 1024 long rows and columns
+         (168, 416, 1, 3), (176, 143, 2, 2), (88, 208, 2, 2), (768, 96, 1, 1), (96, 1024, 1, 1)]
 
 
 @pytest.mark.parametrize("W,H,T,C", SIZES)
@@ -45,7 +49,8 @@ def test_operator_pair(gpu, W, H, T, C, centered):
         assert rel(got, ref) < 2e-6
 
 
-@pytest.mark.parametrize("W,H", [(5, 5), (6, 6), (10, 10), (12, 20), (128, 128), (256, 256), (384, 384), (160, 224)])
+@pytest.mark.parametrize("W,H", [(5, 5), (6, 6), (10, 10), (12, 20), (128, 128), (256, 256), (384, 384), (160, 224),
+                                 (168, 416), (143, 11), (1024, 768)])
 @pytest.mark.parametrize("sign", [-1, 1])
 @pytest.mark.parametrize("centered", [False, True])
 def test_fft2(gpu, W, H, sign, centered):

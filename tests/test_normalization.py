@@ -2,9 +2,10 @@
 raw_data_preparation.cpp:661-686; utils::Median, utils.cpp:1442-1462), on
 the host through the C-ABI.  The hand-worked cases fix the reference's
 selection: the walk starts at the first maximum, so values above 0.9 * max
-that come before it are not counted; the comparison is the float value
-against the double 0.9 * max; an even count takes the mean of the two middle
-values."""
+that come before it are not counted; the threshold is 0.9 * max computed in
+double and stored as float (the comparator's `const RType &`,
+raw_data_preparation.cpp:649-658), compared in float; an even count takes the
+mean of the two middle values."""
 import numpy as np
 import pytest
 
@@ -27,12 +28,16 @@ def test_hand_worked_cases(values, expected):
     assert A.normalization_factor(values) == pytest.approx(expected, rel=1e-7)
 
 
-def test_threshold_is_float_value_against_double_product():
-    # float32(0.9) = 0.89999998 < 0.9 * 1.0 in double: not counted, so the
-    # kept set is {1.0} alone; float32(0.90000004) is counted
-    assert A.normalization_factor([1.0, 0.9]) == 1.0
-    assert A.normalization_factor([1.0, np.nextafter(np.float32(0.9), np.float32(1))]) == \
-        pytest.approx((1.0 + 0.90000004) / 2, rel=1e-7)
+def test_threshold_is_the_float_rounded_product():
+    # 0.9 * 1.0 in double rounds to float32(0.9) = 0.89999998: an element
+    # exactly at that float threshold is kept (a double comparison would drop
+    # it); the next float below it is not
+    thr = np.float32(0.9 * 1.0)
+    assert A.normalization_factor([1.0, thr]) == \
+        pytest.approx(0.5 * float(np.float32(1.0) + thr), rel=1e-7)
+    below = np.nextafter(thr, np.float32(0))
+    assert A.normalization_factor([1.0, below]) == 1.0
+    assert float(thr) < 0.9  # the case where double and float comparisons differ
 
 
 def test_matches_sort_based_median_on_random_magnitudes():
@@ -40,7 +45,8 @@ def test_matches_sort_based_median_on_random_magnitudes():
     for n in (1, 2, 7, 64, 1000, 4097):
         v = np.abs(rng.standard_normal(n)).astype(np.float32)
         i = int(np.argmax(v))
-        kept = np.sort(v[i:][v[i:].astype(np.float64) >= float(v[i]) * 0.9])
+        thr = np.float32(float(v[i]) * 0.9)
+        kept = np.sort(v[i:][v[i:] >= thr])
         k = kept.size
         med = kept[k // 2] if k % 2 else np.float32(0.5 * float(kept[k // 2 - 1] + kept[k // 2]))
         assert A.normalization_factor(v) == np.float32(med)
