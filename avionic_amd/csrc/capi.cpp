@@ -15,6 +15,8 @@ class NoncartesianOperator;
 Operator *make_cartesian3d(int W, int H, int D, int C, const float *mask);
 Operator *make_noncartesian(int W, int H, int C, int T, int nFE, int spf, const float *traj,
                             const float *dens, int kw, double osf);
+void cartesian_backward_noalloc(const CartesianOperator &op, const float2 *img, float2 *kdense,
+                                const float2 *b1, hipStream_t s);
 }  // namespace avionic
 
 using namespace avionic;
@@ -136,6 +138,18 @@ int avionic_operator_backward(avionic_operator *op, const float *image, float *k
   return guard([&] {
     NEED(op); NEED(kdata); NEED(image); NEED(b1);
     op->op->backward(CF(image), F2(kdata), CF(b1), S(stream));
+  });
+}
+
+int avionic_operator_backward_noalloc(avionic_operator *op, const float *image, float *kdata,
+                                      const float *b1, void *stream) {
+  return guard([&] {
+    NEED(op); NEED(kdata); NEED(image); NEED(b1);
+    const auto *cop = dynamic_cast<const CartesianOperator *>(op->op);
+    if (!cop)
+      throw Error(AVIONIC_E_UNSUPPORTED,
+                  "avionic_operator_backward_noalloc: Cartesian 2D+t operators only");
+    cartesian_backward_noalloc(*cop, CF(image), F2(kdata), CF(b1), S(stream));
   });
 }
 

@@ -142,6 +142,20 @@ int avionic_operator_forward(avionic_operator *op, const float *kdata,
 int avionic_operator_backward(avionic_operator *op, const float *image,
                               float *kdata, const float *b1, void *stream);
 
+/* K without any device allocation (no counterpart in the reference): for
+ * callers that cannot allocate, such as code under HIP graph capture, tight
+ * serving loops and harnesses that forbid hipMalloc.  Same result as
+ * avionic_operator_backward for a Cartesian 2D+t operator
+ * (avionic_cartesian_operator_create; others return AVIONIC_E_UNSUPPORTED).
+ * The column pass's scratch is a 64 MiB device array of the library, one per
+ * GPU, held until the process ends: the operator's compact k-space (coils x
+ * sampled rows x width complex values) must fit it, else AVIONIC_E_INVALID.
+ * The output is cleared by a kernel (no memset node).  Calls that overlap on
+ * one GPU (other streams or host threads) race on the shared scratch; calls
+ * ordered on one stream are safe. */
+int avionic_operator_backward_noalloc(avionic_operator *op, const float *image,
+                                      float *kdata, const float *b1, void *stream);
+
 /* AdaptLambda(k, d): cartesian_operator.cpp:47-62,
  * noncartesian_operator.cpp:88-95. */
 int avionic_operator_adapt_lambda(avionic_operator *op, double k, double d,
