@@ -61,6 +61,7 @@ class _Params(ctypes.Structure):
         ("dx2", _D), ("dy2", _D), ("dt2", _D),
         ("adaptSchedule", _I),
         ("dualStorage", _I),
+        ("opnormIterations", ctypes.c_uint),
     ]
 
 
@@ -117,6 +118,7 @@ def _declare(L):
         "avionic_recon_get_params": (_I, [_P, ctypes.POINTER(_Params)]),
         "avionic_recon_set_params": (_I, [_P, ctypes.POINTER(_Params)]),
         "avionic_recon_set_verbose": (_I, [_P, _I]),
+        "avionic_recon_operator_norm": (_I, [_P, _P, _U, ctypes.POINTER(_D), _P]),
         "avionic_recon_set_debug": (_I, [_P, _I, _I]),
         "avionic_recon_adapt_lambda": (_I, [_P, _D, _D, ctypes.POINTER(_D)]),
         "avionic_recon_run": (_I, [_P, _P, _P, _P, _P]),
@@ -353,6 +355,15 @@ class PDRecon:
         """x is updated in place (initial image in, reconstruction out)."""
         _check(lib().avionic_recon_run(self._h, _c64(data), _c64(x), _c64(b1), _stream()))
         return x
+
+    def OperatorNorm(self, b1, iterations=50):
+        """||K_full|| of this solver's stacked regulariser + data operator by
+        an FP64 power iteration on the GPU (matlab/getopnorm.m); the fixed-step
+        mode adaptSchedule = 2 sets sigma = tau = 0.99 / ||K_full||."""
+        n = ctypes.c_double()
+        _check(lib().avionic_recon_operator_norm(self._h, _c64(b1), int(iterations),
+                                                 ctypes.byref(n), _stream()))
+        return n.value
 
     def AdaptStepSize(self, delta, b1):
         _check(lib().avionic_recon_adapt_step_size(self._h, _c64(delta), _c64(b1), _stream()))

@@ -237,6 +237,14 @@ int avionic_recon_run(avionic_recon *recon, const float *data, float *x, const f
   });
 }
 
+int avionic_recon_operator_norm(avionic_recon *recon, const float *b1, unsigned iterations,
+                                double *norm, void *stream) {
+  return guard([&] {
+    NEED(recon); NEED(b1); NEED(norm);
+    *norm = recon->r->operator_norm(CF(b1), (int)iterations, S(stream));
+  });
+}
+
 int avionic_recon_adapt_step_size(avionic_recon *recon, const float *delta, const float *b1,
                                   void *stream) {
   return guard([&] {

@@ -48,7 +48,7 @@ struct Slot {
   Task task{};
 
   bool adapts(int k) const {
-    return params.adaptSchedule == 1 || k < 10 || (k % 50) == 0;
+    return params.adaptSchedule == 1 || (params.adaptSchedule == 0 && (k < 10 || (k % 50) == 0));
   }
   // ictgv2.cpp:462-465
   bool wants_gap(int k) const {
@@ -261,6 +261,7 @@ avionic_params request_params(const avionic_request_desc *d, const Operator &op)
   }
   AV_CHECK(p.lambda > 0.0, "lambda must be positive");
   AV_CHECK(p.dualStorage >= 0 && p.dualStorage <= 2, "dualStorage must be 0, 1 or 2");
+  AV_CHECK(p.adaptSchedule >= 0 && p.adaptSchedule <= 2, "adaptSchedule must be 0, 1 or 2");
   return p;
 }
 
@@ -382,6 +383,18 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     AV_HIP(hipMemcpyAsync(s->ext.p, s->x.p, s->x.bytes, hipMemcpyDeviceToDevice, st));
     AV_HIP(hipMemcpyAsync(s->b1.p, d->b1, s->b1.bytes, hipMemcpyDeviceToDevice, st));
     s->op->gather(reinterpret_cast<const float2 *>(d->data), s->d.as<float2>(), st);
+    if (s->params.adaptSchedule == 2) {
+      // fixed steps sigma = tau = 0.99 / ||K_full|| (FP64 power iteration on
+      // the request's own operator and coil maps, ordered after their copy)
+      const double w6[6] = {s->params.dx, s->params.dy, s->params.dt,
+                            s->params.dx2, s->params.dy2, s->params.dt2};
+      const int its = s->params.opnormIterations ? (int)s->params.opnormIterations
+                                                 : kOpnormDefaultIterations;
+      const double nk = operator_norm_fp64(s->method, s->W, s->H, s->T, w6, *s->op,
+                                           s->b1.as<float2>(), its, st);
+      AV_CHECK(nk > 0.0, "operator norm is zero");
+      s->params.sigma = s->params.tau = 0.99 / nk;
+    }
     DevScalars hs{};
     hs.sigma = s->params.sigma;
     hs.tau = s->params.tau;

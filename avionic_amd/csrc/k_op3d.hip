@@ -174,6 +174,11 @@ class CartesianOperator3D : public Operator {
   }
   int max_partials() const override { return blocks_for(dense_elems()); }
   bool volumetric() const override { return true; }
+  bool khk_fp64(const double2 *u, double2 *out, const float2 *b1, Fp64Scratch &ws,
+                hipStream_t s) const override {
+    cart3_khk_fp64(W, H, T, C, mask_.as<float>(), u, out, b1, ws, s);
+    return true;
+  }
 
  private:
   DevBuf mask_, twx_, twy_, twz_, tmp_, ku_;

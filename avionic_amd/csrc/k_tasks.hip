@@ -47,7 +47,7 @@ __device__ inline int xcd_block(int b, int n) {
 __device__ inline bool task_active(const DevScalars *sc) { return sc->iter < sc->maxit; }
 __device__ inline bool task_adapt(const DevScalars *sc) {
   const int k = sc->iter;
-  return sc->schedule == 1 || k < 10 || (k % 50) == 0;
+  return sc->schedule == 1 || (sc->schedule == 0 && (k < 10 || (k % 50) == 0));
 }
 
 template <int M, typename DT>

@@ -190,6 +190,11 @@ class NoncartesianOperator : public Operator {
     std::vector<float4> rwx;
     std::vector<int> cptr((size_t)T * (G2 + 1));
     std::vector<int2> ce;
+    // double-precision copies for the FP64 power iteration (opnorm.hpp)
+    std::vector<double> w64((size_t)T * M * 2 * kNcTaps), sw64((size_t)T * M), ce_w64;
+    std::vector<double> dxi64(W), dyi64(H);
+    for (int i = 0; i < W; ++i) dxi64[i] = 1.0 / kb_ft((double)(i - W / 2) / Gx);
+    for (int i = 0; i < H; ++i) dyi64[i] = 1.0 / kb_ft((double)(i - H / 2) / Gy);
     rhdr.reserve((size_t)T * M * kNcTaps);
     rwx.reserve((size_t)T * M * kNcTaps);
     for (int t = 0; t < T; ++t) {
@@ -200,6 +205,7 @@ class NoncartesianOperator : public Operator {
         const double d = dn[p];
         AV_CHECK(std::isfinite(d) && d >= 0.0, "density weights must be finite and >= 0");
         sw[p] = (float)(std::sqrt(d) * sc);
+        sw64[p] = std::sqrt(d) * sc;
         const double sx = (double)tr[((size_t)t * 2 + 0) * M + m] * Gx;
         const double sy = (double)tr[((size_t)t * 2 + 1) * M + m] * Gy;
         AV_CHECK(std::isfinite(sx) && std::isfinite(sy), "trajectory must be finite");
@@ -211,6 +217,8 @@ class NoncartesianOperator : public Operator {
           wyd[m * kNcTaps + j] = kb(sy - (double)(qy[m] + j));
           w4x[j] = (float)wxd[m * kNcTaps + j];
           w4y[j] = (float)wyd[m * kNcTaps + j];
+          w64[p * 2 * kNcTaps + j] = wxd[m * kNcTaps + j];
+          w64[p * 2 * kNcTaps + kNcTaps + j] = wyd[m * kNcTaps + j];
         }
         q0[p] = make_int2((int)wrap(qx[m], Gx), (int)wrap(qy[m], Gy));
         wx4[p] = make_float4(w4x[0], w4x[1], w4x[2], w4x[3]);
@@ -244,6 +252,7 @@ class NoncartesianOperator : public Operator {
       cp[0] = (int)ce.size();
       for (long g = 0; g < G2; ++g) cp[g + 1] = cp[g] + count[g];
       ce.resize(cp[G2]);
+      ce_w64.resize(cp[G2]);
       std::vector<int> fill(G2, 0);
       for (long m = 0; m < M; ++m)
         for (int dy = 0; dy < kNcTaps; ++dy)
@@ -254,6 +263,7 @@ class NoncartesianOperator : public Operator {
             const int pos = cp[cell] + fill[cell]++;
             const float wf = (float)(w * std::sqrt((double)dn[(size_t)t * M + m]) * sc);
             ce[pos] = make_int2((int)m, __builtin_bit_cast(int, wf));
+            ce_w64[pos] = w * std::sqrt((double)dn[(size_t)t * M + m]) * sc;
           }
     }
 
@@ -269,6 +279,12 @@ class NoncartesianOperator : public Operator {
     up(rwx_, rwx.data(), sizeof(float4) * rwx.size());
     up(cptr_, cptr.data(), sizeof(int) * cptr.size());
     up(ce_, ce.data(), sizeof(int2) * ce.size());
+    up(q64_, q0.data(), sizeof(int2) * q0.size());
+    up(w64_, w64.data(), sizeof(double) * w64.size());
+    up(sw64_, sw64.data(), sizeof(double) * sw64.size());
+    up(cew64_, ce_w64.data(), sizeof(double) * ce_w64.size());
+    up(dxi64_, dxi64.data(), sizeof(double) * dxi64.size());
+    up(dyi64_, dyi64.data(), sizeof(double) * dyi64.size());
     make_fft_plan(Gx, g_.px, &twx_.p);
     make_fft_plan(Gy, g_.py, &twy_.p);
     // The tables come from pageable host memory: such a copy may return before
@@ -341,11 +357,20 @@ class NoncartesianOperator : public Operator {
                              partial, s);
   }
   int max_partials() const override { return nc_sample_blocks(g_); }
+  bool khk_fp64(const double2 *u, double2 *out, const float2 *b1, Fp64Scratch &ws,
+                hipStream_t s) const override {
+    Nc64 n{W, H, C, T, g_.M, g_.Gx, g_.Gy, q64_.as<int2>(), w64_.as<double>(),
+           sw64_.as<double>(), dxi64_.as<double>(), dyi64_.as<double>(), cptr_.as<int>(),
+           ce_.as<int2>(), cew64_.as<double>()};
+    nc_khk_fp64(n, u, out, b1, ws, s);
+    return true;
+  }
 
  private:
   int spf_;
   NcGeom g_{};
   DevBuf dxi_, dyi_, sw_, rptr_, rhdr_, rwx_, cptr_, ce_, twx_, twy_, tmp_, part_;
+  DevBuf q64_, w64_, sw64_, cew64_, dxi64_, dyi64_;  // FP64 tables
 
   void scratch() {
     const size_t planes = (size_t)T * C;

@@ -331,9 +331,9 @@ double Recon::pdgap_dense(const float2 *x, const float2 *y, const float2 *z, con
   return std::fabs(gs + r);
 }
 
-// IterativeReconstruction: tv.cpp:99-200, tgv2.cpp:180-338, ictgv2.cpp:294-492.
-void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s) {
-  // derivative weights from the time/space ratios (tv.cpp:103-104, ictgv2.cpp:300-306)
+// Derivative weights of a run: from the time/space ratios (tv.cpp:103-104,
+// ictgv2.cpp:300-306), dt as given for TVTEMP, (dx, dy, dz) for TGV2_3D.
+void Recon::prepare_weights() {
   double ds, dt;
   if (temporal) {
     // TVTEMP uses dt as configured, no time/space weights (tv_temp.cpp:98)
@@ -352,6 +352,26 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
              "ComputeTimeSpaceWeights: Invalid ratio.");
     params.dx2 = params.dy2 = ds;
     params.dt2 = dt;
+  }
+}
+
+double Recon::operator_norm(const float2 *b1, int iterations, hipStream_t s) {
+  prepare_weights();
+  double dd[6];
+  d6(dd);
+  return operator_norm_fp64(method, W, H, T, dd, *op, b1, iterations, s);
+}
+
+// IterativeReconstruction: tv.cpp:99-200, tgv2.cpp:180-338, ictgv2.cpp:294-492.
+void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s) {
+  prepare_weights();
+  AV_CHECK(params.adaptSchedule >= 0 && params.adaptSchedule <= 2, "adaptSchedule must be 0, 1 or 2");
+  if (params.adaptSchedule == 2) {
+    // fixed steps sigma = tau = 0.99 / ||K_full|| (FP64 power iteration)
+    const int its = params.opnormIterations ? (int)params.opnormIterations : kOpnormDefaultIterations;
+    const double nk = operator_norm(b1, its, s);
+    AV_CHECK(nk > 0.0, "operator norm is zero");
+    params.sigma = params.tau = 0.99 / nk;
   }
   AV_CHECK(params.lambda > 0.0, "lambda must be positive");
   ensure_partials();
@@ -419,7 +439,8 @@ void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s
     op->dual_update(es, b1, z.as<float2>(), d.as<float2>(), khz.as<float2>(), sc, s);
     // primal descent + extragradient
     launch_primal(method, dstore, g, rp, xs, es, y.p, khz.as<float2>(), sc, s);
-    const bool adapt = params.adaptSchedule == 1 || k < 10 || (k % 50 == 0);
+    const bool adapt = params.adaptSchedule == 1 ||
+                       (params.adaptSchedule == 0 && (k < 10 || k % 50 == 0));
     if (adapt) {
       adapted[k] = 1;
       launch_step_reg(method, g, rp, es, xs, part_a_.as<double>(), s);

@@ -12,6 +12,7 @@
 #include "../../include/avionic.h"
 #include "cartesian.hpp"
 #include "common.hpp"
+#include "opnorm.hpp"
 #include "reg.hpp"
 
 namespace avionic {
@@ -72,6 +73,13 @@ class Operator {
   virtual int max_partials() const = 0;
   // True for the volumetric operator (third axis = depth).
   virtual bool volumetric() const { return false; }
+  // out += K^H K u in double precision (operator-norm power iteration);
+  // false when the operator has no such path.
+  virtual bool khk_fp64(const double2 *u, double2 *out, const float2 *b1, Fp64Scratch &ws,
+                        hipStream_t s) const {
+    (void)u; (void)out; (void)b1; (void)ws; (void)s;
+    return false;
+  }
 };
 
 class CartesianOperator : public Operator {
@@ -91,6 +99,11 @@ class CartesianOperator : public Operator {
   int residual2(const float2 *u, const float2 *b1, const float2 *d, double *partial,
                 hipStream_t s) override;
   int max_partials() const override { return std::max(1, plan_.row_blocks()); }
+  bool khk_fp64(const double2 *u, double2 *out, const float2 *b1, Fp64Scratch &ws,
+                hipStream_t s) const override {
+    cart_khk_fp64(plan_.g, u, out, b1, ws, s);
+    return true;
+  }
   const CartPlan &plan() const { return plan_; }
 
  private:
@@ -144,6 +157,9 @@ class Recon {
   std::vector<float2> x3_host;
 
   void run(const float2 *data, float2 *x, const float2 *b1, hipStream_t s);
+  // ||K_full|| by the FP64 power iteration (adaptSchedule 2), with the
+  // derivative weights this solver uses.
+  double operator_norm(const float2 *b1, int iterations, hipStream_t s);
   void adapt_step(const float2 *delta, const float2 *b1, hipStream_t s);
   double gstar_dense(const float2 *x1, const float2 *y, const float2 *z, const float2 *data,
                      const float2 *b1, hipStream_t s);
@@ -154,6 +170,7 @@ class Recon {
   static void defaults(int method, avionic_params *p);
 
  private:
+  void prepare_weights();
   void d6(double out[6]) const;
   void ensure_partials();
   double reduce_pair(const double *partial, int n, int which, hipStream_t s);

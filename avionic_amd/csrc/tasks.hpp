@@ -9,7 +9,8 @@
 //
 // Per-task predicates evaluated on the device:
 //   active: sc->iter < sc->maxit        (every kernel; a finished task idles)
-//   adapt : schedule == 1 || iter < 10 || iter % 50 == 0
+//   adapt : schedule == 1 || (schedule == 0 && (iter < 10 || iter % 50 == 0));
+//           schedule 2 keeps fixed steps
 //           (step-size kernels; tv.cpp:172, tgv2.cpp:306, ictgv2.cpp:448)
 // so the same captured graph can be replayed for any mix of iteration counts.
 #pragma once

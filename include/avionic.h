@@ -77,13 +77,20 @@ typedef struct avionic_params {
   double timeSpaceWeight2;
   double dx2, dy2, dt2;
   int adaptSchedule;           /* 0: adapt when k<10 || k%50==0 (tv.cpp:172);
-                                  1: every iteration (tv_dmri.m:198, debug=1) */
+                                  1: every iteration (tv_dmri.m:198, debug=1);
+                                  2: fixed steps sigma = tau = 0.99 / ||K_full||
+                                     from an FP64 power iteration on the GPU
+                                     (matlab/getopnorm.m:1-16 on the stacked
+                                     regulariser + data operator), no
+                                     adaptation */
   int dualStorage;             /* 0: float32 duals; 1: bfloat16 duals;
                                   2: 16-bit fixed point scaled by each dual's
                                   projection radius.  1 and 2 store 4 bytes
                                   per complex value; arithmetic is float32
                                   (no counterpart in the reference, which
                                   keeps float32 duals) */
+  unsigned opnormIterations;   /* power-iteration steps of adaptSchedule 2
+                                  (0 selects 50) */
 } avionic_params;
 
 typedef struct avionic_operator avionic_operator;
@@ -220,6 +227,14 @@ int avionic_recon_compute_pdgap(avionic_recon *recon, const float *x,
                                 const float *y, const float *z,
                                 const float *data, const float *b1,
                                 double *pdgap, void *stream);
+
+/* ||K_full||, the norm of the stacked regulariser + data operator of this
+ * solver (its method, time/space weights and operator), by `iterations` steps
+ * of an FP64 power iteration on the GPU (matlab/getopnorm.m:1-16): the
+ * Rayleigh quotient sqrt(<K_full^* K_full x, x>) for unit x from a
+ * deterministic start.  Cartesian 2D+t and volumetric operators. */
+int avionic_recon_operator_norm(avionic_recon *recon, const float *b1, unsigned iterations,
+                                double *norm, void *stream);
 
 /* Step-size history of the last run: sigma/tau after each iteration.
  * n: capacity in, entries written out. */

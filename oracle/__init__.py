@@ -373,6 +373,43 @@ def recon(params: dict, data, b1, mask=None, x0=None, nthreads=0, logs=False):
                 gap=gaps[: ng.value])
 
 
+def opnorm(method, b1, mask, W, H, T, d6, iters, centered=False):
+    """||K_full|| after `iters` power-iteration steps (Cartesian 2D+t)."""
+    L = lib()
+    L.orc_opnorm.argtypes = [_I, _P, _P, _I, _I, _I, _I, _I, _P, _I]
+    L.orc_opnorm.restype = _D
+    b1 = _c(b1)
+    m = None if mask is None else _c(mask, np.float64)
+    d = np.ascontiguousarray(d6, np.float64)
+    return L.orc_opnorm(int(method), _ptr(b1), _ptr(m), W, H, b1.shape[0], T, int(bool(centered)),
+                        _ptr(d), int(iters))
+
+
+def opnorm3(b1, mask, W, H, D, d6, iters):
+    """||K_full|| of TGV2_3D on a volume (b1 [C][D][H][W])."""
+    L = lib()
+    L.orc_opnorm3.argtypes = [_P, _P, _I, _I, _I, _I, _P, _I]
+    L.orc_opnorm3.restype = _D
+    b1 = _c(b1)
+    m = None if mask is None else _c(mask, np.float64)
+    d = np.ascontiguousarray(d6, np.float64)
+    return L.orc_opnorm3(_ptr(b1), _ptr(m), W, H, D, b1.shape[0], _ptr(d), int(iters))
+
+
+def opnorm_nc(method, b1, traj, dcf, T, d6, iters, J=3, osf=1.5):
+    """||K_full|| with the non-Cartesian (gridding) data operator."""
+    L = lib()
+    L.orc_opnorm_nc.argtypes = [_I, _P, _I, _I, _I, _I, _I, _I, _D, _P, _P, _P, _I]
+    L.orc_opnorm_nc.restype = _D
+    b1 = _c(b1)
+    traj = _c(traj, np.float64)
+    w = None if dcf is None else _c(dcf, np.float64)
+    C, H, W = b1.shape
+    d = np.ascontiguousarray(d6, np.float64)
+    return L.orc_opnorm_nc(int(method), _ptr(b1), W, H, C, T, traj.shape[2], int(J), float(osf),
+                           _ptr(traj), _ptr(w), _ptr(d), int(iters))
+
+
 def fft3(a, sign):
     """Unnormalised 3D DFT of a [D][H][W] array (sign -1: e^{-i})."""
     a = _c(a).copy()

@@ -695,6 +695,80 @@ void orc_solver_adapt_step(int method, const double *deltar, const double *b1,
 }
 
 /* ======================================================================== */
+/* Operator norm by power iteration                                           */
+/* ======================================================================== */
+
+/* Deterministic start vector: entry j of the real array (real and imaginary
+ * parts in turn) is splitmix64(j + seed) mapped to [-1, 1). */
+static double hash_unit(unsigned long long j) {
+  unsigned long long z = j + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+/* M x = K_reg^* K_reg x + [K^H K u, 0, ...]: K_full^* K_full for the stacked
+ * operator [K_reg; K] with K_reg^* the adjoint of the primal descent (the
+ * pairing with off-diagonal weight 2). */
+static void opnorm_apply(int method, const dc *x, const double *b1, const orc_op *op,
+                         const double *d6, dc *out, dc *ky, dc *tmp, dc *kz, dc *img) {
+  long N = (long)op->W * op->H * op->T;
+  reg_K(method, x, op->W, op->H, op->T, d6, ky, tmp);
+  reg_KT(method, ky, op->W, op->H, op->T, d6, out);
+  op_K(op, (const double *)x, b1, (double *)kz);
+  op_KH(op, (const double *)kz, b1, (double *)img);
+  for (long i = 0; i < N; i++) out[i] += img[i];
+}
+
+/* ||K_full|| after `iters` power-iteration steps, following
+ * matlab/getopnorm.m:1-16 (x = y / |y|; y = M x; l = <y, x>), returned as
+ * sqrt(|l|) (getopnorm returns the Rayleigh quotient l itself). */
+static double opnorm_op(int method, const double *b1, const orc_op *op, const double *d6,
+                        int iters) {
+  long N = (long)op->W * op->H * op->T;
+  int P = orc_primal_components(method), D = orc_dual_components(method);
+  long n = (long)P * N;
+  dc *x = malloc(sizeof(dc) * n), *y = malloc(sizeof(dc) * n);
+  dc *ky = malloc(sizeof(dc) * D * N), *tmp = malloc(sizeof(dc) * N);
+  dc *kz = malloc(sizeof(dc) * (size_t)op->kn), *img = malloc(sizeof(dc) * N);
+  for (long i = 0; i < n; i++)
+    x[i] = hash_unit(2 * (unsigned long long)i) + I * hash_unit(2 * (unsigned long long)i + 1);
+  opnorm_apply(method, x, b1, op, d6, y, ky, tmp, kz, img);
+  double l = 0.0;
+  for (int it = 0; it < iters; it++) {
+    double n2 = 0.0;
+    for (long i = 0; i < n; i++) n2 += abs2(y[i]);
+    double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 1.0;
+    for (long i = 0; i < n; i++) x[i] = y[i] * inv;
+    opnorm_apply(method, x, b1, op, d6, y, ky, tmp, kz, img);
+    l = 0.0;
+    for (long i = 0; i < n; i++) l += creal(conj(y[i]) * x[i]);
+  }
+  free(x); free(y); free(ky); free(tmp); free(kz); free(img);
+  return sqrt(fabs(l));
+}
+
+double orc_opnorm(int method, const double *b1, const double *mask, int W, int H, int C, int T,
+                  int centered, const double *d6, int iters) {
+  orc_op op = {W, H, C, T, (long)C * W * H * T, centered, mask, 0, 0, 0, 0.0, NULL, NULL};
+  return opnorm_op(method, b1, &op, d6, iters);
+}
+
+double orc_opnorm3(const double *b1, const double *mask, int W, int H, int D, int C,
+                   const double *d6, int iters) {
+  orc_op op = {W, H, C, D, (long)C * W * H * D, 0, mask, 0, 0, 0, 0.0, NULL, NULL, 1};
+  return opnorm_op(ORC_TGV2, b1, &op, d6, iters);
+}
+
+double orc_opnorm_nc(int method, const double *b1, int W, int H, int C, int T, int M, int J,
+                     double osf, const double *traj, const double *dcf, const double *d6,
+                     int iters) {
+  orc_op op = {W, H, C, T, (long)C * M * T, 0, NULL, 1, M, J, osf, traj, dcf};
+  return opnorm_op(method, b1, &op, d6, iters);
+}
+
+/* ======================================================================== */
 /* Functionals and the primal-dual gap                                        */
 /* ======================================================================== */
 
@@ -929,7 +1003,7 @@ static int recon_core(const orc_params *p, const orc_op *op, const double *datar
       ext[i] = 2.0 * xn[i] - x[i];
       x[i] = xn[i];
     }
-    int adapt = (p->schedule == 1) || (k < 10) || (k % 50 == 0);
+    int adapt = (p->schedule == 1) || (p->schedule == 0 && ((k < 10) || (k % 50 == 0)));
     if (adapt) {
       for (long i = 0; i < (long)P * N; i++) delta[i] = ext[i] - x[i];
       adapt_step_op(method, delta, b1, op, d6, p->sigmaTauRatio, &sigma, &tau, NULL, NULL);

@@ -158,6 +158,18 @@ int orc_recon_nc(const orc_params *p, int W, int H, int C, int T, int M, int J,
                  double *x_out_all, double *sig_log, double *tau_log,
                  int nthreads);
 
+/* ||K_full|| of the stacked regulariser + data operator by power iteration
+ * (matlab/getopnorm.m:1-16 on K_full^* K_full): Cartesian 2D+t, volumetric
+ * (TGV2_3D) and non-Cartesian data operators.  d6 = {dx, dy, dt, dx2, dy2,
+ * dt2}. */
+double orc_opnorm(int method, const double *b1, const double *mask, int W, int H, int C, int T,
+                  int centered, const double *d6, int iters);
+double orc_opnorm3(const double *b1, const double *mask, int W, int H, int D, int C,
+                   const double *d6, int iters);
+double orc_opnorm_nc(int method, const double *b1, int W, int H, int C, int T, int M, int J,
+                     double osf, const double *traj, const double *dcf, const double *d6,
+                     int iters);
+
 #ifdef __cplusplus
 }
 #endif

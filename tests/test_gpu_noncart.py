@@ -115,7 +115,10 @@ def test_invalid_geometry_rejected(gpu):
     with pytest.raises(Exception):
         make_op(gpu, 32, 32, 1, 1, 32, 5, tr, dcf, 1, 2.0)  # beta^2 < 0 for J = 1
     with pytest.raises(Exception):
-        make_op(gpu, 400, 400, 1, 1, 32, 5, tr, dcf, 3, 1.5)  # grid 600 > 512
+        make_op(gpu, 700, 700, 1, 1, 32, 5, tr, dcf, 3, 1.5)  # grid 1050 > 1024
+    with pytest.raises(Exception):
+        make_op(gpu, 34, 34, 1, 1, 32, 5, tr, dcf, 2, 1.0)  # grid 34 = 2 x 17: no radix 17
+    make_op(gpu, 400, 400, 1, 1, 32, 5, tr, dcf, 3, 1.5)  # grid 600 = 2^3 3 5^2: accepted
 
 
 def _recon_gpu(gpu, method, p, data, b1, tr, dcf, n_fe, spf, J, osf):
