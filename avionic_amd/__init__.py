@@ -734,3 +734,112 @@ def engine_time_kernels(engine: Engine, reps=20):
     names = (ctypes.c_char_p * 16)()
     _check(L.avionic_engine_time_kernels(engine._h, int(reps), ms, names, ctypes.byref(n)))
     return {names[i].decode(): ms[i] for i in range(n.value)}
+
+
+# --------------------------------------------------------- coil construction
+class _CoilParams(ctypes.Structure):
+    _fields_ = [("uH1mu", _D), ("uReg", _D), ("uNrIt", _U), ("uSigma", _D), ("uTau", _D),
+                ("uAlpha0", _D), ("uAlpha1", _D), ("b1Reg", _D), ("b1NrIt", _U),
+                ("b1Sigma", _D), ("b1Tau", _D), ("b1FinalReg", _D), ("b1FinalNrIt", _U)]
+
+
+_COIL_SIGS = {
+    "avionic_default_coil_params": (_I, [ctypes.POINTER(_CoilParams)]),
+    "avionic_coil_time_averaged": (_I, [_P, _P, _P, _P, _I, _P]),
+    "avionic_coil_b1_from_uh1": (_I, [_U, _U, _U, _P, _P, _D, _P, _P]),
+    "avionic_coil_find_max_sum": (_I, [_U, _U, _U, _P, ctypes.POINTER(_I), _P]),
+    "avionic_coil_utgv2": (_I, [_U, _U, _U, _P, _P, ctypes.POINTER(_I), _I,
+                                ctypes.POINTER(_CoilParams), _P, _P]),
+    "avionic_coil_b1_recon": (_I, [_U, _U, _U, _P, _P, _U, _D, ctypes.POINTER(_CoilParams), _P,
+                                   _P]),
+    "avionic_coil_construction": (_I, [_P, _P, ctypes.POINTER(_CoilParams), _P, _P,
+                                       ctypes.POINTER(_I), _P]),
+}
+
+
+def _coil_lib():
+    L = lib()
+    if not getattr(L, "_coil_declared", False):
+        for name, (res, args) in _COIL_SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._coil_declared = True
+    return L
+
+
+def default_coil_params() -> dict:
+    """CoilConstruction::InitParams (coil_construction.cpp:22-46)."""
+    p = _CoilParams()
+    _check(_coil_lib().avionic_default_coil_params(ctypes.byref(p)))
+    return {name: getattr(p, name) for name, _ in _CoilParams._fields_}
+
+
+class CoilConstruction:
+    """CoilConstruction / CartesianCoilConstruction (CUDA/include/
+    coil_construction.h:58-141, cartesian_coil_construction.h): coil
+    sensitivities and an image estimate from k-space data.  Device tensors:
+    k-space [T, C, H, W], images [C, H, W] (one image [H, W])."""
+
+    def __init__(self, width, height, coils, frames, mrOp=None, params=None):
+        self.W, self.H, self.C, self.T = int(width), int(height), int(coils), int(frames)
+        self.op = mrOp
+        base = default_coil_params()
+        if params:
+            base.update(params)
+        self._p = _CoilParams(**{k: int(v) if k.endswith("NrIt") else float(v)
+                                 for k, v in base.items()})
+
+    def GetParams(self):
+        return {name: getattr(self._p, name) for name, _ in _CoilParams._fields_}
+
+    def _img(self, n):
+        import torch
+        return torch.zeros((n, self.H, self.W) if n > 1 else (self.H, self.W),
+                           dtype=torch.complex64, device="cuda")
+
+    def TimeAveragedReconstruction(self, kdata, applyPhase=True):
+        """(u [H, W], crec [C, H, W])."""
+        u, crec = self._img(1), self._img(self.C)
+        _check(_coil_lib().avionic_coil_time_averaged(self.op._h, _c64(kdata), _c64(u),
+                                                      _c64(crec), int(bool(applyPhase)),
+                                                      _stream()))
+        return u, crec
+
+    def B1FromUH1(self, u, crec):
+        b1 = self._img(crec.shape[0] if crec.dim() == 3 else 1)
+        C = crec.shape[0] if crec.dim() == 3 else 1
+        _check(_coil_lib().avionic_coil_b1_from_uh1(self.W, self.H, C, _c64(u), _c64(crec),
+                                                    float(self._p.uH1mu), _c64(b1), _stream()))
+        return b1
+
+    def FindMaximumSum(self, crec):
+        i = ctypes.c_int()
+        C = crec.shape[0] if crec.dim() == 3 else 1
+        _check(_coil_lib().avionic_coil_find_max_sum(self.W, self.H, C, _c64(crec),
+                                                     ctypes.byref(i), _stream()))
+        return i.value
+
+    def UTGV2Recon(self, b1, crec, x, inds):
+        """x [H, W] is updated in place."""
+        C = crec.shape[0] if crec.dim() == 3 else 1
+        arr = (ctypes.c_int * len(inds))(*[int(i) for i in inds])
+        _check(_coil_lib().avionic_coil_utgv2(self.W, self.H, C, _c64(b1), _c64(crec), arr,
+                                              len(inds), ctypes.byref(self._p), _c64(x),
+                                              _stream()))
+        return x
+
+    def B1Recon(self, u0, crec, coils, maxIt=500, mu=2.0):
+        b1 = self._img(int(coils))
+        _check(_coil_lib().avionic_coil_b1_recon(self.W, self.H, int(coils), _c64(u0), _c64(crec),
+                                                 int(maxIt), float(mu), ctypes.byref(self._p),
+                                                 _c64(b1), _stream()))
+        return b1
+
+    def PerformCoilConstruction(self, kdata):
+        """(u [C, H, W], b1 [C, H, W], coil order)."""
+        u, b1 = self._img(self.C), self._img(self.C)
+        order = (ctypes.c_int * self.C)()
+        _check(_coil_lib().avionic_coil_construction(self.op._h, _c64(kdata), ctypes.byref(self._p),
+                                                     _c64(u), _c64(b1), order, _stream()))
+        return u, b1, list(order)

@@ -4,8 +4,10 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/avionic.h"
+#include "coil.hpp"
 #include "solver.hpp"
 
 namespace avionic {
@@ -428,6 +430,115 @@ int avionic_fft2(float *data, unsigned height, unsigned width, unsigned batch, i
     const float sc = unitary ? (float)(1.0 / std::sqrt((double)width * height)) : 1.f;
     launch_fft2(F2(data), pw, ph, (int)height, (int)width, (int)batch, sign, centered, sc, S(stream));
     AV_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+// ------------------------------------------------------- coil construction
+static const Operator &coil_op(avionic_operator *op) {
+  NEED(op);
+  if (!dynamic_cast<const CartesianOperator *>(op->op) && !is_noncartesian(*op->op))
+    throw Error(AVIONIC_E_UNSUPPORTED,
+                "coil construction needs a Cartesian 2D+t or a non-Cartesian operator");
+  return *op->op;
+}
+static avionic_coil_params coil_params(const avionic_coil_params *p) {
+  avionic_coil_params d;
+  coil_default_params(&d);
+  return p ? *p : d;
+}
+
+int avionic_default_coil_params(avionic_coil_params *params) {
+  return guard([&] {
+    NEED(params);
+    coil_default_params(params);
+  });
+}
+
+int avionic_coil_time_averaged(avionic_operator *op, const float *kdata, float *u, float *crec,
+                               int apply_phase, void *stream) {
+  return guard([&] {
+    NEED(kdata); NEED(u); NEED(crec);
+    coil_time_averaged(coil_op(op), CF(kdata), F2(u), F2(crec), apply_phase != 0, S(stream));
+  });
+}
+
+int avionic_coil_b1_from_uh1(unsigned width, unsigned height, unsigned coils, const float *u,
+                             const float *crec, double mu, float *b1, void *stream) {
+  return guard([&] {
+    NEED(u); NEED(crec); NEED(b1);
+    AV_CHECK(width && height && coils, "empty image");
+    const CoilCtx k{(int)width, (int)height, (int)coils, (long)width * height, S(stream)};
+    coil_b1_from_uh1(k, CF(u), CF(crec), mu, F2(b1));
+  });
+}
+
+int avionic_coil_find_max_sum(unsigned width, unsigned height, unsigned coils, const float *crec,
+                              int *index, void *stream) {
+  return guard([&] {
+    NEED(crec); NEED(index);
+    AV_CHECK(width && height && coils, "empty image");
+    const CoilCtx k{(int)width, (int)height, (int)coils, (long)width * height, S(stream)};
+    *index = coil_find_max_sum(k, CF(crec));
+  });
+}
+
+int avionic_coil_utgv2(unsigned width, unsigned height, unsigned coils, const float *b1,
+                       const float *crec, const int *inds, int n_inds,
+                       const avionic_coil_params *params, float *x, void *stream) {
+  return guard([&] {
+    NEED(b1); NEED(crec); NEED(inds); NEED(x);
+    AV_CHECK(width && height && coils, "empty image");
+    AV_CHECK(n_inds >= 1 && n_inds <= (int)coils, "n_inds must be in 1..coils");
+    std::vector<int> v(inds, inds + n_inds);
+    for (int i : v) AV_CHECK(i >= 0 && i < (int)coils, "coil index out of range");
+    const CoilCtx k{(int)width, (int)height, (int)coils, (long)width * height, S(stream)};
+    coil_utgv2(k, CF(b1), CF(crec), v, coil_params(params), F2(x));
+  });
+}
+
+int avionic_coil_b1_recon(unsigned width, unsigned height, unsigned coils, const float *u0,
+                          const float *crec, unsigned maxIt, double mu,
+                          const avionic_coil_params *params, float *b1, void *stream) {
+  return guard([&] {
+    NEED(u0); NEED(crec); NEED(b1);
+    AV_CHECK(width && height && coils, "empty image");
+    AV_CHECK(mu > 0.0, "mu must be positive");
+    const CoilCtx k{(int)width, (int)height, (int)coils, (long)width * height, S(stream)};
+    coil_b1_recon(k, CF(u0), CF(crec), (int)coils, maxIt, mu, coil_params(params), F2(b1));
+  });
+}
+
+int avionic_coil_construction(avionic_operator *op, const float *kdata,
+                              const avionic_coil_params *params, float *u, float *b1, int *order,
+                              void *stream) {
+  return guard([&] {
+    NEED(kdata); NEED(u); NEED(b1);
+    const Operator &c = coil_op(op);
+    const CoilCtx k{c.W, c.H, c.C, (long)c.W * c.H, S(stream)};
+    DevBuf u0, crec;
+    u0.ensure(sizeof(float2) * k.N);
+    crec.ensure(sizeof(float2) * k.N * k.C);
+    coil_time_averaged(c, CF(kdata), u0.as<float2>(), crec.as<float2>(), true, k.s);
+    std::vector<int> ord;
+    coil_construction_from(k, crec.as<float2>(), u0.as<float2>(), coil_params(params), F2(u),
+                           F2(b1), &ord);
+    if (order)
+      for (int i = 0; i < k.C; ++i) order[i] = ord[i];
+  });
+}
+
+int avionic_coil_construction_from_images(unsigned width, unsigned height, unsigned coils,
+                                          const float *crec, const float *u0,
+                                          const avionic_coil_params *params, float *u,
+                                          float *b1, int *order, void *stream) {
+  return guard([&] {
+    NEED(crec); NEED(u0); NEED(u); NEED(b1);
+    AV_CHECK(width && height && coils, "empty image");
+    const CoilCtx k{(int)width, (int)height, (int)coils, (long)width * height, S(stream)};
+    std::vector<int> ord;
+    coil_construction_from(k, CF(crec), CF(u0), coil_params(params), F2(u), F2(b1), &ord);
+    if (order)
+      for (int i = 0; i < k.C; ++i) order[i] = ord[i];
   });
 }
 

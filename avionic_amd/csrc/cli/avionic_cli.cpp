@@ -223,6 +223,28 @@ void apply_cfg(const Options &o, const std::string &sec, avionic_params &p) {
   top("adaptlambda.d", p.adaptLambdaD);
 }
 
+// [coil] section of the parameter file (options_parser.cpp:173-189) over the
+// InitParams defaults; coil.uSigmaTauRatio / b1SigmaTauRatio are accepted and,
+// as in the reference, only InitParams applies them.
+avionic_coil_params coil_params_of(const Options &o) {
+  avionic_coil_params cp;
+  check(avionic_default_coil_params(&cp), "coil parameters");
+  auto get = [&](const char *k, double &dst) {
+    auto it = o.cfg.find(std::string("coil.") + k);
+    if (it != o.cfg.end()) dst = std::atof(it->second.c_str());
+  };
+  auto geti = [&](const char *k, unsigned &dst) {
+    auto it = o.cfg.find(std::string("coil.") + k);
+    if (it != o.cfg.end()) dst = (unsigned)std::atol(it->second.c_str());
+  };
+  get("uH1mu", cp.uH1mu); get("uReg", cp.uReg); geti("uNrIt", cp.uNrIt);
+  get("uTau", cp.uTau); get("uSigma", cp.uSigma); get("uAlpha0", cp.uAlpha0);
+  get("uAlpha1", cp.uAlpha1); get("b1Reg", cp.b1Reg); geti("b1NrIt", cp.b1NrIt);
+  get("b1Tau", cp.b1Tau); get("b1Sigma", cp.b1Sigma); get("b1FinalReg", cp.b1FinalReg);
+  geti("b1FinalNrIt", cp.b1FinalNrIt);
+  return cp;
+}
+
 struct HostVec {
   std::vector<double> re, im;
   bool complex = false;
@@ -285,6 +307,32 @@ std::vector<float> rss_of(size_t C, size_t HW, const std::function<void(size_t, 
   return u;
 }
 
+// Root-sum-of-squares of the time-averaged coil images of radial data
+// (NoncartesianCoilConstruction::TimeAveragedReconstruction without phase,
+// noncartesian_coil_construction.cpp:21-123) from the sqrt(density)-weighted
+// data: every frame's points of a coil gridded together, kernel width 3,
+// grid oversampling 2 (:84-86).
+std::vector<float> radial_tar_rss(unsigned W, unsigned H, unsigned C, unsigned T, unsigned nfe,
+                                  unsigned spf, const std::vector<float> &kdh,
+                                  const std::vector<float> &maskh, const std::vector<float> &wh) {
+  const size_t HW = (size_t)W * H;
+  Dev dtraj(sizeof(float) * maskh.size()), dw(sizeof(float) * wh.size()), dk(sizeof(float) * kdh.size()),
+      du(sizeof(float) * 2 * HW), dcrec(sizeof(float) * 2 * HW * C);
+  upload(dtraj, maskh);
+  upload(dw, wh);
+  upload(dk, kdh);
+  avionic_operator *op = nullptr;
+  check(avionic_noncartesian_operator_create(W, H, C, T, nfe, spf, dtraj.f(), dw.f(), 3, 2.0, &op),
+        "time-averaging operator");
+  check(avionic_coil_time_averaged(op, dk.f(), du.f(), dcrec.f(), 0, nullptr), "time-averaged images");
+  std::vector<float> u(2 * HW), rss(HW);
+  if (hipMemcpy(u.data(), du.p, sizeof(float) * u.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    die("device-to-host copy failed", 1);
+  avionic_operator_destroy(op);
+  for (size_t i = 0; i < HW; ++i) rss[i] = u[2 * i];
+  return rss;
+}
+
 // Normalisation factor of -o (main.cpp:575-579), computed from the prepared
 // data.  Cartesian (RawDataPreparation::NormalizeCartData,
 // raw_data_preparation.cpp:821-844): the time-averaged coil images
@@ -330,28 +378,9 @@ float datanorm_of(bool nonuniform, unsigned W, unsigned H, unsigned C, unsigned 
       download(img);
     });
   } else {
-    // every frame's points as one frame: trajectory [2][T*M], density [T*M]
-    const size_t P = (size_t)T * M;
-    std::vector<float> traj(2 * P), zc(2 * P);
-    for (size_t t = 0; t < T; ++t)
-      for (size_t m = 0; m < M; ++m) {
-        traj[t * M + m] = maskh[2 * t * M + m];
-        traj[P + t * M + m] = maskh[2 * t * M + M + m];
-      }
-    Dev dtraj(sizeof(float) * traj.size()), dw(sizeof(float) * wh.size()), dz(sizeof(float) * zc.size());
-    upload(dtraj, traj);
-    upload(dw, wh);
-    check(avionic_noncartesian_operator_create(W, H, 1, 1, nfe, spf * T, dtraj.f(), dw.f(), 3, 2.0, &op),
-          "normalisation operator");
-    u = rss_of(C, HW, [&](size_t c, std::vector<float> &img) {
-      for (size_t t = 0; t < T; ++t)
-        std::memcpy(&zc[2 * t * M], &kdh[2 * (t * C + c) * M], sizeof(float) * 2 * M);
-      upload(dz, zc);
-      check(avionic_operator_forward(op, dz.f(), dimg.f(), dones.f(), nullptr), "time-averaged image");
-      download(img);
-    });
+    u = radial_tar_rss(W, H, C, T, nfe, spf, kdh, maskh, wh);
   }
-  avionic_operator_destroy(op);
+  if (op) avionic_operator_destroy(op);
   float median = 0.f;
   check(avionic_normalization_factor(u.data(), u.size(), &median), "normalisation factor");
   return nonuniform ? 255.f * (float)T / median : 255.f / median;
@@ -403,6 +432,9 @@ int main(int argc, char **argv) {
   if (o.cfg.count("gpunufft.kernelWidth")) kw = (int)std::atof(o.cfg["gpunufft.kernelWidth"].c_str());
   if (o.cfg.count("gpunufft.osf")) osf = std::atof(o.cfg["gpunufft.osf"].c_str());
 
+  if (vol && o.sens.empty())  // main.cpp:621-626
+    die("coil construction for 3D reconstruction is not implemented: provide the coil maps (-s)", 3);
+
   // ---- inputs
   HostVec kd = load(o.kdata, "k-space data");
   if (kd.n != K) die("k-space data has " + std::to_string(kd.n) + " values, dims imply " + std::to_string(K));
@@ -413,11 +445,13 @@ int main(int argc, char **argv) {
         " values, dims imply " + std::to_string(mask_n));
   HostVec b1, u0, w;
   const bool have_b1 = !o.sens.empty();
-  if (!have_b1)
-    die("coil maps (-s) are required: the coil-map estimation is not part of this build", 3);
-  b1 = load(o.sens, "coil maps");
   const size_t B1N = vol ? N * C : HW * C;
-  if (b1.n != B1N) die("coil maps have " + std::to_string(b1.n) + " values, dims imply " + std::to_string(B1N));
+  if (have_b1) {
+    b1 = load(o.sens, "coil maps");
+    if (b1.n != B1N)
+      die("coil maps have " + std::to_string(b1.n) + " values, dims imply " + std::to_string(B1N));
+  }
+  const avionic_coil_params coilp = coil_params_of(o);
   if (!o.u0.empty()) {
     u0 = load(o.u0, "initial image");
     if (vol ? u0.n != N : (u0.n != HW && u0.n != N))
@@ -437,13 +471,14 @@ int main(int argc, char **argv) {
         "\"sigma\": %g, \"tau\": %g, \"sigmaTauRatio\": %g, \"timeSpaceWeight\": %g, "
         "\"timeSpaceWeight2\": %g, \"lambda\": %g, \"adaptLambda\": %d, \"adaptLambdaK\": %g, "
         "\"adaptLambdaD\": %g, \"alpha0\": %g, \"alpha1\": %g, \"alpha\": %g, "
-        "\"kernelWidth\": %d, \"osf\": %g, \"initial_image\": \"%s\", \"normalize\": %s, "
-        "\"output\": \"%s\"}\n",
+        "\"kernelWidth\": %d, \"osf\": %g, \"initial_image\": \"%s\", \"coil_maps\": \"%s\", "
+        "\"normalize\": %s, \"output\": \"%s\"}\n",
         o.method.c_str(), W, H, C, T, o.nonuniform ? "true" : "false", M, p.maxIt, p.stopPDGap,
         p.sigma, p.tau, p.sigmaTauRatio, p.timeSpaceWeight, p.timeSpaceWeight2, p.lambda,
         p.adaptLambda, p.adaptLambdaK, p.adaptLambdaD, p.alpha0, p.alpha1, p.alpha, kw, osf,
-        o.u0.empty() ? (o.nonuniform ? "zero" : "time-averaged") : "file",
-        o.normalize ? "true" : "false", o.output.c_str());
+        !o.u0.empty() ? "file" : (vol ? "zero" : (have_b1 ? "time-averaged" : "coil-construction")),
+        have_b1 ? "file" : "coil-construction", o.normalize ? "true" : "false",
+        o.output.c_str());
     return 0;
   }
 
@@ -496,7 +531,7 @@ int main(int argc, char **argv) {
       db1(sizeof(float) * 2 * B1N), dx(sizeof(float) * 2 * N), dw(sizeof(float) * (wh.size() + 1));
   upload(dk, kdh);
   upload(dmask, maskh);
-  upload(db1, to_c64(b1));
+  if (have_b1) upload(db1, to_c64(b1));
   if (!wh.empty()) upload(dw, wh);
 
   avionic_operator *op = nullptr;
@@ -509,44 +544,82 @@ int main(int argc, char **argv) {
   else
     check(avionic_cartesian_operator_create(W, H, C, T, dmask.f(), 0, &op), "Cartesian operator");
 
-  // ---- initial image: the file (one frame replicated, main.cpp:772-778), or
-  // for Cartesian data the time-averaged zero-filled image
-  // u0 = sum_c conj(b1_c) F(sum_t d_t / max(1, sum_t M_t)) (get_sos_crec_opt.m:7-15)
-  std::vector<float> x0(2 * N, 0.f);
+  // ---- coil maps and initial image (main.cpp:592-654).  Both derive from
+  // the time-averaged coil images crec_c: Cartesian, the uncentred masked
+  // operator's TimeAveragedReconstruction (cartesian_coil_construction.cpp
+  // :23-87, the operator of main.cpp:199-200); radial, every frame's points
+  // gridded together (noncartesian_coil_construction.cpp:21-123).  The
+  // library's avionic_coil_time_averaged computes both.
+  //   no -s: coil construction (coil_construction.cpp:447-557); u0 = the
+  //     image estimate of the last coil step (main.cpp:639), divided by the
+  //     frame count for radial data (:641-644); u0_reconstructed.bin and
+  //     b1_reconstructed.bin are written next to the output (:645-648);
+  //   -s without -u: u0 = (|crec| + sum_c conj(b1_c) crec_c) / 2, the
+  //     root-sum-of-squares without phase (PerformInitalizationGivenB1,
+  //     main.cpp:250-287; the reference builds its radial operator there for
+  //     Cartesian data too, here Cartesian data keeps its own operator);
+  //   -s and -u: u0 from the file, one frame replicated (main.cpp:772-778).
+  std::vector<float> x0(2 * N, 0.f), u1(2 * HW, 0.f);
+  const bool need_tar = !vol && (!have_b1 || o.u0.empty());
+  std::vector<float> crec_h, tu_h(2 * HW);
+  Dev dcrec(sizeof(float) * 2 * HW * C), dtu(sizeof(float) * 2 * HW);
+  if (need_tar) {
+    // PerformCoilConstruction takes the images with phase, GivenB1 without
+    check(avionic_coil_time_averaged(op, dk.f(), dtu.f(), dcrec.f(), have_b1 ? 0 : 1, nullptr),
+          "time-averaged reconstruction");
+    crec_h.resize(2 * HW * C);
+    if (hipMemcpy(crec_h.data(), dcrec.p, sizeof(float) * crec_h.size(), hipMemcpyDeviceToHost) !=
+            hipSuccess ||
+        hipMemcpy(tu_h.data(), dtu.p, sizeof(float) * tu_h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      die("device-to-host copy failed", 1);
+  }
+  if (!have_b1) {
+    Dev du(sizeof(float) * 2 * HW * C);
+    check(avionic_coil_construction_from_images(W, H, C, dcrec.f(), dtu.f(), &coilp, du.f(), db1.f(),
+                                                nullptr, nullptr),
+          "coil construction");
+    std::vector<float> uall(2 * HW * C), b1h(2 * HW * C);
+    if (hipMemcpy(uall.data(), du.p, sizeof(float) * uall.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(b1h.data(), db1.p, sizeof(float) * b1h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      die("device-to-host copy failed", 1);
+    const float s = o.nonuniform ? 1.f / (float)T : 1.f;
+    for (size_t i = 0; i < 2 * HW; ++i) u1[i] = uall[2 * (C - 1) * HW + i] * s;
+    const std::string dir = dir_of(o.output);
+    auto write_c = [&](const char *name, const std::vector<float> &v) {
+      const size_t n = v.size() / 2;
+      std::vector<double> re(n), im(n);
+      for (size_t i = 0; i < n; ++i) {
+        re[i] = v[2 * i];
+        im[i] = v[2 * i + 1];
+      }
+      check(avionic_bin_write((dir + "/" + name).c_str(), re.data(), im.data(), n), name);
+    };
+    write_c("u0_reconstructed.bin", u1);
+    write_c("b1_reconstructed.bin", b1h);
+    if (o.verbose) std::printf("avionic: coil maps estimated (%u coils)\n", C);
+  } else if (!vol && o.u0.empty()) {
+    const std::vector<float> bh = to_c64(b1);
+    for (size_t i = 0; i < HW; ++i) {
+      double re = 0.0, im = 0.0;
+      for (size_t c = 0; c < C; ++c) {
+        const double br = bh[2 * (c * HW + i)], bi = bh[2 * (c * HW + i) + 1];
+        const double cr = crec_h[2 * (c * HW + i)], ci = crec_h[2 * (c * HW + i) + 1];
+        re += br * cr + bi * ci;  // conj(b1) crec
+        im += br * ci - bi * cr;
+      }
+      u1[2 * i] = (float)(0.5 * (tu_h[2 * i] + re));
+      u1[2 * i + 1] = (float)(0.5 * (tu_h[2 * i + 1] + im));
+    }
+  }
   if (vol) {
     // the volume from the file (main.cpp:770-773), or zero
     if (!o.u0.empty()) x0 = to_c64(u0);
-  } else if (!o.u0.empty()) {
+  } else if (have_b1 && !o.u0.empty()) {
     const std::vector<float> u = to_c64(u0);
     for (size_t t = 0; t < T; ++t)
       for (size_t i = 0; i < 2 * HW; ++i) x0[2 * t * HW + i] = u[(u0.n == N ? 2 * t * HW : 0) + i];
-  } else if (!o.nonuniform) {
-    std::vector<float> avg(2 * HW * C, 0.f), cnt(HW, 0.f);
-    for (size_t t = 0; t < T; ++t)
-      for (size_t i = 0; i < HW; ++i) cnt[i] += maskh[t * HW + i];
-    for (size_t t = 0; t < T; ++t)
-      for (size_t c = 0; c < C; ++c)
-        for (size_t i = 0; i < HW; ++i) {
-          avg[2 * (c * HW + i)] += kdh[2 * ((t * C + c) * HW + i)];
-          avg[2 * (c * HW + i) + 1] += kdh[2 * ((t * C + c) * HW + i) + 1];
-        }
-    for (size_t c = 0; c < C; ++c)
-      for (size_t i = 0; i < HW; ++i) {
-        const float s = 1.f / std::max(1.f, cnt[i]);
-        avg[2 * (c * HW + i)] *= s;
-        avg[2 * (c * HW + i) + 1] *= s;
-      }
-    avionic_operator *op1 = nullptr;
-    check(avionic_cartesian_operator_create(W, H, C, 1, nullptr, 0, &op1), "averaging operator");
-    Dev davg(sizeof(float) * avg.size()), du(sizeof(float) * 2 * HW);
-    upload(davg, avg);
-    check(avionic_operator_forward(op1, davg.f(), du.f(), db1.f(), nullptr), "initial image");
-    std::vector<float> u(2 * HW);
-    if (hipMemcpy(u.data(), du.p, sizeof(float) * u.size(), hipMemcpyDeviceToHost) != hipSuccess)
-      die("device-to-host copy failed", 1);
-    avionic_operator_destroy(op1);
-    for (size_t t = 0; t < T; ++t)
-      std::memcpy(&x0[2 * t * HW], u.data(), sizeof(float) * 2 * HW);
+  } else {
+    for (size_t t = 0; t < T; ++t) std::memcpy(&x0[2 * t * HW], u1.data(), sizeof(float) * 2 * HW);
   }
   upload(dx, x0);
 

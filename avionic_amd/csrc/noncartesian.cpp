@@ -140,7 +140,7 @@ class NoncartesianOperator : public Operator {
  public:
   NoncartesianOperator(int W_, int H_, int C_, int T_, int nFE, int spf, const float *traj,
                        const float *dens, int J, double osf)
-      : spf_(spf) {
+      : spf_(spf), nfe_(nFE) {
     W = W_; H = H_; C = C_; T = T_;
     AV_CHECK(W > 0 && H > 0 && C > 0 && T > 0 && nFE > 0 && spf > 0,
              "invalid operator dimensions");
@@ -180,6 +180,8 @@ class NoncartesianOperator : public Operator {
     std::vector<float> tr((size_t)T * 2 * M), dn((size_t)T * M, 1.f);
     AV_HIP(hipMemcpy(tr.data(), traj, sizeof(float) * tr.size(), hipMemcpyDeviceToHost));
     if (dens) AV_HIP(hipMemcpy(dn.data(), dens, sizeof(float) * dn.size(), hipMemcpyDeviceToHost));
+    traj_h_ = tr;
+    dens_h_ = dn;
     const double sc = 1.0 / std::sqrt((double)W * H);
 
     std::vector<float> sw((size_t)T * M);
@@ -366,8 +368,16 @@ class NoncartesianOperator : public Operator {
     return true;
   }
 
+  // The trajectory and density as given (host copies) and the geometry, for
+  // the all-frames operator of the time-averaged coil images.
+  int spokes_per_frame() const { return spf_; }
+  int readout_points() const { return nfe_; }
+  const std::vector<float> &traj_host() const { return traj_h_; }
+  const std::vector<float> &dens_host() const { return dens_h_; }
+
  private:
-  int spf_;
+  int spf_, nfe_;
+  std::vector<float> traj_h_, dens_h_;
   NcGeom g_{};
   DevBuf dxi_, dyi_, sw_, rptr_, rhdr_, rwx_, cptr_, ce_, twx_, twy_, tmp_, part_;
   DevBuf q64_, w64_, sw64_, cew64_, dxi64_, dyi64_;  // FP64 tables
@@ -388,6 +398,35 @@ class NoncartesianOperator : public Operator {
 Operator *make_noncartesian(int W, int H, int C, int T, int nFE, int spf, const float *traj,
                             const float *dens, int kw, double osf) {
   return new NoncartesianOperator(W, H, C, T, nFE, spf, traj, dens, kw, osf);
+}
+
+// One coil, one frame holding every frame's points (frame-major), kernel
+// width 3 and grid oversampling 2: the operator of NoncartesianCoilConstruction::
+// TimeAveragedReconstruction (noncartesian_coil_construction.cpp:26-86).
+// nullptr when `op` is not a non-Cartesian operator.
+bool is_noncartesian(const Operator &op) {
+  return dynamic_cast<const NoncartesianOperator *>(&op) != nullptr;
+}
+
+Operator *make_noncartesian_all_frames(const Operator &op) {
+  const auto *nc = dynamic_cast<const NoncartesianOperator *>(&op);
+  if (!nc) return nullptr;
+  const int T = nc->T, spf = nc->spokes_per_frame(), nFE = nc->readout_points();
+  const size_t M = (size_t)spf * nFE, P = (size_t)T * M;
+  const std::vector<float> &tr = nc->traj_host();  // [T][2][M]
+  std::vector<float> all(2 * P);                  // [2][T*M]
+  for (size_t t = 0; t < (size_t)T; ++t)
+    for (size_t m = 0; m < M; ++m) {
+      all[t * M + m] = tr[2 * t * M + m];
+      all[P + t * M + m] = tr[2 * t * M + M + m];
+    }
+  DevBuf dtr, dd;
+  dtr.ensure(sizeof(float) * all.size());
+  dd.ensure(sizeof(float) * P);
+  AV_HIP(hipMemcpy(dtr.p, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice));
+  AV_HIP(hipMemcpy(dd.p, nc->dens_host().data(), sizeof(float) * P, hipMemcpyHostToDevice));
+  return new NoncartesianOperator(nc->W, nc->H, 1, 1, nFE, spf * T, dtr.as<float>(), dd.as<float>(),
+                                  3, 2.0);
 }
 
 }  // namespace avionic

@@ -298,6 +298,81 @@ int avionic_fft2(float *data, unsigned height, unsigned width, unsigned batch,
                  int sign, int centered, int unitary, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Coil-sensitivity estimation (CUDA/include/coil_construction.h:58-141,      */
+/* CUDA/src/coil_construction.cpp, cartesian_coil_construction.cpp)          */
+/* ------------------------------------------------------------------------ */
+/* Images are device arrays [coils][height][width] (one [height][width] image
+ * for u0).  Arithmetic is float as in the reference. */
+
+/* CoilConstructionParams (coil_construction.h:18-41). */
+typedef struct avionic_coil_params {
+  double uH1mu;        /* H1 weight of B1FromUH1 */
+  double uReg;         /* TGV weight nu of UTGV2Recon */
+  unsigned uNrIt;      /* UTGV2Recon iterations */
+  double uSigma, uTau, uAlpha0, uAlpha1;
+  double b1Reg;        /* mu of the per-coil B1Recon */
+  unsigned b1NrIt;
+  double b1Sigma, b1Tau;
+  double b1FinalReg;   /* mu of the final B1Recon over all coils */
+  unsigned b1FinalNrIt;
+} avionic_coil_params;
+
+/* CoilConstruction::InitParams (coil_construction.cpp:22-46). */
+int avionic_default_coil_params(avionic_coil_params *params);
+
+/* TimeAveragedReconstruction.  Cartesian 2D+t operator
+ * (CartesianCoilConstruction, cartesian_coil_construction.cpp:23-87): crec_c =
+ * the image transform of sum_t kdata[t][c] / max(1, sum_t M_t) with the
+ * operator's mask and centring; kdata [frames][coils][h][w].  Non-Cartesian
+ * operator (NoncartesianCoilConstruction, noncartesian_coil_construction.cpp
+ * :21-123): crec_c = the adjoint gridding of every frame's points of coil c
+ * together, with the operator's trajectory and density, kernel width 3 and
+ * grid oversampling 2 whatever the operator's own; kdata [frames][coils][M],
+ * sqrt(density)-weighted as the operator takes it.  Both: u = sqrt(sum_c
+ * |crec_c|^2), with the phase of sum_c crec_c when apply_phase. */
+int avionic_coil_time_averaged(avionic_operator *op, const float *kdata, float *u, float *crec,
+                               int apply_phase, void *stream);
+/* CoilConstruction::B1FromUH1 (coil_construction.cpp:363-446): per coil the
+ * solution of (mu |u|^2 + L) x = mu conj(u) |crec_c| (L the periodic 2D
+ * Laplacian; conjugate gradients, squared-residual tolerance 1e-12, at most
+ * 800 iterations), returned as |x_c| / sqrt(sum_c |x_c|^2). */
+int avionic_coil_b1_from_uh1(unsigned width, unsigned height, unsigned coils, const float *u,
+                             const float *crec, double mu, float *b1, void *stream);
+/* CoilConstruction::FindMaximumSum (coil_construction.cpp:52-73). */
+int avionic_coil_find_max_sum(unsigned width, unsigned height, unsigned coils, const float *crec,
+                              int *index, void *stream);
+/* CoilConstruction::UTGV2Recon (coil_construction.cpp:75-216): params->uNrIt
+ * PD iterations of min_u nu TGV2(u) + 1/2 sum_{c in inds} |b1_c u - crec_c|^2
+ * from u = x (in/out).  inds: host array of coil indices into b1 / crec. */
+int avionic_coil_utgv2(unsigned width, unsigned height, unsigned coils, const float *b1,
+                       const float *crec, const int *inds, int n_inds,
+                       const avionic_coil_params *params, float *x, void *stream);
+/* CoilConstruction::B1Recon (coil_construction.cpp:217-362): maxIt PD
+ * iterations per coil of min_b mu/2 |grad b|^2 + 1/2 |b u_n - crec_c|^2,
+ * u_n = u0 / max |u0|, from b = 0, then normalised to unit sum of squares
+ * over the coils.  b1 (out): [coils][h][w]. */
+int avionic_coil_b1_recon(unsigned width, unsigned height, unsigned coils, const float *u0,
+                          const float *crec, unsigned maxIt, double mu,
+                          const avionic_coil_params *params, float *b1, void *stream);
+/* CoilConstruction::PerformCoilConstruction (coil_construction.cpp:447-557)
+ * on a Cartesian 2D+t or non-Cartesian operator, from its time-averaged coil
+ * images (avionic_coil_time_averaged, with phase): u (out) [coils][h][w] holds the image
+ * estimate after each coil step (the reference's CLI takes the last one as
+ * u0, main.cpp:639), b1 (out) the coil maps; order (host, optional, `coils`
+ * ints) the greedy coil order.  params NULL selects the defaults. */
+int avionic_coil_construction(avionic_operator *op, const float *kdata,
+                              const avionic_coil_params *params, float *u, float *b1, int *order,
+                              void *stream);
+/* PerformCoilConstruction from given time-averaged images: crec
+ * [coils][h][w] and u0 [h][w] as TimeAveragedReconstruction returns them
+ * (the non-Cartesian path of the reference grids every frame's points into
+ * one image per coil, noncartesian_coil_construction.cpp:21-123). */
+int avionic_coil_construction_from_images(unsigned width, unsigned height, unsigned coils,
+                                          const float *crec, const float *u0,
+                                          const avionic_coil_params *params, float *u,
+                                          float *b1, int *order, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Raw-data normalisation (host memory; no device needed)                    */
 /* ------------------------------------------------------------------------ */
 /* RawDataPreparation::FindNormalizationFactor (raw_data_preparation.cpp

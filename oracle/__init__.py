@@ -410,6 +410,140 @@ def opnorm_nc(method, b1, traj, dcf, T, d6, iters, J=3, osf=1.5):
                            _ptr(traj), _ptr(w), _ptr(d), int(iters))
 
 
+# ------------------------------------------------------- coil construction
+COIL_DEFAULTS = dict(uH1mu=1e-5, uReg=0.2, uNrIt=100, uSigma=np.sqrt(1.0 / 12.0),
+                     uTau=np.sqrt(1.0 / 12.0), uAlpha0=np.sqrt(2.0), uAlpha1=1.0, b1Reg=2.0,
+                     b1NrIt=500, b1Sigma=np.sqrt(1.0 / 8.0), b1Tau=np.sqrt(1.0 / 8.0),
+                     b1FinalReg=0.1, b1FinalNrIt=1000)
+_COIL_ORDER = ["uH1mu", "uReg", "uNrIt", "uSigma", "uTau", "uAlpha0", "uAlpha1", "b1Reg",
+               "b1NrIt", "b1Sigma", "b1Tau", "b1FinalReg", "b1FinalNrIt"]
+
+
+def _coil_lib():
+    L = lib()
+    if not hasattr(L, "_coil_declared"):
+        L.orc_laplace_periodic.argtypes = [_P, _I, _I, _P]
+        L.orc_coil_tar.argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P]
+        L.orc_coil_b1_from_uh1.argtypes = [_P, _P, _I, _I, _I, _D, _P]
+        L.orc_coil_b1_from_uh1.restype = _I
+        L.orc_coil_find_max_sum.argtypes = [_P, _I, _I, _I]
+        L.orc_coil_find_max_sum.restype = _I
+        L.orc_coil_utgv2.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P]
+        L.orc_coil_b1_recon.argtypes = [_P, _P, _I, _I, _I, _I, _D, _P, _P]
+        L.orc_coil_construction.argtypes = [_P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P]
+        L.orc_coil_construction_from.argtypes = [_P, _P, _I, _I, _I, _P, _P, _P, _P]
+        L._coil_declared = True
+    return L
+
+
+def laplace_periodic(u):
+    u = _c(u)
+    out = np.zeros_like(u)
+    _coil_lib().orc_laplace_periodic(_ptr(u), u.shape[1], u.shape[0], _ptr(out))
+    return out
+
+
+def coil_tar(kdata, mask=None, centered=False, apply_phase=True):
+    """TimeAveragedReconstruction: kdata [T][C][H][W] -> (u [H][W], crec [C][H][W])."""
+    kdata = _c(kdata)
+    T, C, H, W = kdata.shape
+    m = None if mask is None else _c(mask, np.float64)
+    u = np.zeros((H, W), np.complex128)
+    crec = np.zeros((C, H, W), np.complex128)
+    _coil_lib().orc_coil_tar(_ptr(kdata), _ptr(m), W, H, C, T, int(bool(centered)),
+                             int(bool(apply_phase)), _ptr(u), _ptr(crec))
+    return u, crec
+
+
+def coil_tar_nc(kdata, traj, dcf, shape, apply_phase=True):
+    """NoncartesianCoilConstruction::TimeAveragedReconstruction
+    (noncartesian_coil_construction.cpp:21-123): kdata [T][C][M]
+    (sqrt(density)-weighted, as the operator takes it), traj [T][2][M],
+    dcf [T][M] or None, shape (H, W).  Every frame's points of a coil are
+    gridded together by the adjoint NUFFT (nufft_KH on one frame holding all
+    T*M points, kernel width 3, grid oversampling 2, :26-86) into crec_c;
+    u = sqrt(sum_c |crec_c|^2), with the phase of sum_c crec_c when
+    apply_phase (:108-123).  Returns (u [H][W], crec [C][H][W])."""
+    kdata = _c(kdata)
+    traj = _c(traj, np.float64)
+    T, C, M = kdata.shape
+    H, W = shape
+    tr = np.stack([traj[:, 0, :].ravel(), traj[:, 1, :].ravel()])[None]
+    w = None if dcf is None else _c(dcf, np.float64).reshape(1, T * M)
+    ones = np.ones((1, H, W), np.complex128)
+    crec = np.stack([nufft_KH(kdata[:, c, :].reshape(1, 1, T * M), ones, tr, w, J=3, osf=2.0)[0]
+                     for c in range(C)])
+    a = np.sqrt((np.abs(crec) ** 2).sum(0))
+    u = a * np.exp(1j * np.angle(crec.sum(0))) if apply_phase else a.astype(np.complex128)
+    return u, crec
+
+
+def coil_construction_from(crec, u0, **params):
+    """CoilConstruction::PerformCoilConstruction (coil_construction.cpp:447-557)
+    from time-averaged coil images crec [C][H][W] and u0 [H][W]: returns
+    (u [C][H][W], b1 [C][H][W], order)."""
+    crec, u0 = _c(crec), _c(u0)
+    C, H, W = crec.shape
+    prm = dict(COIL_DEFAULTS, **params)
+    pa = np.array([prm[k] for k in _COIL_ORDER], np.float64)
+    u = np.zeros((C, H, W), np.complex128)
+    b1 = np.zeros((C, H, W), np.complex128)
+    order = np.zeros(C, np.int32)
+    _coil_lib().orc_coil_construction_from(_ptr(crec), _ptr(u0), W, H, C, _ptr(pa), _ptr(u),
+                                           _ptr(b1), _ptr(order))
+    return u, b1, order
+
+
+def coil_b1_from_uh1(u, crec, mu=1e-5):
+    u, crec = _c(u), _c(crec)
+    C, H, W = crec.shape
+    b1 = np.zeros_like(crec)
+    _coil_lib().orc_coil_b1_from_uh1(_ptr(u), _ptr(crec), W, H, C, float(mu), _ptr(b1))
+    return b1
+
+
+def coil_find_max_sum(crec):
+    crec = _c(crec)
+    C, H, W = crec.shape
+    return _coil_lib().orc_coil_find_max_sum(_ptr(crec), W, H, C)
+
+
+def coil_utgv2(b1, crec, inds, x0, nu=0.2, alpha0=np.sqrt(2.0), alpha1=1.0,
+               sigma=np.sqrt(1.0 / 12.0), tau=np.sqrt(1.0 / 12.0), iters=100):
+    b1, crec = _c(b1), _c(crec)
+    C, H, W = crec.shape
+    x = _c(x0).copy()
+    ind = np.ascontiguousarray(inds, np.int32)
+    p = np.array([nu, alpha0, alpha1, sigma, tau, iters], np.float64)
+    _coil_lib().orc_coil_utgv2(_ptr(b1), _ptr(crec), _ptr(ind), len(ind), W, H, _ptr(p), _ptr(x))
+    return x
+
+
+def coil_b1_recon(u0, crec, iters=500, mu=2.0, sigma=np.sqrt(1.0 / 8.0), tau=np.sqrt(1.0 / 8.0)):
+    u0, crec = _c(u0), _c(crec)
+    C, H, W = crec.shape
+    x = np.zeros_like(crec)
+    p = np.array([sigma, tau], np.float64)
+    _coil_lib().orc_coil_b1_recon(_ptr(u0), _ptr(crec), C, W, H, int(iters), float(mu), _ptr(p),
+                                  _ptr(x))
+    return x
+
+
+def coil_construction(kdata, mask=None, centered=False, **params):
+    """PerformCoilConstruction: returns (u [C][H][W], b1 [C][H][W], order)."""
+    kdata = _c(kdata)
+    T, C, H, W = kdata.shape
+    m = None if mask is None else _c(mask, np.float64)
+    prm = dict(COIL_DEFAULTS, **params)
+    pa = np.array([prm[k] for k in _COIL_ORDER], np.float64)
+    u = np.zeros((C, H, W), np.complex128)
+    b1 = np.zeros((C, H, W), np.complex128)
+    order = np.zeros(C, np.int32)
+    _coil_lib().orc_coil_construction(_ptr(kdata), _ptr(m), W, H, C, T, int(bool(centered)),
+                                      _ptr(pa), _ptr(u), _ptr(b1), _ptr(order))
+    return u, b1, order
+
+
 def fft3(a, sign):
     """Unnormalised 3D DFT of a [D][H][W] array (sign -1: e^{-i})."""
     a = _c(a).copy()

@@ -158,6 +158,24 @@ int orc_recon_nc(const orc_params *p, int W, int H, int C, int T, int M, int J,
                  double *x_out_all, double *sig_log, double *tau_log,
                  int nthreads);
 
+/* Coil-sensitivity estimation (oracle_coil.c; coil_construction.cpp:22-557,
+ * cartesian_coil_construction.cpp:23-87).  Complex arrays interleaved;
+ * images [C][H][W]. */
+void orc_laplace_periodic(const double *u, int W, int H, double *out);
+void orc_coil_tar(const double *kdata, const double *mask, int W, int H, int C, int T,
+                  int centered, int apply_phase, double *u, double *crec);
+int orc_coil_b1_from_uh1(const double *u, const double *crec, int W, int H, int C, double mu,
+                         double *b1);
+int orc_coil_find_max_sum(const double *crec, int W, int H, int C);
+void orc_coil_utgv2(const double *b1, const double *crec, const int *inds, int n_inds, int W,
+                    int H, const double *p, double *x);
+void orc_coil_b1_recon(const double *u0, const double *crec, int coils, int W, int H, int iters,
+                       double mu, const double *p, double *x);
+void orc_coil_construction_from(const double *crec, const double *u0, int W, int H, int C,
+                                const double *prm, double *u, double *b1, int *order);
+void orc_coil_construction(const double *kdata, const double *mask, int W, int H, int C, int T,
+                           int centered, const double *prm, double *u, double *b1, int *order);
+
 /* ||K_full|| of the stacked regulariser + data operator by power iteration
  * (matlab/getopnorm.m:1-16 on K_full^* K_full): Cartesian 2D+t, volumetric
  * (TGV2_3D) and non-Cartesian data operators.  d6 = {dx, dy, dt, dx2, dy2,

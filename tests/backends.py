@@ -86,6 +86,28 @@ class OracleBackend:
     def recon_logs(self, params, data, b1, mask, x0):
         return O.recon(params, data, b1, mask, x0, logs=True)
 
+    # -- coil construction (coil_construction.cpp)
+    def laplace_periodic(self, u):
+        return O.laplace_periodic(u)
+
+    def coil_tar(self, kdata, mask=None, centered=False):
+        return O.coil_tar(kdata, mask, centered)
+
+    def coil_b1_from_uh1(self, u, crec):
+        return O.coil_b1_from_uh1(u, crec)
+
+    def coil_find_max_sum(self, crec):
+        return O.coil_find_max_sum(crec)
+
+    def coil_utgv2(self, b1, crec, inds, x0):
+        return O.coil_utgv2(b1, crec, inds, x0)
+
+    def coil_b1_recon(self, u0, crec, iters=500, mu=2.0):
+        return O.coil_b1_recon(u0, crec, iters, mu)
+
+    def coil_construction(self, kdata, mask=None, centered=False):
+        return O.coil_construction(kdata, mask, centered)
+
 
 class GpuBackend:
     """HIP product path through the C-ABI.  Inputs are converted to
@@ -240,3 +262,40 @@ class GpuBackend:
         s.IterativeReconstruction(self._t(data), x, self._t(b1))
         return dict(x=self._n(x), sigma=np.array(s.sigma_log()), tau=np.array(s.tau_log()),
                     primal=None)
+
+    # -- coil construction (coil_construction.cpp) through the C-ABI
+    def _coil(self, W, H, C, T, mask=None, centered=False):
+        op = self.A.CartesianOperator(W, H, C, T, self._r(mask), bool(centered))
+        return self.A.CoilConstruction(W, H, C, T, op), op
+
+    def coil_tar(self, kdata, mask=None, centered=False):
+        T, C, H, W = kdata.shape
+        cc, _op = self._coil(W, H, C, T, mask, centered)
+        u, crec = cc.TimeAveragedReconstruction(self._t(kdata))
+        return self._n(u), self._n(crec)
+
+    def coil_b1_from_uh1(self, u, crec):
+        C, H, W = crec.shape
+        cc = self.A.CoilConstruction(W, H, C, 1)
+        return self._n(cc.B1FromUH1(self._t(u), self._t(crec)))
+
+    def coil_find_max_sum(self, crec):
+        C, H, W = crec.shape
+        return self.A.CoilConstruction(W, H, C, 1).FindMaximumSum(self._t(crec))
+
+    def coil_utgv2(self, b1, crec, inds, x0):
+        C, H, W = crec.shape
+        x = self._t(x0)
+        self.A.CoilConstruction(W, H, C, 1).UTGV2Recon(self._t(b1), self._t(crec), x, inds)
+        return self._n(x)
+
+    def coil_b1_recon(self, u0, crec, iters=500, mu=2.0):
+        C, H, W = crec.shape
+        cc = self.A.CoilConstruction(W, H, C, 1)
+        return self._n(cc.B1Recon(self._t(u0), self._t(crec), C, iters, mu))
+
+    def coil_construction(self, kdata, mask=None, centered=False):
+        T, C, H, W = kdata.shape
+        cc, _op = self._coil(W, H, C, T, mask, centered)
+        u, b1, order = cc.PerformCoilConstruction(self._t(kdata))
+        return self._n(u), self._n(b1), order

@@ -91,8 +91,11 @@ def test_cli_resolves_config_and_overrides(tmp_path):
 def test_cli_rejects_what_it_cannot_do(tmp_path):
     dims = _problem(tmp_path)
     files = [str(tmp_path / "k.bin"), str(tmp_path / "m.bin"), str(tmp_path / "out.bin")]
-    # coil-map estimation is not part of this build
-    r = _run(["-d", dims, "-p", str(tmp_path / "p.cfg")] + files + ["--dry-run"], ok=False)
+    # without -s the coil maps are estimated (2D+t); a volume needs them given
+    cfg = json.loads(_run(["-d", dims, "-p", str(tmp_path / "p.cfg")] + files + ["--dry-run"]).stdout)
+    assert cfg["coil_maps"] == "coil-construction" and cfg["initial_image"] == "coil-construction"
+    r = _run(["-m", "TGV2_3D", "-a", "-d", "8:6:3:8:6:3:3:1", "-p", str(tmp_path / "p.cfg")] + files +
+             ["--dry-run"], ok=False)
     assert r.returncode == 3 and "coil maps" in r.stderr
     # wrong sizes
     r = _run(["-d", "8:6:1:8:6:1:3:3", "-s", str(tmp_path / "b1.bin"), "-p",

@@ -88,8 +88,13 @@ def test_cli_radial_matches_library(gpu, tmp_path):
     p.update(maxIt=40, adaptLambda=0)
     p["lambda"] = 2.0
     rec.SetParams(p)
-    x = gpu._t(np.zeros((T, H, W)))
     data = (k * np.sqrt(dcf.astype(np.float32))[:, None, :]).astype(np.complex64)
+    # -s without -u: u0 = (|crec| + sum_c conj(b1_c) crec_c) / 2 from the
+    # all-frames time average (PerformInitalizationGivenB1, main.cpp:250-287)
+    tu, crec = A.CoilConstruction(W, H, C, T, op).TimeAveragedReconstruction(gpu._t(data), False)
+    tu, crec = gpu._n(tu), gpu._n(crec)
+    u0 = 0.5 * (tu + (np.conj(b1) * crec).sum(0))
+    x = gpu._t(np.repeat(u0[None], T, 0))
     rec.IterativeReconstruction(gpu._t(data), x, gpu._t(b1))
     ref = gpu._n(x)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
@@ -270,4 +275,118 @@ def test_cli_tgv2_3d_matches_library(gpu, tmp_path):
     x = gpu._t(u0)
     rec.IterativeReconstruction(gpu._t(k * mask[None]), x, gpu._t(b1))
     ref = gpu._n(x)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def _coil_phantom(rng, T, C, H, W):
+    from avionic_amd import synth
+    rho = synth.phantom(max(W, H), T, 5, 7)[:, :H, :W]
+    b1 = synth.coil_maps(max(W, H), C, 7)[:, :H, :W]
+    return rho, b1
+
+
+def test_cli_coil_construction_cartesian(gpu, tmp_path):
+    """No -s: the coil maps and u0 come from PerformCoilConstruction on the
+    uncentred masked operator (main.cpp:618-649); b1_reconstructed.bin and
+    u0_reconstructed.bin hold them and the reconstruction starts from u0."""
+    rng = np.random.default_rng(8)
+    W, H, C, T = 40, 32, 4, 5
+    rho, b1t = _coil_phantom(rng, T, C, H, W)
+    mask = (rng.random((T, H, 1)) < 0.5) * np.ones((1, 1, W))
+    mask[:, :4] = 1
+    k = np.stack([np.fft.ifft2(b1t * rho[t][None], norm="ortho") for t in range(T)]) * 100
+    for name, v in (("k", k), ("m", mask.astype(float))):
+        A.write_bin(tmp_path / f"{name}.bin", v)
+    (tmp_path / "p.cfg").write_text("[tgv2]\nlambda = 4.0\n")
+    _run(["-m", "TGV2", "-i", "20", "-p", str(tmp_path / "p.cfg"), "-d",
+          f"{W}:{H}:1:{W}:{H}:1:{C}:{T}", str(tmp_path / "k.bin"), str(tmp_path / "m.bin"),
+          str(tmp_path / "x.bin")])
+    op = A.CartesianOperator(W, H, C, T, gpu._r(mask), centered=False)
+    data = gpu._t(k * mask[:, None])
+    u, b1, _ = A.CoilConstruction(W, H, C, T, op).PerformCoilConstruction(data)
+    u0, b1 = gpu._n(u)[C - 1], gpu._n(b1)
+    got_b1 = A.read_bin(tmp_path / "b1_reconstructed.bin").reshape(C, H, W)
+    got_u0 = A.read_bin(tmp_path / "u0_reconstructed.bin").reshape(H, W)
+    np.testing.assert_allclose(got_b1, b1, rtol=0, atol=1e-5 * np.abs(b1).max())
+    np.testing.assert_allclose(got_u0, u0, rtol=0, atol=1e-5 * np.abs(u0).max())
+    rec = A.TGV2(W, H, C, T, op, None)
+    p = rec.GetParams()
+    p.update(maxIt=20, adaptLambda=0)
+    p["lambda"] = 4.0
+    rec.SetParams(p)
+    x = gpu._t(np.repeat(u0[None], T, 0))
+    rec.IterativeReconstruction(data, x, gpu._t(b1))
+    ref = gpu._n(x)
+    got = A.read_bin(tmp_path / "x.bin").reshape(T, H, W)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def test_cli_coil_construction_radial(gpu, tmp_path):
+    """No -s on radial data: PerformNonCartesianCoilConstruction
+    (main.cpp:225-247), u0 divided by the frame count (:641-644)."""
+    rng = np.random.default_rng(9)
+    W = H = 32
+    C, T, nfe, spf = 3, 4, 48, 9
+    M = nfe * spf
+    rho, b1t = _coil_phantom(rng, T, C, H, W)
+    tr = O.radial_trajectory(nfe, spf, T).astype(np.float32).astype(np.float64)
+    dcf = np.sqrt(tr[:, 0] ** 2 + tr[:, 1] ** 2) + 0.01
+    k = O.nufft_K(rho, b1t, tr, None, J=3, osf=2.0) * 100
+    for name, v in (("k", k), ("traj", tr), ("w", dcf)):
+        A.write_bin(tmp_path / f"{name}.bin", v)
+    (tmp_path / "p.cfg").write_text("[tv]\nlambda = 2.0\n[gpunufft]\nkernelWidth = 3\nosf = 1.5\n")
+    _run(["-n", "-m", "TV", "-i", "20", "-p", str(tmp_path / "p.cfg"), "-d",
+          f"{W}:{H}:1:{nfe}:{spf}:1:{C}:{T}", "-w", str(tmp_path / "w.bin"), str(tmp_path / "k.bin"),
+          str(tmp_path / "traj.bin"), str(tmp_path / "x.bin")])
+    op = A.NoncartesianOperator(W, H, C, T, nfe, spf, gpu._r(tr), gpu._r(dcf), kernel_width=3,
+                                osf=1.5)
+    data = gpu._t((k * np.sqrt(dcf.astype(np.float32))[:, None, :]).astype(np.complex64))
+    u, b1, _ = A.CoilConstruction(W, H, C, T, op).PerformCoilConstruction(data)
+    u0, b1 = gpu._n(u)[C - 1] / T, gpu._n(b1)
+    got_b1 = A.read_bin(tmp_path / "b1_reconstructed.bin").reshape(C, H, W)
+    got_u0 = A.read_bin(tmp_path / "u0_reconstructed.bin").reshape(H, W)
+    np.testing.assert_allclose(got_b1, b1, rtol=0, atol=1e-5 * np.abs(b1).max())
+    np.testing.assert_allclose(got_u0, u0, rtol=0, atol=1e-5 * np.abs(u0).max())
+    rec = A.TV(W, H, C, T, op, None)
+    p = rec.GetParams()
+    p.update(maxIt=20, adaptLambda=0)
+    p["lambda"] = 2.0
+    rec.SetParams(p)
+    x = gpu._t(np.repeat(u0[None], T, 0))
+    rec.IterativeReconstruction(data, x, gpu._t(b1))
+    ref = gpu._n(x)
+    got = A.read_bin(tmp_path / "x.bin").reshape(T, H, W)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def test_cli_given_b1_initial_image_cartesian(gpu, tmp_path):
+    """-s without -u on Cartesian data: u0 = (|crec| + sum_c conj(b1_c)
+    crec_c) / 2 from the operator's own time average."""
+    rng = np.random.default_rng(10)
+    W, H, C, T = 24, 20, 3, 4
+    rho, b1 = _coil_phantom(rng, T, C, H, W)
+    mask = (rng.random((T, H, 1)) < 0.5) * np.ones((1, 1, W))
+    k = np.stack([np.fft.ifft2(b1 * rho[t][None], norm="ortho") for t in range(T)]) * 100
+    for name, v in (("k", k), ("m", mask.astype(float)), ("b1", b1)):
+        A.write_bin(tmp_path / f"{name}.bin", v)
+    (tmp_path / "p.cfg").write_text("[tv]\nlambda = 3.0\n")
+    _run(["-m", "TV", "-i", "15", "-p", str(tmp_path / "p.cfg"), "-d",
+          f"{W}:{H}:1:{W}:{H}:1:{C}:{T}", "-s", str(tmp_path / "b1.bin"), str(tmp_path / "k.bin"),
+          str(tmp_path / "m.bin"), str(tmp_path / "x.bin")])
+    assert not os.path.exists(tmp_path / "b1_reconstructed.bin")
+    op = A.CartesianOperator(W, H, C, T, gpu._r(mask), centered=False)
+    data = gpu._t(k * mask[:, None])
+    tu, crec = A.CoilConstruction(W, H, C, T, op).TimeAveragedReconstruction(data, False)
+    tu, crec = gpu._n(tu), gpu._n(crec)
+    b1c = b1.astype(np.complex64).astype(np.complex128)
+    u0 = 0.5 * (tu + (np.conj(b1c) * crec).sum(0))
+    rec = A.TV(W, H, C, T, op, None)
+    p = rec.GetParams()
+    p.update(maxIt=15, adaptLambda=0)
+    p["lambda"] = 3.0
+    rec.SetParams(p)
+    x = gpu._t(np.repeat(u0[None], T, 0))
+    rec.IterativeReconstruction(data, x, gpu._t(b1))
+    ref = gpu._n(x)
+    got = A.read_bin(tmp_path / "x.bin").reshape(T, H, W)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())

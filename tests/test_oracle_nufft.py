@@ -73,3 +73,27 @@ def test_recon_nc_runs_and_decreases_residual():
     r0 = np.linalg.norm(data)
     r1 = np.linalg.norm(O.nufft_K(x, b1, traj, dcf) - data)
     assert r1 < 0.5 * r0
+
+
+def test_coil_tar_nc_sums_frames():
+    """The all-frames time average equals the sum of each frame's own adjoint
+    gridding (the adjoint is a sum over points), so a frame or axis mix-up in
+    the rearrangement of noncartesian_coil_construction.cpp:38-60 fails;
+    u is the root-sum-of-squares with the phase of the coil sum."""
+    rng = np.random.default_rng(11)
+    H, W, C, T, nfe, spf = 12, 14, 3, 4, 16, 3
+    M = nfe * spf
+    tr = O.radial_trajectory(nfe, spf, T)
+    dcf = rng.random((T, M)) + 0.1
+    k = rng.standard_normal((T, C, M)) + 1j * rng.standard_normal((T, C, M))
+    u, crec = O.coil_tar_nc(k, tr, dcf, (H, W))
+    ones = np.ones((1, H, W))
+    for c in range(C):
+        ref = sum(O.nufft_KH(k[t:t + 1, c:c + 1], ones, tr[t:t + 1], dcf[t:t + 1], J=3, osf=2.0)[0]
+                  for t in range(T))
+        np.testing.assert_allclose(crec[c], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    s = crec.sum(0)
+    np.testing.assert_allclose(np.abs(u), np.sqrt((np.abs(crec) ** 2).sum(0)), rtol=1e-12)
+    np.testing.assert_allclose(u * np.abs(s), np.abs(u) * s, rtol=0, atol=1e-10 * np.abs(u * s).max())
+    u2, _ = O.coil_tar_nc(k, tr, dcf, (H, W), apply_phase=False)
+    np.testing.assert_allclose(u2, np.abs(u), rtol=1e-12)
