@@ -11,18 +11,24 @@ on its schedule k < 10 || k % 50 == 0) over every resident reconstruction.
            task-list launch sequence per iteration, replayed from a hipGraph.
 
 value = batched aggregate PD iterations/s (iterations x reconstructions per
-second); inputs are resident in HBM before the timed region.  The roofline
-entry is the dominant kernel's algorithmic bytes per launch over its average
-duration, timed with HIP events on the engine's stream.  cpu_baseline is the
-FP64 oracle (C, OpenMP) on the host cores for a few iterations of the same
-reconstruction.
+second); inputs are resident in HBM before the timed region.  The batched
+requests then run to 1000 iterations and the first result's NRMSE against the
+FP64 oracle's reconstruction of the same inputs (tests/golden/fullsize) is
+reported with the whole-reconstruction rate.  The roofline entry is the
+dominant kernel's algorithmic bytes per launch over its average duration,
+timed with HIP events on the engine's stream, against the 8 TB/s peak and
+against a 4 GiB device copy.  Legs: TGV2 256x256x25x12 plain and batched
+with fp32 duals, the headline workload with fp32 duals, TV 128x128x12x8
+batched, radial TGV2, and the serving stream.  cpu_baseline is the FP64
+oracle (C, OpenMP) timed on the host cores over steady-state iterations.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-                       [--dual fp32|bf16|q16] [--no-cpu]
+                       [--dual fp32|bf16|q16] [--no-cpu] [--no-legs]
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -139,7 +145,7 @@ def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500):
             "all_finite": finite}
 
 
-def radial_leg(A, synth, torch, dev, iters, dump_dir=None):
+def radial_leg(A, synth, torch, dev, iters):
     """Non-Cartesian TGV2: golden-angle radial, 512 samples x 21 spokes per
     frame, 30 frames, 12 coils, 256x256, Kaiser-Bessel width 3 on a 1.5x grid,
     density |k|.  One reconstruction through the PDRecon API (eager launches);
@@ -175,10 +181,6 @@ def radial_leg(A, synth, torch, dev, iters, dump_dir=None):
     rec.IterativeReconstruction(data, x, b1)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    if dump_dir:
-        os.makedirs(dump_dir, exist_ok=True)
-        np.save(os.path.join(dump_dir, f"{wl.name}_{iters}_fp32.npy"), x.cpu().numpy())
-        np.save(os.path.join(dump_dir, f"{wl.name}_data.npy"), data.cpu().numpy())
     return {"workload": wl.name, "method": "tgv2", "iterations": iters,
             "iters_per_s": round(iters / dt, 1), "ms_per_iter": round(1e3 * dt / iters, 4),
             "operator_setup_s": round(t_setup, 3), "lambda": round(p["lambda"], 6),
@@ -215,6 +217,201 @@ def small_leg(A, synth, torch, dev, B, steps, warmup):
             "batched_iters_per_s": round(batched, 1), "batched_over_plain": round(batched / plain, 2)}
 
 
+FIXTURE_DIR = os.path.join(ROOT, "tests", "golden", "fullsize")
+FIXTURE_ITERATIONS = {"tv_128x12x8": 500, "tgv2_256x25x12": 1000, "ictgv2_256x30x16": 1000}
+# AdaptLambda constants of InitLambda (tv.cpp / tgv2.cpp, ictgv2.cpp)
+LAMBDA_KD = {"tv": (0.4 * 0.2991, 10.0 * 0.2991), "tgv2": (0.4 * 0.2991, 10.0 * 0.2991),
+             "ictgv2": (0.4 / (5.1191 * 0.35), 10.0 / (5.1191 * 0.35))}
+STEP_RULE_PHASE = 10  # iterations k < 10 adapt sigma / tau every iteration
+
+
+def input_digest(data, b1, x0, mask):
+    """sha256 prefix of the inputs as the GPU receives them (complex64 data,
+    coil maps and u0, float32 mask), widened to 128 / 64 bits: the key the
+    FP64 fixtures are stored under."""
+    h = hashlib.sha256()
+    for a in (data, b1, x0):
+        h.update(np.ascontiguousarray(np.asarray(a, np.complex64).astype(np.complex128)).tobytes())
+    h.update(np.ascontiguousarray(np.asarray(mask, np.float32).astype(np.float64)).tobytes())
+    return h.hexdigest()[:16]
+
+
+def load_fixture(name):
+    path = os.path.join(FIXTURE_DIR, f"{name}_{FIXTURE_ITERATIONS.get(name, 0)}.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as f:
+        return dict(x=f["x"].astype(np.complex128), digest=str(f["digest"]),
+                    iterations=int(f["iterations"]), lam=float(f["lam"]))
+
+
+def nrmse(x, ref):
+    """||x - x_ref|| / ||x_ref|| over the complex image (matlab/myerror.m:68-70)."""
+    x, ref = np.asarray(x, np.complex128).ravel(), np.asarray(ref, np.complex128).ravel()
+    return float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+
+
+def cartesian_leg(A, synth, torch, dev, name, batch, dual, steps, warmup, plain=True,
+                  check_nrmse=True, kernel_times=False):
+    """Batched (and plain) PD iterations/s of one Cartesian workload.
+
+    Protocol: every request first runs the k < 10 step-rule phase untimed
+    (setup), then `warmup` untimed steps, then exactly `steps` timed steps
+    (one step = one PD iteration of every resident reconstruction, the
+    k % 50 == 0 step-rule iterations included as they come).  With
+    check_nrmse the batched requests then run to the fixture's iteration
+    count, the whole reconstruction is timed too, and the first result's
+    NRMSE against the FP64 fixture is returned."""
+    wl = synth.CONFIGS[name]
+    inp = synth.make(wl)
+    T, C, H, W = inp["data"].shape
+    method = {"tv": 0, "tgv2": 1, "ictgv2": 2}[wl.method]
+    dual_kind = {"fp32": 0, "bf16": 1, "q16": 2}[dual]
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.complex64)).to(dev)
+    data, b1, x0 = c64(inp["data"]), c64(inp["b1"]), c64(inp["x0"])
+    mask = torch.from_numpy(np.ascontiguousarray(inp["mask"], np.float32)).to(dev)
+    if "lambda" in wl.params:
+        lam = float(wl.params["lambda"])
+    else:  # the Cartesian AdaptLambda (cartesian_operator.cpp:47-62)
+        m = np.asarray(inp["mask"], np.float32).astype(np.float64)
+        k, d = LAMBDA_KD[wl.method]
+        lam = float(m.size / float((m ** 2).sum()) * k + d)
+    fx = load_fixture(name) if check_nrmse else None
+    fixture_ok = (fx is not None and fx["digest"] == input_digest(inp["data"], inp["b1"],
+                                                                   inp["x0"], inp["mask"]))
+    if fixture_ok:
+        lam = fx["lam"]
+    run_its = STEP_RULE_PHASE + warmup + steps
+    full_its = fx["iterations"] if fixture_ok else 0
+    params = {k: v for k, v in wl.params.items() if k in ("alpha0", "alpha1")}
+    params.update({"lambda": lam, "adaptLambda": 0, "dualStorage": dual_kind,
+                   "maxIt": max(full_its, run_its + 10)})
+
+    def request():
+        return A.Request(method, data, b1, mask=mask, x0=x0, centered=False, params=params)
+
+    def run(engine):
+        """(setup s, warmup s, timed s)."""
+        out = []
+        for n in (STEP_RULE_PHASE, warmup, steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                engine.step()
+            torch.cuda.synchronize()
+            out.append(time.perf_counter() - t0)
+        return out
+
+    res = {"workload": name, "method": wl.method, "matrix": [H, W], "frames": T, "coils": C,
+           "batch": batch, "dual_storage": dual}
+    if plain:
+        log(f"{name}: plain path ({dual} duals)")
+        eng = A.Engine(use_graphs=False, max_batch=1)
+        eng.submit(request())
+        t_plain = run(eng)[2]
+        res["plain_iters_per_s"] = round(steps / t_plain, 2)
+        del eng
+    log(f"{name}: batched path (B={batch}, {dual} duals)")
+    eng = A.Engine(use_graphs=True, max_batch=batch)
+    ids = [eng.submit(request()) for _ in range(batch)]
+    assert all(i is not None for i in ids)
+    t_setup, t_warm, t_batch = run(eng)
+    res["batched_iters_per_s"] = round(batch * steps / t_batch, 2)
+    res["ms_per_step"] = round(1e3 * t_batch / steps, 4)
+    if plain:
+        res["batched_over_plain"] = round(res["batched_iters_per_s"] / res["plain_iters_per_s"], 3)
+    if fixture_ok:
+        x = torch.empty_like(x0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while not eng.poll(ids[0], x):
+            eng.step()
+        torch.cuda.synchronize()
+        t_rest = time.perf_counter() - t0
+        t_full = t_setup + t_warm + t_batch + t_rest
+        res["full_reconstruction"] = {
+            "iterations": full_its,
+            "seconds": round(t_full, 3),
+            "iters_per_s": round(batch * full_its / t_full, 2),
+            "note": "the batch's whole reconstructions, every step-rule iteration included"}
+        res["nrmse"] = float(f"{nrmse(x.cpu().numpy(), fx['x']):.4g}")
+        res["nrmse_vs"] = f"FP64 oracle, {full_its} iterations (tests/golden/fullsize)"
+    elif check_nrmse:
+        res["nrmse"] = None
+        res["nrmse_vs"] = "fixture missing or inputs differ"
+    del eng
+    if kernel_times:
+        log(f"{name}: kernel timing")
+        eng = A.Engine(use_graphs=False, max_batch=batch)
+        for _ in range(batch):
+            eng.submit(request())
+        res["_kernel_ms"] = A.engine_time_kernels(eng, reps=20)
+        res["_voxels"] = W * H * T
+        del eng
+    return res
+
+
+def copy_calibration(torch, dev, gib=4):
+    """A plain device copy of `gib` GiB into another `gib` GiB (a working set
+    many times the 256 MB MALL): the achievable streaming rate, timed with
+    HIP events, and the reference point that scales rocprofv3 FETCH_SIZE /
+    WRITE_SIZE counters to bytes."""
+    n = gib * (1 << 30) // 4
+    src = torch.empty(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"bytes_per_copy": 2 * n * 4, "ms": round(ms, 4),
+            "gbs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
+
+
+def cpu_baseline(synth, name, seconds):
+    """The FP64 C oracle (OpenMP) on the host cores: steady-state PD iterations
+    (fixed sigma, tau: no step-rule reductions) of the full-size workload,
+    after one untimed warm-up iteration; all cores for about `seconds`, then
+    one iteration on one core."""
+    import oracle as O
+    wl = synth.CONFIGS[name]
+    inp = synth.make(wl)
+    T, C, H, W = inp["data"].shape
+    p = O.default_params(wl.method)
+    m = np.asarray(inp["mask"], np.float64)
+    k, d = LAMBDA_KD[wl.method]
+    lam = wl.params.get("lambda") or float(m.size / float((m ** 2).sum()) * k + d)
+    p.update(lambda_=lam, schedule=2, sigma=0.25, tau=0.25,
+             **{kk: v for kk, v in wl.params.items() if kk in ("alpha0", "alpha1")})
+    args = (inp["data"], inp["b1"], inp["mask"], inp["x0"])
+    cores = min(16, os.cpu_count() or 1)
+
+    def timed(its, threads):
+        t0 = time.perf_counter()
+        O.recon(dict(p, maxIt=its), *args, nthreads=threads)
+        return time.perf_counter() - t0
+
+    t1 = timed(1, cores)  # warm-up (also sizes the sample)
+    its = int(min(20, max(3, seconds / max(t1, 1e-3))))
+    dt = timed(its, cores)
+    dt1 = timed(1, 1)
+    return {"value": round(its / dt, 4), "unit": "PD iterations/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{its} steady-state PD iterations (fixed sigma/tau) of one full-size "
+                      f"{name} reconstruction after 1 warm-up iteration, FP64 C oracle, "
+                      f"OpenMP {cores} threads",
+            "single_core": {"value": round(1.0 / dt1, 5), "cores": 1,
+                            "sample": "1 steady-state PD iteration, 1 thread"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,7 +424,8 @@ def main():
                          "4-byte formats that meets NRMSE <= 1e-4 at full size)")
     ap.add_argument("--workload", default="ictgv2_256x30x16")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="target length of the all-cores CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window of the serving stream (0: skip the serving leg)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
@@ -236,12 +434,12 @@ def main():
                          "small-request leg (0: skip)")
     ap.add_argument("--radial-iters", type=int, default=200,
                     help="iterations of the radial TGV2 leg (0: skip)")
-    ap.add_argument("--dump-radial", default=None,
-                    help="directory: save the radial leg's x and k-space data as .npy")
-    ap.add_argument("--dump-x", default=None,
-                    help="directory: also run one full reconstruction (the workload's iteration "
-                         "count) with fp32 and with bf16 duals and save x as .npy, for the NRMSE "
-                         "check against the FP64 oracle (python -m oracle.nrmse)")
+    ap.add_argument("--no-nrmse", action="store_true",
+                    help="skip running the batched requests to 1000 iterations for the NRMSE "
+                         "(shorter profiler runs)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="only the headline workload (no cfg1 / fp32-dual / small / radial / "
+                         "serving legs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -258,92 +456,42 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
     wl = synth.CONFIGS[args.workload]
-    log(f"generating {wl.name} inputs")
-    inp = synth.make(wl)
-    T, C, H, W = inp["data"].shape
-    method = {"tv": 0, "tgv2": 1, "ictgv2": 2}[wl.method]
-    dual_kind = {"fp32": 0, "bf16": 1, "q16": 2}[args.dual]
-    bf16 = dual_kind != 0  # 4-byte duals
+    bf16 = args.dual != "fp32"  # 4-byte duals
 
-    def c64(a):
-        return torch.from_numpy(np.ascontiguousarray(a, np.complex64)).to(dev)
-
-    data, b1, x0 = c64(inp["data"]), c64(inp["b1"]), c64(inp["x0"])
-    mask = torch.from_numpy(np.ascontiguousarray(inp["mask"], np.float32)).to(dev)
-    total_its = args.warmup + args.steps + 10
-    params = dict(wl.params)
-    params.update(maxIt=total_its, dualStorage=dual_kind)
-    if "lambda" in wl.params:
-        params["adaptLambda"] = 0
-
-    def request():
-        return A.Request(method, data, b1, mask=mask, x0=x0, centered=False, params=params)
-
-    def timed(engine, n_req):
-        for _ in range(args.warmup):
-            engine.step()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            engine.step()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    # ---- plain: one reconstruction, eager launches
-    log("plain path")
-    eng = A.Engine(use_graphs=False, max_batch=1)
-    rid = eng.submit(request())
-    t_plain = timed(eng, 1)
-    plain_its = args.steps / t_plain
-    del eng
-
-    # ---- batched: B reconstructions, one graph-replayed launch sequence
-    log(f"batched path (B={args.batch})")
-    eng = A.Engine(use_graphs=True, max_batch=args.batch)
-    ids = [eng.submit(request()) for _ in range(args.batch)]
-    assert all(i is not None for i in ids)
-    t_batch = timed(eng, args.batch)
-    batched_its = args.batch * args.steps / t_batch
-    ms_per_step = 1e3 * t_batch / args.steps
-    del eng
+    head = cartesian_leg(A, synth, torch, dev, wl.name, args.batch, args.dual, args.steps,
+                         args.warmup, plain=True, check_nrmse=not args.no_nrmse,
+                         kernel_times=True)
+    ktimes = head.pop("_kernel_ms")
+    N = head.pop("_voxels")
 
     # ---- same result alone and in the mix (60 iterations covers the k<10
     #      and k%50 step-rule iterations)
     log("alone vs in-mix check")
-    p60 = dict(params, maxIt=60)
+    inp = synth.make(wl)
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.complex64)).to(dev)
+    data, b1, x0 = c64(inp["data"]), c64(inp["b1"]), c64(inp["x0"])
+    mask = torch.from_numpy(np.ascontiguousarray(inp["mask"], np.float32)).to(dev)
+    method = {"tv": 0, "tgv2": 1, "ictgv2": 2}[wl.method]
+    p60 = dict(wl.params, maxIt=60, dualStorage={"fp32": 0, "bf16": 1, "q16": 2}[args.dual])
     alone = A.Engine(use_graphs=False, max_batch=1)
     ra = alone.submit(A.Request(method, data, b1, mask=mask, x0=x0, params=p60))
     xa = torch.empty_like(x0)
     while not alone.poll(ra, xa):
         alone.step()
     mix = A.Engine(use_graphs=True, max_batch=4)
-    other = dict(params, maxIt=90)
-    mix.submit(A.Request(method, data, b1, mask=mask, x0=torch.zeros_like(x0), params=other))
+    mix.submit(A.Request(method, data, b1, mask=mask, x0=torch.zeros_like(x0),
+                         params=dict(p60, maxIt=90)))
     rm = mix.submit(A.Request(method, data, b1, mask=mask, x0=x0, params=p60))
     xm = torch.empty_like(x0)
     while not mix.poll(rm, xm):
         mix.step()
     torch.cuda.synchronize()
     mix_diff = float((xa - xm).abs().max().item())
-    del alone, mix
+    del alone, mix, data, b1, x0, mask
 
-    # ---- a plain device copy of known size, the reference point that scales
-    #      rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes
-    calib = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
-    calib_out = torch.empty_like(calib)
-    calib_out.copy_(calib)
-    torch.cuda.synchronize()
-    del calib, calib_out
+    log("copy calibration")
+    calib = copy_calibration(torch, dev)
 
-    # ---- per-kernel times (HIP events on the engine stream), fresh engine
-    log("kernel timing")
-    eng = A.Engine(use_graphs=False, max_batch=args.batch)
-    for _ in range(args.batch):
-        eng.submit(request())
-    ktimes = A.engine_time_kernels(eng, reps=20)
-    del eng
-    N = W * H * T
-    roof = None
     per_voxel = {k: algorithmic_bytes_per_voxel(k, wl.method, bf16) for k in ("dual", "primal")}
     dom = max(per_voxel, key=lambda k: ktimes[k])
     bytes_per_launch = per_voxel[dom] * N * args.batch
@@ -351,81 +499,69 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(wl.name, args.batch, args.dual, dom),
-            "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(ktimes[dom], 4)}
+            "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(ktimes[dom], 4),
+            "copy_gbs": calib["gbs"], "frac_of_copy": round(achieved / calib["gbs"], 4),
+            "copy_calibration": calib}
 
+    legs = {}
+    if not args.no_legs:
+        # the TGV2 workload as the plain fp32 path it is quoted on, and batched
+        legs["tgv2_256x25x12"] = cartesian_leg(A, synth, torch, dev, "tgv2_256x25x12",
+                                               args.batch, "fp32", args.steps, args.warmup)
+        # the headline workload with 8-byte (fp32) duals, the reference's storage
+        legs["fp32_duals"] = cartesian_leg(A, synth, torch, dev, wl.name, args.batch, "fp32",
+                                           args.steps, args.warmup, plain=False,
+                                           check_nrmse=False)
     small = None
-    if args.small_batch > 0:
+    if args.small_batch > 0 and not args.no_legs:
         log(f"small-request leg (tv_128x12x8, B={args.small_batch})")
         small = small_leg(A, synth, torch, dev, args.small_batch, args.steps, args.warmup)
 
-    if args.dump_x:
-        os.makedirs(args.dump_x, exist_ok=True)
-        for ds in ("fp32", "bf16", "q16"):
-            p_full = dict(params, maxIt=wl.iterations,
-                          dualStorage={"fp32": 0, "bf16": 1, "q16": 2}[ds])
-            e1 = A.Engine(use_graphs=True, max_batch=1)
-            rid = e1.submit(A.Request(method, data, b1, mask=mask, x0=x0, params=p_full))
-            xo = torch.empty_like(x0)
-            while not e1.poll(rid, xo):
-                e1.step()
-            torch.cuda.synchronize()
-            path = os.path.join(args.dump_x, f"{wl.name}_{wl.iterations}_{ds}.npy")
-            np.save(path, xo.cpu().numpy())
-            log(f"saved {path}")
-            del e1
-
     radial = None
-    if args.radial_iters > 0:
+    if args.radial_iters > 0 and not args.no_legs:
         log(f"radial TGV2 leg ({synth.RADIAL.name}, {args.radial_iters} iterations)")
-        radial = radial_leg(A, synth, torch, dev, args.radial_iters, args.dump_radial)
+        radial = radial_leg(A, synth, torch, dev, args.radial_iters)
 
     serving = None
-    if args.serve_seconds > 0:
+    if args.serve_seconds > 0 and not args.no_legs:
         log(f"serving stream ({args.serve_rate}/s for {args.serve_seconds}s)")
         serving = serving_leg(A, synth, torch, dev, args.serve_rate, args.serve_seconds)
 
-    # ---- CPU baseline: FP64 oracle on the host cores, a few iterations
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (FP64 oracle)")
-        import oracle as O
-        cores = min(16, os.cpu_count() or 1)
-        p = O.default_params(wl.method)
-        k, d = O.adapt_lambda_defaults(wl.method)
-        lam = wl.params.get("lambda") or O.adapt_lambda_cart(inp["mask"], W, H, T, k, d)
-        p.update(maxIt=args.cpu_iters, lambda_=lam, **{kk: v for kk, v in wl.params.items()
-                                                      if kk in ("alpha0", "alpha1")})
-        t0 = time.perf_counter()
-        O.recon(p, inp["data"], inp["b1"], inp["mask"], inp["x0"], nthreads=cores)
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(args.cpu_iters / dt, 4), "unit": "PD iterations/s", "cores": cores,
-               "kind": "port",
-               "sample": f"{args.cpu_iters} PD iterations of one {wl.name} reconstruction "
-                         f"(FP64 C oracle, OpenMP {cores} threads)"}
+        cpu = cpu_baseline(synth, wl.name, args.cpu_seconds)
 
     out = {
         "metric": "PD iters/s ICTGV2 256\u00b2\u00d730\u00d716c batched vs plain @NRMSE\u22641e-4; "
                   "p50/p99 latency",
-        "value": round(batched_its, 2),
+        "value": head["batched_iters_per_s"],
         "unit": "PD iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64" + ({0: "", 1: "+bf16-duals", 2: "+q16-duals"}[dual_kind]),
+        "dtype": "c64" + {"fp32": "", "bf16": "+bf16-duals", "q16": "+q16-duals"}[args.dual],
         "data": "synthetic",
-        "config": {"workload": wl.name, "method": wl.method, "matrix": [H, W], "frames": T,
-                   "coils": C, "accel": wl.accel, "batch": args.batch,
-                   "dual_storage": args.dual, "parallelism": "single-gpu"},
-        "plain_iters_per_s": round(plain_its, 2),
-        "batched_over_plain": round(batched_its / plain_its, 3),
+        "config": {"workload": wl.name, "method": wl.method, "matrix": head["matrix"],
+                   "frames": head["frames"], "coils": head["coils"], "accel": wl.accel,
+                   "batch": args.batch, "dual_storage": args.dual, "parallelism": "single-gpu"},
+        "protocol": f"each request runs its k<{STEP_RULE_PHASE} step-rule phase untimed, then "
+                    f"{args.warmup} warmup + {args.steps} timed steps (k%50==0 step-rule "
+                    f"iterations included); full_reconstruction times the whole run",
+        "plain_iters_per_s": head["plain_iters_per_s"],
+        "batched_over_plain": head["batched_over_plain"],
+        "nrmse": head.get("nrmse"),
+        "nrmse_vs": head.get("nrmse_vs"),
+        "full_reconstruction": head.get("full_reconstruction"),
         "alone_vs_mix_max_abs_diff": mix_diff,
         "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "legs": legs,
         "small_requests": small,
         "radial": radial,
         "serving": serving,

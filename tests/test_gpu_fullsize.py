@@ -38,9 +38,16 @@ def engine_run(gpu, case, ds, graphs=True, x0=True, maxit=None):
     return gpu._n(x), eng
 
 
+# bfloat16 duals (dualStorage 1) miss the 1e-4 bar on these workloads; their
+# NRMSE is measured and held to the looser bar it is known to meet.
+BF16_BAR = 3e-3
+
+
 @pytest.mark.parametrize("name,ds", [("tv_128x12x8", 0), ("tv_128x12x8", 2),
                                      ("tgv2_256x25x12", 0), ("tgv2_256x25x12", 2),
-                                     ("ictgv2_256x30x16", 0), ("ictgv2_256x30x16", 2)])
+                                     ("ictgv2_256x30x16", 0), ("ictgv2_256x30x16", 2),
+                                     ("tv_128x12x8", 1), ("tgv2_256x25x12", 1),
+                                     ("ictgv2_256x30x16", 1)])
 def test_cartesian_nrmse_vs_fp64_fixture(gpu, name, ds):
     print(f"[gpu fullsize] {name} dualStorage={ds}", flush=True)
     fx = FS.load_fixture(name)
@@ -50,7 +57,7 @@ def test_cartesian_nrmse_vs_fp64_fixture(gpu, name, ds):
     x, _ = engine_run(gpu, case, ds)
     e = FS.nrmse(x, fx["x"])
     print(f"  NRMSE {e:.3e}", flush=True)
-    assert e <= NRMSE_BAR
+    assert e <= (BF16_BAR if ds == 1 else NRMSE_BAR)
 
 
 def radial_run(gpu, case, maxit):
