@@ -571,7 +571,13 @@ def fft2(a, sign=-1, centered=False, unitary=False):
 # ------------------------------------------------------- batching / serving
 class _EngineConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", _SZ), ("use_graphs", _I), ("max_batch", _I),
-                ("iterations_per_tick", _I)]
+                ("iterations_per_tick", _I), ("resident_bytes_cap", _SZ), ("max_queue", _I)]
+
+
+class _EngineCounts(ctypes.Structure):
+    _fields_ = [("held", _I), ("running", _I), ("queued", _I), ("bytes_held", _SZ),
+                ("bytes_running", _SZ), ("hbm_budget_bytes", _SZ),
+                ("graph_captures", ctypes.c_longlong), ("capture_ms", _D)]
 
 
 class _ReqDesc(ctypes.Structure):
@@ -590,6 +596,7 @@ _ENGINE_SIGS = {
     "avionic_engine_poll": (_I, [_P, ctypes.c_longlong, _P, _P]),
     "avionic_engine_cancel": (_I, [_P, ctypes.c_longlong]),
     "avionic_engine_stats": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_SZ)]),
+    "avionic_engine_counters": (_I, [_P, ctypes.POINTER(_EngineCounts)]),
     "avionic_engine_get_step_log": (_I, [_P, ctypes.c_longlong, _P, _P, ctypes.POINTER(_SZ)]),
     "avionic_engine_get_gap_log": (_I, [_P, ctypes.c_longlong, _P, _P, _P, ctypes.POINTER(_SZ)]),
 }
@@ -643,14 +650,18 @@ class Request:
 
 
 class Engine:
-    """Continuous-batching executor (avionic_engine_*): requests are admitted
-    against an HBM budget, stay resident, and advance together in task-list
-    launches (hipGraph-replayed when use_graphs)."""
+    """Continuous-batching executor (avionic_engine_*): requests are held
+    against an HBM budget (default 90% of the free device memory), admitted in
+    FIFO order while the running requests stay within max_batch and
+    resident_bytes_cap, and advance together in task-list launches
+    (hipGraph-replayed when use_graphs).  max_queue: waiting requests beyond
+    which submit returns None (default 0: no queue; < 0: unbounded)."""
 
-    def __init__(self, hbm_budget_bytes=0, use_graphs=True, max_batch=0, iterations_per_tick=1):
+    def __init__(self, hbm_budget_bytes=0, use_graphs=True, max_batch=0, iterations_per_tick=1,
+                 resident_bytes_cap=0, max_queue=0):
         L = _engine_lib()
         cfg = _EngineConfig(int(hbm_budget_bytes), int(bool(use_graphs)), int(max_batch),
-                            int(iterations_per_tick))
+                            int(iterations_per_tick), int(resident_bytes_cap), int(max_queue))
         h = ctypes.c_void_p()
         _check(L.avionic_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -697,6 +708,13 @@ class Engine:
         r, b = ctypes.c_int(), ctypes.c_size_t()
         _check(_engine_lib().avionic_engine_stats(self._h, ctypes.byref(r), ctypes.byref(b)))
         return r.value, b.value
+
+    def counters(self) -> dict:
+        """held / running / queued requests, their bytes, the HBM budget, and
+        the iteration graphs captured so far with the host time they took."""
+        c = _EngineCounts()
+        _check(_engine_lib().avionic_engine_counters(self._h, ctypes.byref(c)))
+        return {name: getattr(c, name) for name, _ in _EngineCounts._fields_}
 
     def step_log(self, rid):
         n = ctypes.c_size_t(1 << 30)

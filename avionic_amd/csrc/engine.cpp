@@ -8,9 +8,11 @@
 // and adaptation schedule on the device, so a request's result does not
 // depend on what else is resident.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -39,6 +41,7 @@ struct Slot {
   avionic_params params{};
   int maxit = 0, k_host = 0;
   bool done = false;
+  bool admitted = false;  // in its group's launch set (else waiting in the queue)
   // in-loop PD gap (avionic_request_desc::verbose / debug, params.stopPDGap)
   bool verbose = false, debug = false;
   int debugstep = 1;
@@ -100,9 +103,46 @@ struct avionic_engine {
   hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_req = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   long long next_id = 1;
-  size_t used = 0;
+  size_t used = 0;           // bytes of every request held (queued, running, finished)
+  size_t resident_bytes = 0; // bytes of the admitted, unfinished requests
+  int resident = 0;          // admitted, unfinished requests
+  std::deque<Slot *> queue;  // submitted, not yet admitted (FIFO)
+  long long captures = 0;    // graph captures + instantiations
+  double capture_ms = 0.0;   // host time spent in them
   std::map<long long, std::unique_ptr<Slot>> slots;
   std::map<int, Group> groups;  // key = method * 3 + dual storage
+
+  // FIFO admission: the queue head joins the launch set when the resident
+  // count stays within max_batch and the resident bytes within
+  // resident_bytes_cap (a request always runs when nothing else does).
+  bool fits(const Slot &s) const {
+    if (resident == 0) return true;
+    if (resident >= cfg.max_batch) return false;
+    return cfg.resident_bytes_cap == 0 || resident_bytes + s.bytes <= cfg.resident_bytes_cap;
+  }
+  void admit(Slot *s) {
+    Group &grp = groups[s->method * 3 + s->ds];
+    grp.method = s->method;
+    grp.ds = s->ds;
+    grp.slots.push_back(s);
+    grp.dirty = true;
+    s->admitted = true;
+    resident += 1;
+    resident_bytes += s->bytes;
+  }
+  void admit_queued() {
+    while (!queue.empty() && fits(*queue.front())) {
+      admit(queue.front());
+      queue.pop_front();
+    }
+  }
+  // a request leaves the launch set (finished or cancelled)
+  void retire(Slot *s) {
+    if (!s->admitted) return;
+    s->admitted = false;
+    resident -= 1;
+    resident_bytes -= s->bytes;
+  }
 
   ~avionic_engine() {
     for (auto &kv : groups) kv.second.drop_graphs();
@@ -203,12 +243,16 @@ struct avionic_engine {
     }
     hipGraphExec_t &ex = with_step ? g.g_step : g.g_plain;
     if (!ex) {
+      const auto t0 = std::chrono::steady_clock::now();
       hipGraph_t graph = nullptr;
       AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st);
       AV_HIP(hipStreamEndCapture(stream, &graph));
       AV_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);
+      captures += 1;
+      capture_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                        .count();
     }
     AV_HIP(hipGraphLaunch(ex, stream));
   }
@@ -286,6 +330,12 @@ int avionic_engine_create(const avionic_engine_config *cfg, avionic_engine **eng
     e->cfg = *cfg;
     if (e->cfg.iterations_per_tick <= 0) e->cfg.iterations_per_tick = 1;
     if (e->cfg.max_batch <= 0) e->cfg.max_batch = 1 << 20;
+    if (e->cfg.hbm_budget_bytes == 0) {
+      // default: 90% of the device memory free now
+      size_t free_b = 0, total_b = 0;
+      AV_HIP(hipMemGetInfo(&free_b, &total_b));
+      e->cfg.hbm_budget_bytes = free_b / 10 * 9;
+    }
     AV_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     AV_HIP(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
     AV_HIP(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
@@ -341,11 +391,11 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->debugstep = d->debugstep > 0 ? d->debugstep : 1;
     AV_CHECK(s->params.stopPDGap >= 0.0, "stopPDGap must not be negative");
     s->bytes = slot_bytes(s->method, s->ds, s->N, s->K, (long)s->C * s->H * s->W, s->maxit);
-    if (e->cfg.hbm_budget_bytes && e->used + s->bytes > e->cfg.hbm_budget_bytes)
+    if (e->used + s->bytes > e->cfg.hbm_budget_bytes)
       throw Error(AVIONIC_E_CAPACITY, "request does not fit the HBM budget now");
-    int resident = 0;
-    for (auto &kv : e->slots) resident += kv.second->done ? 0 : 1;
-    if (resident >= e->cfg.max_batch) throw Error(AVIONIC_E_CAPACITY, "batch is full");
+    const bool runs_now = e->queue.empty() && e->fits(*s);
+    if (!runs_now && e->cfg.max_queue >= 0 && (int)e->queue.size() >= e->cfg.max_queue)
+      throw Error(AVIONIC_E_CAPACITY, "admission queue is full");
 
     const int P = primal_components(s->method), D = dual_components(s->method);
     // inputs produced on the caller's stream; admissions copy on their own
@@ -435,12 +485,13 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->id = e->next_id++;
     *id = s->id;
     e->used += s->bytes;
-    Group &grp = e->groups[s->method * 3 + s->ds];
-    grp.method = s->method;
-    grp.ds = s->ds;
-    grp.slots.push_back(s.get());
-    grp.dirty = true;
-    if (s->maxit == 0) s->done = true;
+    if (s->maxit == 0) {
+      s->done = true;
+    } else if (runs_now) {
+      e->admit(s.get());
+    } else {
+      e->queue.push_back(s.get());
+    }
     e->slots[s->id] = std::move(s);
   });
 }
@@ -453,6 +504,7 @@ int avionic_engine_step(avionic_engine *e, void *stream) {
     hipStream_t us = (hipStream_t)stream;
     AV_HIP(hipEventRecord(e->ev_in, us));
     AV_HIP(hipStreamWaitEvent(e->stream, e->ev_in, 0));
+    e->admit_queued();
     for (auto &kv : e->groups) {
       Group &g = kv.second;
       // drop finished requests from the launch set
@@ -479,7 +531,10 @@ int avionic_engine_step(avionic_engine *e, void *stream) {
           }
       }
       for (Slot *s : g.slots)
-        if (s->k_host >= s->maxit) s->done = true;
+        if (s->k_host >= s->maxit && !s->done) {
+          s->done = true;
+          e->retire(s);
+        }
     }
     AV_HIP(hipEventRecord(e->ev_out, e->stream));
     AV_HIP(hipStreamWaitEvent(us, e->ev_out, 0));
@@ -539,8 +594,26 @@ int avionic_engine_cancel(avionic_engine *e, long long id) {
         kv.second.dirty = true;
       }
     }
+    auto q = std::find(e->queue.begin(), e->queue.end(), s);
+    if (q != e->queue.end()) e->queue.erase(q);
+    e->retire(s);
     e->used -= s->bytes;
     e->slots.erase(it);
+  });
+}
+
+int avionic_engine_counters(const avionic_engine *e, avionic_engine_counts *c) {
+  return eguard([&] {
+    AV_CHECK(e && c, "null argument");
+    std::lock_guard<std::mutex> lock(e->mu);
+    c->held = (int)e->slots.size();
+    c->running = e->resident;
+    c->queued = (int)e->queue.size();
+    c->bytes_held = e->used;
+    c->bytes_running = e->resident_bytes;
+    c->hbm_budget_bytes = e->cfg.hbm_budget_bytes;
+    c->graph_captures = e->captures;
+    c->capture_ms = e->capture_ms;
   });
 }
 

@@ -76,12 +76,16 @@ def measured_traffic(workload, batch, dual, kernel):
     return rec.get("hbm_bytes_per_launch", {}).get(kernel)
 
 
-def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500):
+def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500, cap_bytes=0,
+                max_queue=-1):
     """Continuous batching under Poisson arrivals: mixed TV / TGV2 / ICTGV2
     Cartesian requests (n in 128..256, T 8..30, C 8..20, 500 iterations each).
-    Request inputs are built on the device before the clock starts; a
-    request's latency runs from its scheduled arrival to the moment its
-    result has been copied out."""
+    Request inputs are built on the device before the clock starts.  Each
+    request is submitted at its arrival time; the engine holds it, admits it
+    in FIFO order while the running requests' bytes stay within cap_bytes,
+    and queues up to max_queue more (submit refuses beyond: load shedding,
+    counted as rejected).  A request's latency runs from its arrival to the
+    moment its result has been copied out."""
     stream = synth.serving_stream(rate, seconds)
     pool = synth.serving_pool()
     reqs = []
@@ -104,44 +108,50 @@ def serving_leg(A, synth, torch, dev, rate, seconds, iterations=500):
                                        params={"maxIt": iterations})))
         del op
     torch.cuda.synchronize()
-    eng = A.Engine(use_graphs=True)
+    eng = A.Engine(use_graphs=True, resident_bytes_cap=int(cap_bytes), max_queue=max_queue)
     outs = {}
-    lat, waiting, running = [], list(range(len(reqs))), {}
+    lat, held, rejected = [], {}, 0
+    nxt, steps, max_queued = 0, 0, 0
     t0 = time.perf_counter()
-    steps = 0
-    while waiting or running:
+    while nxt < len(reqs) or held:
         now = time.perf_counter() - t0
-        while waiting and reqs[waiting[0]]["t"] <= now:
-            i = waiting[0]
-            rid = eng.submit(reqs[i]["req"])
+        while nxt < len(reqs) and reqs[nxt]["t"] <= now:
+            rid = eng.submit(reqs[nxt]["req"])
             if rid is None:
-                break  # HBM budget full: retry after completions
-            running[rid] = i
-            waiting.pop(0)
-        if not running:
-            time.sleep(max(0.0, reqs[waiting[0]]["t"] - (time.perf_counter() - t0)))
+                rejected += 1
+            else:
+                held[rid] = nxt
+            nxt += 1
+        if not held:
+            time.sleep(max(0.0, reqs[nxt]["t"] - (time.perf_counter() - t0)))
             continue
         eng.step()
         steps += 1
-        for rid, i in list(running.items()):
+        max_queued = max(max_queued, eng.counters()["queued"])
+        for rid, i in list(held.items()):
             r = reqs[i]
             if i not in outs:
                 outs[i] = torch.empty((r["T"], r["n"], r["n"]), dtype=torch.complex64, device=dev)
             if eng.poll(rid, outs[i]):
                 lat.append(time.perf_counter() - t0 - r["t"])
-                del running[rid]
+                del held[rid]
     makespan = time.perf_counter() - t0
-    lat_ms = np.array(lat) * 1e3
-    total_its = iterations * len(reqs)
-    vox_its = sum(r["n"] * r["n"] * r["T"] for r in reqs) * iterations
+    c = eng.counters()
+    done = list(outs)
+    lat_ms = np.array(lat) * 1e3 if lat else np.zeros(1)
+    total_its = iterations * len(lat)
+    vox_its = sum(reqs[i]["n"] ** 2 * reqs[i]["T"] for i in done) * iterations
     finite = all(bool(torch.isfinite(torch.view_as_real(o)).all()) for o in outs.values())
-    return {"requests": len(reqs), "rate_per_s": rate, "window_s": seconds,
-            "iterations_each": iterations, "engine_steps": steps,
+    return {"rate_per_s": rate, "window_s": seconds, "requests": len(reqs),
+            "completed": len(lat), "rejected": rejected,
+            "resident_bytes_cap": int(cap_bytes), "max_queue": max_queue,
+            "max_queued": max_queued, "iterations_each": iterations, "engine_steps": steps,
             "makespan_s": round(makespan, 3),
             "p50_latency_ms": round(float(np.percentile(lat_ms, 50)), 1),
             "p99_latency_ms": round(float(np.percentile(lat_ms, 99)), 1),
             "iters_per_s": round(total_its / makespan, 1),
             "voxel_iters_per_s": round(vox_its / makespan, 1),
+            "graph_captures": c["graph_captures"], "capture_ms": round(c["capture_ms"], 1),
             "all_finite": finite}
 
 
@@ -427,8 +437,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target length of the all-cores CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
-                    help="arrival window of the serving stream (0: skip the serving leg)")
-    ap.add_argument("--serve-rate", type=float, default=50.0)
+                    help="arrival window of each serving run (0: skip the serving leg)")
+    ap.add_argument("--serve-rates", default="10,15,20,50",
+                    help="Poisson arrival rates (requests/s) of the serving load sweep")
+    ap.add_argument("--serve-cap-gib", type=float, default=4.0,
+                    help="running requests' bytes admitted at once (engine resident_bytes_cap)")
+    ap.add_argument("--serve-queue", type=int, default=16,
+                    help="waiting requests the engine queues before refusing (load shedding)")
     ap.add_argument("--small-batch", type=int, default=64,
                     help="reconstructions of the TV 128x128x12x8 workload batched in the "
                          "small-request leg (0: skip)")
@@ -524,8 +539,12 @@ def main():
 
     serving = None
     if args.serve_seconds > 0 and not args.no_legs:
-        log(f"serving stream ({args.serve_rate}/s for {args.serve_seconds}s)")
-        serving = serving_leg(A, synth, torch, dev, args.serve_rate, args.serve_seconds)
+        serving = []
+        for rate in [float(r) for r in args.serve_rates.split(",") if r]:
+            log(f"serving stream ({rate}/s for {args.serve_seconds}s)")
+            serving.append(serving_leg(A, synth, torch, dev, rate, args.serve_seconds,
+                                       cap_bytes=args.serve_cap_gib * (1 << 30),
+                                       max_queue=args.serve_queue))
 
     cpu = None
     if not args.no_cpu:

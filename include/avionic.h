@@ -408,12 +408,35 @@ int avionic_bin_write(const char *path, const double *re, const double *im,
 
 typedef struct avionic_engine avionic_engine;
 
+/* Admission.  A submitted request is held (its inputs copied, its state
+ * allocated) while the bytes of all held requests stay within
+ * hbm_budget_bytes; otherwise submit returns AVIONIC_E_CAPACITY.  Held
+ * requests join the batched launch set in submission order (FIFO) as long as
+ * the running requests number at most max_batch and their bytes stay within
+ * resident_bytes_cap, which bounds the work of one step and so the latency of
+ * each running request; the others wait in the queue and are admitted by
+ * avionic_engine_step as running requests finish.  A request always runs
+ * when nothing else does. */
 typedef struct avionic_engine_config {
-  size_t hbm_budget_bytes;  /* admission limit for resident requests */
+  size_t hbm_budget_bytes;  /* bytes of held requests; 0: 90% of the device
+                               memory free at creation */
   int use_graphs;           /* capture each iteration in a hipGraph */
-  int max_batch;            /* max requests in one batched launch */
+  int max_batch;            /* max running requests; <= 0: no limit */
   int iterations_per_tick;  /* iterations run per engine step */
+  size_t resident_bytes_cap; /* bytes of running requests; 0: no limit */
+  int max_queue;            /* waiting requests beyond which submit returns
+                               AVIONIC_E_CAPACITY; < 0: no limit */
 } avionic_engine_config;
+
+/* Engine counters (avionic_engine_counters). */
+typedef struct avionic_engine_counts {
+  int held;                 /* requests held: queued, running or finished */
+  int running;              /* in the launch set */
+  int queued;               /* waiting for admission */
+  size_t bytes_held, bytes_running, hbm_budget_bytes;
+  long long graph_captures; /* iteration graphs captured and instantiated */
+  double capture_ms;        /* host time spent capturing them */
+} avionic_engine_counts;
 
 typedef struct avionic_request_desc {
   avionic_method method;
@@ -443,21 +466,24 @@ int avionic_engine_create(const avionic_engine_config *cfg,
 int avionic_engine_destroy(avionic_engine *engine);
 /* HBM bytes a request of this shape keeps resident. */
 size_t avionic_request_footprint(const avionic_request_desc *desc);
-/* Admit a request (copies its inputs).  AVIONIC_E_CAPACITY if it does not fit
- * the remaining budget now; the caller may retry after others finish. */
+/* Hold a request (copies its inputs) and admit it now or queue it (see
+ * avionic_engine_config).  AVIONIC_E_CAPACITY if it does not fit the HBM
+ * budget or the queue now; the caller may retry after others finish. */
 int avionic_engine_submit(avionic_engine *engine,
                           const avionic_request_desc *desc, long long *id);
-/* Run up to iterations_per_tick iterations of every resident request as one
- * batched launch sequence. Returns the number of resident requests. */
+/* Admit queued requests that fit, then run up to iterations_per_tick
+ * iterations of every running request as one batched launch sequence.
+ * Returns the number of requests held. */
 int avionic_engine_step(avionic_engine *engine, void *stream);
 /* 1 done (result copied to x_out, device [frames][height][width]), 0 running,
  * AVIONIC_E_NOTFOUND unknown.  A finished request is released on retrieval. */
 int avionic_engine_poll(avionic_engine *engine, long long id, float *x_out,
                         void *stream);
 int avionic_engine_cancel(avionic_engine *engine, long long id);
-/* Requests resident / bytes in use. */
+/* Requests held / bytes in use. */
 int avionic_engine_stats(const avionic_engine *engine, int *resident,
                          size_t *bytes_in_use);
+int avionic_engine_counters(const avionic_engine *engine, avionic_engine_counts *counts);
 /* Average duration in ms of each kernel of one iteration (dual,
  * col_forward, row_dual_update, col_adjoint, primal, step_reg,
  * col_forward_delta, row_norm2, tail) over the first resident batch, each

@@ -176,6 +176,46 @@ def test_engine_admission_cancel_and_errors(gpu):
     assert e.stats() == (0, 0)
 
 
+def test_engine_fifo_queue_and_work_cap(gpu):
+    """Held requests join the launch set in FIFO order within the
+    resident-bytes cap; max_queue bounds the waiting ones; each result equals
+    the same request run alone."""
+    import torch
+    import avionic_amd as A
+    data, b1, mask = _inputs(3, 16, 16, 2, seed=9)
+    reqs = [A.Request(0, _dev(torch, data), _dev(torch, b1), mask=_dev(torch, mask, np.float32),
+                      params=dict(maxIt=m)) for m in (20, 30, 25)]
+    fp = A.Engine.footprint(reqs[1])
+    e = A.Engine(use_graphs=True, resident_bytes_cap=int(fp * 1.5), max_queue=1)
+    a = e.submit(reqs[0])
+    b = e.submit(reqs[1])
+    assert a is not None and b is not None
+    assert e.submit(reqs[2]) is None  # one running, the queue (1) is full
+    c = e.counters()
+    assert (c["held"], c["running"], c["queued"]) == (2, 1, 1)
+    assert c["hbm_budget_bytes"] > 0  # the default budget from free memory
+    out = {k: torch.empty(3, 16, 16, dtype=torch.complex64, device="cuda") for k in (a, b)}
+    finished, steps = [], 0
+    while len(finished) < 2:
+        e.step()
+        steps += 1
+        c = e.counters()
+        assert c["running"] <= 1
+        for k in (a, b):
+            if k not in finished and e.poll(k, out[k]):
+                finished.append(k)
+    assert finished == [a, b]  # FIFO: b ran after a
+    assert steps >= 20 + 30
+    for k, r in ((a, reqs[0]), (b, reqs[1])):
+        alone = A.Engine(use_graphs=False, max_batch=1)
+        i = alone.submit(r)
+        x = torch.empty(3, 16, 16, dtype=torch.complex64, device="cuda")
+        while not alone.poll(i, x):
+            alone.step()
+        assert torch.equal(x, out[k])
+    assert e.counters()["graph_captures"] >= 2
+
+
 def test_engine_step_log_matches_oracle(gpu):
     import torch
     import avionic_amd as A
