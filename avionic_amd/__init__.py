@@ -584,7 +584,7 @@ class _ReqDesc(ctypes.Structure):
     _fields_ = [("method", _I), ("width", _U), ("height", _U), ("coils", _U), ("frames", _U),
                 ("params", ctypes.POINTER(_Params)), ("mask", _P), ("centered", _I),
                 ("data", _P), ("b1", _P), ("x0", _P), ("stream", _P), ("verbose", _I),
-                ("debug", _I), ("debugstep", _I)]
+                ("debug", _I), ("debugstep", _I), ("op", _P)]
 
 
 _ENGINE_SIGS = {
@@ -617,13 +617,21 @@ def _engine_lib():
 
 
 class Request:
-    """Inputs of one reconstruction request (device tensors)."""
+    """Inputs of one reconstruction request (device tensors).  Without op the
+    engine builds a Cartesian operator from mask / centered (data [T, C, H, W]);
+    with op (a CartesianOperator or NoncartesianOperator that outlives the
+    request) data is that operator's k-space, e.g. [T, C, M] for radial data."""
 
     def __init__(self, method, data, b1, mask=None, x0=None, centered=False, params=None,
-                 verbose=False, debug=False, debugstep=1):
+                 verbose=False, debug=False, debugstep=1, op=None):
         self.method = int(method)
         self.verbose, self.debug, self.debugstep = bool(verbose), bool(debug), int(debugstep)
-        T, C, H, W = data.shape
+        self.op = op
+        if op is not None:
+            C, H, W = b1.shape
+            T = data.shape[0]
+        else:
+            T, C, H, W = data.shape
         self.shape = (W, H, C, T)
         self.data, self.b1, self.mask, self.x0 = data, b1, mask, x0
         self.centered = bool(centered)
@@ -646,6 +654,7 @@ class Request:
         d.x0 = _c64(self.x0)
         d.stream = _stream()
         d.verbose, d.debug, d.debugstep = int(self.verbose), int(self.debug), self.debugstep
+        d.op = self.op.handle if self.op is not None else None
         return d
 
 

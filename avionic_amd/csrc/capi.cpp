@@ -23,9 +23,6 @@ void cartesian_backward_noalloc(const CartesianOperator &op, const float2 *img, 
 
 using namespace avionic;
 
-struct avionic_operator {
-  Operator *op;
-};
 struct avionic_recon {
   Recon *r;
 };

@@ -22,7 +22,6 @@ void coil_time_averaged(const Operator &op, const float2 *kdata, float2 *u, floa
 // The all-frames, one-coil gridding operator of a non-Cartesian operator
 // (noncartesian.cpp); nullptr for any other operator.
 Operator *make_noncartesian_all_frames(const Operator &op);
-bool is_noncartesian(const Operator &op);
 void coil_b1_from_uh1(const CoilCtx &k, const float2 *u, const float2 *crec, double mu, float2 *b1);
 int coil_find_max_sum(const CoilCtx &k, const float2 *crec);
 void coil_utgv2(const CoilCtx &k, const float2 *b1, const float2 *crec, const std::vector<int> &inds,

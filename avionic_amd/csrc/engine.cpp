@@ -36,7 +36,9 @@ struct Slot {
   int ds = 0;  // dual storage
   int W = 0, H = 0, C = 0, T = 0;
   long N = 0, K = 0;
-  std::unique_ptr<CartesianOperator> op;
+  std::unique_ptr<CartesianOperator> own_op;  // built from the request's mask
+  Operator *op = nullptr;                     // the request's operator
+  const CartesianOperator *cart = nullptr;    // op when it runs the task-list passes
   DevBuf x, ext, y, z, d, khz, kcol, zhat, sc, slog, tlog, preg, pk, b1;
   avionic_params params{};
   int maxit = 0, k_host = 0;
@@ -210,7 +212,7 @@ struct avionic_engine {
     gs.alpha1 = s.params.alpha1;
     gs.alpha = s.params.alpha;
     gs.lambda = s.params.lambda;
-    gs.op = s.op.get();
+    gs.op = s.op;
     gs.x = s.x.as<float2>();
     gs.y = s.y.p;
     gs.z = s.z.as<float2>();
@@ -234,11 +236,33 @@ struct avionic_engine {
     }
   }
 
+  // Data-term work of the group's non-Cartesian requests, launched from
+  // launch_iteration (and captured with it): their operator's dual update
+  // (z prox and K^H z), and on step iterations K(ext - x) norm partials.
+  static void nc_hook(void *ctx, int phase, hipStream_t st) {
+    const Group &g = *static_cast<const Group *>(ctx);
+    for (Slot *s : g.slots) {
+      if (s->cart) continue;
+      const Task &t = s->task;
+      if (phase == 0)
+        s->op->dual_update(t.ext, t.b1, t.z, t.d, t.khz, t.sc, st);
+      else
+        (void)s->op->norm2(t.ext, t.x, t.b1, t.part_k, st);
+    }
+  }
+
   void run_iteration(Group &g, bool with_step) {
     const int n = (int)g.slots.size();
     const IterStreams st{stream, side, ev_fork, ev_join};
+    IterHooks hooks;
+    for (Slot *s : g.slots)
+      if (!s->cart) {
+        hooks.fn = &avionic_engine::nc_hook;
+        hooks.ctx = &g;
+        break;
+      }
     if (!cfg.use_graphs) {
-      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st);
+      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st, hooks);
       return;
     }
     hipGraphExec_t &ex = with_step ? g.g_step : g.g_plain;
@@ -246,7 +270,7 @@ struct avionic_engine {
       const auto t0 = std::chrono::steady_clock::now();
       hipGraph_t graph = nullptr;
       AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st);
+      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st, hooks);
       AV_HIP(hipStreamEndCapture(stream, &graph));
       AV_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);
@@ -359,13 +383,19 @@ int avionic_engine_destroy(avionic_engine *engine) {
 size_t avionic_request_footprint(const avionic_request_desc *d) {
   try {
     check_desc(d);
-    CartPlan plan;
-    plan.init((int)d->width, (int)d->height, (int)d->coils, (int)d->frames, d->mask, d->centered,
-              nullptr);
+    long K = 0;
+    if (d->op && d->op->op) {
+      K = d->op->op->compact_elems();
+    } else {
+      CartPlan plan;
+      plan.init((int)d->width, (int)d->height, (int)d->coils, (int)d->frames, d->mask,
+                d->centered, nullptr);
+      K = plan.kspace_elems();
+    }
     const long N = (long)d->width * d->height * d->frames;
     const int maxit = d->params ? (int)d->params->maxIt : 500;
     const int ds = d->params ? d->params->dualStorage : 0;
-    return slot_bytes(kernel_method((int)d->method), ds, N, plan.kspace_elems(),
+    return slot_bytes(kernel_method((int)d->method), ds, N, K,
                       (long)d->coils * d->height * d->width, maxit);
   } catch (...) {
     return 0;
@@ -381,7 +411,19 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->method = kernel_method((int)d->method);
     s->W = (int)d->width; s->H = (int)d->height; s->C = (int)d->coils; s->T = (int)d->frames;
     s->N = (long)s->W * s->H * s->T;
-    s->op = std::make_unique<CartesianOperator>(s->W, s->H, s->C, s->T, d->mask, d->centered);
+    if (d->op) {
+      AV_CHECK(d->op->op, "request operator is null");
+      s->op = d->op->op;
+      AV_CHECK(s->op->W == s->W && s->op->H == s->H && s->op->C == s->C && s->op->T == s->T,
+               "request operator dimensions differ from the request's");
+      s->cart = dynamic_cast<const CartesianOperator *>(s->op);
+      AV_CHECK(s->cart || is_noncartesian(*s->op),
+               "engine requests take Cartesian 2D+t or non-Cartesian operators");
+    } else {
+      s->own_op = std::make_unique<CartesianOperator>(s->W, s->H, s->C, s->T, d->mask, d->centered);
+      s->op = s->own_op.get();
+      s->cart = s->own_op.get();
+    }
     s->K = s->op->compact_elems();
     s->params = request_params(d, *s->op);
     s->ds = s->params.dualStorage;
@@ -417,9 +459,10 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     s->tlog.ensure(sizeof(double) * std::max(1, s->maxit));
     const Grid3 g{s->W, s->H, s->T, N};
     const int nvb = step_blocks(g);
-    const int nrb = s->op->plan().fused_row_blocks();
+    const int nrb = s->cart ? s->cart->plan().fused_row_blocks() : 0;
+    const int npk = s->cart ? nrb : s->op->max_partials();
     s->preg.ensure(sizeof(double) * 2 * nvb);
-    s->pk.ensure(sizeof(double) * std::max(1, nrb));
+    s->pk.ensure(sizeof(double) * std::max(1, npk));
     if (s->gap_enabled()) {
       const size_t np = gap_partials(g, *s->op);
       s->gap_a.ensure(sizeof(double) * np);
@@ -462,7 +505,7 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     Task &t = s->task;
     t.g = g;
     t.rp = make_reg_params(s->method, d6, s->params.alpha0, s->params.alpha1, s->params.alpha);
-    t.cg = s->op->plan().g;
+    if (s->cart) t.cg = s->cart->plan().g;  // else zero: no column / row blocks
     t.x = s->x.as<float2>();
     t.ext = s->ext.as<float2>();
     t.y = s->y.p;
@@ -478,8 +521,9 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     t.part_reg = s->preg.as<double>();
     t.part_k = s->pk.as<double>();
     t.nvox_blocks = nvb;
-    t.ncol_blocks = t.cg.nxbf * t.cg.T;
+    t.ncol_blocks = s->cart ? t.cg.nxbf * t.cg.T : 0;
     t.nrow_blocks = nrb;
+    t.npart_k = npk;
     t.schedule = s->params.adaptSchedule;
 
     s->id = e->next_id++;

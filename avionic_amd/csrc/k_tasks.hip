@@ -236,7 +236,7 @@ __global__ __launch_bounds__(RT) void kt_tail(const Task *__restrict__ tasks, in
       a += T.part_reg[2 * j];
       b += T.part_reg[2 * j + 1];
     }
-    for (int j = tid; j < T.nrow_blocks; j += RT) a += T.part_k[j];
+    for (int j = tid; j < T.npart_k; j += RT) a += T.part_k[j];
     ra[tid] = a;
     rb[tid] = b;
     __syncthreads();
@@ -362,7 +362,7 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
 }
 
 void launch_iteration(int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
-                      bool with_step, const IterStreams &st) {
+                      bool with_step, const IterStreams &st, const IterHooks &hooks) {
   if (n <= 0) return;
   hipStream_t s = st.main;
   const bool two = st.side != nullptr;
@@ -421,6 +421,7 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   launch_row<kRowDualUpdate>(gr, d_tasks, n, s);
   launch_col_adj(gr, d_tasks, n, s);
   AV_LAUNCH_CHECK();
+  if (hooks.fn) hooks.fn(hooks.ctx, 0, s);
   join();
   if (gr.vox > 0) AV_BYMETHOD(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();
@@ -437,6 +438,7 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
     launch_col_fwd<true>(gr, d_tasks, n, s);
     launch_row<kRowNorm2>(gr, d_tasks, n, s);
     AV_LAUNCH_CHECK();
+    if (hooks.fn) hooks.fn(hooks.ctx, 1, s);
     join();
   }
   hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n);

@@ -140,6 +140,9 @@ double regularizer_norm_of(const GapState &st, hipStream_t s);
 double data_fidelity_of(const GapState &st, hipStream_t s);
 
 // Primal-dual solver (PDRecon + the three regularisers).
+// True for operators built by avionic_noncartesian_operator_create.
+bool is_noncartesian(const Operator &op);
+
 class Recon {
  public:
   Recon(int method, int W, int H, int C, int T, const avionic_params *params, Operator *op);
@@ -178,3 +181,8 @@ class Recon {
 };
 
 }  // namespace avionic
+
+// The C-ABI operator handle (include/avionic.h).
+struct avionic_operator {
+  avionic::Operator *op;
+};

@@ -35,8 +35,9 @@ struct Task {
   DevScalars *sc;
   double *sig_log, *tau_log;  // indexed by iteration
   double *part_reg;    // 2 per voxel block
-  double *part_k;      // 1 per row block
+  double *part_k;      // K-norm partials of the step rule (npart_k of them)
   int nvox_blocks, ncol_blocks, nrow_blocks;
+  int npart_k;                // Cartesian: nrow_blocks; other operators: their own
   int b_vox, b_col, b_row;    // first block of this task per kernel family
   int schedule;               // 0 reference schedule, 1 every iteration
 };
@@ -61,8 +62,16 @@ struct IterStreams {
   hipStream_t main = nullptr, side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
 };
+// Work of tasks whose operator has no task-list passes (non-Cartesian
+// operators, ncol_blocks = nrow_blocks = 0): fn(ctx, 0, main) runs their
+// data-term dual update after the Cartesian z chain, fn(ctx, 1, main) their
+// step-rule K norms after the Cartesian ones; both before the join.
+struct IterHooks {
+  void (*fn)(void *ctx, int phase, hipStream_t s) = nullptr;
+  void *ctx = nullptr;
+};
 void launch_iteration(int method, int dual_storage, const Task *d_tasks, int n, const TaskGrid &grid,
-                      bool with_step, const IterStreams &st);
+                      bool with_step, const IterStreams &st, const IterHooks &hooks = IterHooks{});
 
 // Kernel families of one iteration, in launch order.
 enum KernelId {

@@ -459,6 +459,13 @@ typedef struct avionic_request_desc {
   int verbose;
   int debug;
   int debugstep;                 /* <= 0 reads as 1 */
+  /* Optional operator of the request (NULL: a Cartesian operator built
+   * from mask / centered).  A Cartesian 2D+t or non-Cartesian operator of
+   * the request's width, height, coils and frames; it must outlive the
+   * request, and requests sharing one operator share its scratch (they run
+   * one after the other on the engine's stream).  data is the operator's
+   * dense k-space ([frames][coils][M] for non-Cartesian operators). */
+  const avionic_operator *op;
 } avionic_request_desc;
 
 int avionic_engine_create(const avionic_engine_config *cfg,
