@@ -571,7 +571,8 @@ def fft2(a, sign=-1, centered=False, unitary=False):
 # ------------------------------------------------------- batching / serving
 class _EngineConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", _SZ), ("use_graphs", _I), ("max_batch", _I),
-                ("iterations_per_tick", _I), ("resident_bytes_cap", _SZ), ("max_queue", _I)]
+                ("iterations_per_tick", _I), ("resident_bytes_cap", _SZ), ("max_queue", _I),
+                ("voxels_per_lane", _I)]
 
 
 class _EngineCounts(ctypes.Structure):
@@ -664,13 +665,16 @@ class Engine:
     FIFO order while the running requests stay within max_batch and
     resident_bytes_cap, and advance together in task-list launches
     (hipGraph-replayed when use_graphs).  max_queue: waiting requests beyond
-    which submit returns None (default 0: no queue; < 0: unbounded)."""
+    which submit returns None (default 0: no queue; < 0: unbounded).
+    voxels_per_lane: 1 runs the dual / primal updates with one voxel per lane;
+    0 (default) with a pair of voxels per lane for requests of even width."""
 
     def __init__(self, hbm_budget_bytes=0, use_graphs=True, max_batch=0, iterations_per_tick=1,
-                 resident_bytes_cap=0, max_queue=0):
+                 resident_bytes_cap=0, max_queue=0, voxels_per_lane=0):
         L = _engine_lib()
         cfg = _EngineConfig(int(hbm_budget_bytes), int(bool(use_graphs)), int(max_batch),
-                            int(iterations_per_tick), int(resident_bytes_cap), int(max_queue))
+                            int(iterations_per_tick), int(resident_bytes_cap), int(max_queue),
+                            int(voxels_per_lane))
         h = ctypes.c_void_p()
         _check(L.avionic_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -46,6 +46,15 @@ __host__ __device__ inline float2 cmulc(float2 a, float2 b) {
 __host__ __device__ inline float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 __host__ __device__ inline float2 czero() { return make_float2(0.f, 0.f); }
 
+// Two complex values of consecutive elements (x, x + 1) handled by one lane;
+// the arithmetic is elementwise, in the same order as on one value.
+struct C2 {
+  float2 a, b;
+};
+__host__ __device__ inline C2 cadd(C2 p, C2 q) { return {cadd(p.a, q.a), cadd(p.b, q.b)}; }
+__host__ __device__ inline C2 csub(C2 p, C2 q) { return {csub(p.a, q.a), csub(p.b, q.b)}; }
+__host__ __device__ inline C2 cscale(C2 p, float s) { return {cscale(p.a, s), cscale(p.b, s)}; }
+
 // ------------------------------------------------------ dual storage types
 // bfloat16 complex pair: round-to-nearest-even from float32.
 struct bf16x2 {
@@ -106,13 +115,21 @@ __device__ __forceinline__ void gst(T *base, unsigned i, const T &v) {
   base[i] = v;
 #endif
 }
-// Streaming (non-temporal) load of a 4- or 8-byte value that no other lane
-// reads.
+// Integer (or integer vector) type of a 4-, 8- or 16-byte value, the operand
+// type of the non-temporal builtins.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+using StreamWord = typename std::conditional<
+    sizeof(T) == 16, u32x4,
+    typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type>::type;
+
+// Streaming (non-temporal) load of a 4-, 8- or 16-byte value that no other
+// lane reads.
 template <class T>
 __device__ __forceinline__ T gld_stream(const T *base, unsigned i) {
-  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "4-, 8- or 16-byte values");
 #if defined(__HIP_DEVICE_COMPILE__)
-  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  typedef StreamWord<T> U;
   typedef __attribute__((address_space(1))) const char gchar;
   typedef __attribute__((address_space(1))) const U gU;
   return __builtin_bit_cast(T, __builtin_nontemporal_load((gU *)((gchar *)base + i * (unsigned)sizeof(T))));
@@ -121,19 +138,37 @@ __device__ __forceinline__ T gld_stream(const T *base, unsigned i) {
 #endif
 }
 
-// Streaming (non-temporal) store of a 4- or 8-byte value, for outputs that
-// are not read again before they would have left the caches.
+// Streaming (non-temporal) store of a 4-, 8- or 16-byte value, for outputs
+// that are not read again before they would have left the caches.
 template <class T>
 __device__ __forceinline__ void gst_stream(T *base, unsigned i, const T &v) {
-  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "4-, 8- or 16-byte values");
 #if defined(__HIP_DEVICE_COMPILE__)
-  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  typedef StreamWord<T> U;
   typedef __attribute__((address_space(1))) char gchar;
   typedef __attribute__((address_space(1))) U gU;
   __builtin_nontemporal_store(__builtin_bit_cast(U, v), (gU *)((gchar *)base + i * (unsigned)sizeof(T)));
 #else
   base[i] = v;
 #endif
+}
+
+// Both complex values of a pair of elements (i, i + 1), i even, in one
+// 16-byte access (the base must be 16-byte aligned).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ C2 c2_of(f32x4 v) {
+  return {make_float2(v.x, v.y), make_float2(v.z, v.w)};
+}
+__device__ __forceinline__ f32x4 f4_of(C2 v) { return f32x4{v.a.x, v.a.y, v.b.x, v.b.y}; }
+__device__ __forceinline__ C2 gld2(const float2 *base, unsigned i) {
+  return c2_of(gld(reinterpret_cast<const f32x4 *>(base), i >> 1));
+}
+__device__ __forceinline__ C2 gld2_stream(const float2 *base, unsigned i) {
+  return c2_of(gld_stream(reinterpret_cast<const f32x4 *>(base), i >> 1));
+}
+__device__ __forceinline__ void gst2_stream(float2 *base, unsigned i, C2 v) {
+  gst_stream(reinterpret_cast<f32x4 *>(base), i >> 1, f4_of(v));
 }
 
 // Dual storage kinds (avionic_params::dualStorage).
@@ -151,6 +186,12 @@ struct DualIO<float2> {
     return gld_stream(p, i);
   }
   __device__ static inline void store(float2 *p, unsigned i, float2 v, float) { gst_stream(p, i, v); }
+  // pairs (i, i + 1), i even
+  __device__ static inline C2 load2(const float2 *p, unsigned i, float) { return gld2(p, i); }
+  __device__ static inline C2 load2_stream(const float2 *p, unsigned i, float) {
+    return gld2_stream(p, i);
+  }
+  __device__ static inline void store2(float2 *p, unsigned i, C2 v, float) { gst2_stream(p, i, v); }
 };
 template <>
 struct DualIO<bf16x2> {
@@ -163,9 +204,23 @@ struct DualIO<bf16x2> {
   __device__ static inline float2 decode(uint32_t w) {
     return make_float2(u2f(w << 16), u2f(w & 0xffff0000u));
   }
+  __device__ static inline uint32_t encode(float2 v) {
+    return (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
+  }
   __device__ static inline void store(bf16x2 *p, unsigned i, float2 v, float) {
-    const uint32_t w = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
-    gst_stream(reinterpret_cast<uint32_t *>(p), i, w);
+    gst_stream(reinterpret_cast<uint32_t *>(p), i, encode(v));
+  }
+  // pairs (i, i + 1), i even: one 8-byte access
+  __device__ static inline C2 load2(const bf16x2 *p, unsigned i, float) {
+    const u32x2 w = gld(reinterpret_cast<const u32x2 *>(p), i >> 1);
+    return {decode(w.x), decode(w.y)};
+  }
+  __device__ static inline C2 load2_stream(const bf16x2 *p, unsigned i, float) {
+    const u32x2 w = gld_stream(reinterpret_cast<const u32x2 *>(p), i >> 1);
+    return {decode(w.x), decode(w.y)};
+  }
+  __device__ static inline void store2(bf16x2 *p, unsigned i, C2 v, float) {
+    gst_stream(reinterpret_cast<u32x2 *>(p), i >> 1, u32x2{encode(v.a), encode(v.b)});
   }
 };
 template <>
@@ -183,11 +238,26 @@ struct DualIO<q16x2> {
   // The encode scale 32767 / r through the hardware reciprocal (1 ulp): the
   // stored value may differ from the correctly rounded quotient's by one
   // step of the int16 grid, far below the format's own resolution.
+  __device__ static inline uint32_t encode(float2 v, float inv) {
+    return (uint32_t)(uint16_t)f32_to_q16(v.x, inv) |
+           ((uint32_t)(uint16_t)f32_to_q16(v.y, inv) << 16);
+  }
   __device__ static inline void store(q16x2 *p, unsigned i, float2 v, float r) {
     const float inv = kQ16Max * __builtin_amdgcn_rcpf(r);
-    const uint32_t w = (uint32_t)(uint16_t)f32_to_q16(v.x, inv) |
-                       ((uint32_t)(uint16_t)f32_to_q16(v.y, inv) << 16);
-    gst_stream(reinterpret_cast<uint32_t *>(p), i, w);
+    gst_stream(reinterpret_cast<uint32_t *>(p), i, encode(v, inv));
+  }
+  // pairs (i, i + 1), i even: one 8-byte access
+  __device__ static inline C2 load2(const q16x2 *p, unsigned i, float r) {
+    const u32x2 w = gld(reinterpret_cast<const u32x2 *>(p), i >> 1);
+    return {decode(w.x, r), decode(w.y, r)};
+  }
+  __device__ static inline C2 load2_stream(const q16x2 *p, unsigned i, float r) {
+    const u32x2 w = gld_stream(reinterpret_cast<const u32x2 *>(p), i >> 1);
+    return {decode(w.x, r), decode(w.y, r)};
+  }
+  __device__ static inline void store2(q16x2 *p, unsigned i, C2 v, float r) {
+    const float inv = kQ16Max * __builtin_amdgcn_rcpf(r);
+    gst_stream(reinterpret_cast<u32x2 *>(p), i >> 1, u32x2{encode(v.a, inv), encode(v.b, inv)});
   }
 };
 

@@ -168,12 +168,14 @@ struct avionic_engine {
       return ka != kb ? ka < kb : row_kind(a->task.cg) < row_kind(b->task.cg);
     });
     std::vector<Task> tasks;
-    int bv = 0, bc = 0, br = 0;
+    int bv = 0, bd1 = 0, bd2 = 0, bc = 0, br = 0;
     g.grid = TaskGrid{};
     bool col_seen[kColKinds] = {}, row_seen[kRowKinds] = {};
     for (Slot *s : g.slots) {
       Task t = s->task;
       t.b_vox = bv;
+      t.b_dp1 = bd1;
+      t.b_dp2 = bd2;
       t.b_col = bc;
       t.b_row = br;
       const int ck = col_kind(t.cg), rk = row_kind(t.cg);
@@ -181,6 +183,8 @@ struct avionic_engine {
       if (!row_seen[rk]) g.grid.row_lo[rk] = br;
       col_seen[ck] = row_seen[rk] = true;
       bv += t.nvox_blocks;
+      bd1 += t.ndp1_blocks;
+      bd2 += t.ndp2_blocks;
       bc += t.ncol_blocks;
       br += t.nrow_blocks;
       g.grid.col_hi[ck] = bc;
@@ -188,6 +192,8 @@ struct avionic_engine {
       tasks.push_back(t);
     }
     g.grid.vox = bv;
+    g.grid.dp1 = bd1;
+    g.grid.dp2 = bd2;
     g.grid.col = bc;
     g.grid.row = br;
     if (!tasks.empty()) {
@@ -521,6 +527,9 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     t.part_reg = s->preg.as<double>();
     t.part_k = s->pk.as<double>();
     t.nvox_blocks = nvb;
+    const bool pairs = e->cfg.voxels_per_lane != 1 && g.W % 2 == 0;
+    t.ndp1_blocks = pairs ? 0 : nvb;
+    t.ndp2_blocks = pairs ? pair_blocks(g) : 0;
     t.ncol_blocks = s->cart ? t.cg.nxbf * t.cg.T : 0;
     t.nrow_blocks = nrb;
     t.npart_k = npk;

@@ -4,27 +4,37 @@
 // reg_device.hpp.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "reg_device.hpp"
 
 namespace avionic {
 
-template <int M, typename DT>
+// One voxel per lane (VPL = 1), or a pair of voxels of a row (VPL = 2, even
+// widths), as in the task-list kernels.
+__device__ inline bool block_voxels(const Grid3 &g, Vox &v) {
+  return vox_of_block<false>(g, blockIdx.x, threadIdx.x, v);
+}
+__device__ inline bool block_voxels(const Grid3 &g, Vox2 &v) {
+  return vox2_of_block(g, blockIdx.x, threadIdx.x, v);
+}
+
+template <int M, typename DT, int VPL>
 __global__ __launch_bounds__(RT) void k_dual(Grid3 g, RegParams rp, const float2 *__restrict__ ext,
                                              DT *__restrict__ y, const DevScalars *__restrict__ sc) {
-  Vox v;
-  if (!vox_of_block<false>(g, blockIdx.x, threadIdx.x, v)) return;
+  typename std::conditional<VPL == 2, Vox2, Vox>::type v;
+  if (!block_voxels(g, v)) return;
   dual_voxel<M, DT>(g, rp, ext, y, (float)sc->sigma, v);
 }
 
-template <int M, typename DT>
+template <int M, typename DT, int VPL>
 __global__ __launch_bounds__(RT) void k_primal(Grid3 g, RegParams rp, float2 *__restrict__ x,
                                                float2 *__restrict__ ext, const DT *__restrict__ y,
                                                const float2 *__restrict__ khz,
                                                const DevScalars *__restrict__ sc) {
-  Vox v;
-  if (!vox_of_block<false>(g, blockIdx.x, threadIdx.x, v)) return;
+  typename std::conditional<VPL == 2, Vox2, Vox>::type v;
+  if (!block_voxels(g, v)) return;
   primal_voxel<M, DT>(g, rp, x, ext, y, khz, (float)sc->tau, v);
 }
 
@@ -251,8 +261,15 @@ static inline unsigned nblk(long n) { return (unsigned)((n + RT - 1) / RT); }
 void launch_dual(int method, int ds, const Grid3 &g, const RegParams &rp, const float2 *ext, void *y,
                  const DevScalars *sc, hipStream_t s) {
   check_grid(g);
-  dim3 grid((unsigned)step_blocks(g)), block(RT);
-#define AV_DUAL(M, DT) hipLaunchKernelGGL((k_dual<M, DT>), grid, block, 0, s, g, rp, ext, (DT *)y, sc)
+  const bool pairs = g.W % 2 == 0;
+  dim3 grid((unsigned)(pairs ? pair_blocks(g) : step_blocks(g))), block(RT);
+#define AV_DUAL(M, DT)                                                                      \
+  do {                                                                                      \
+    if (pairs)                                                                              \
+      hipLaunchKernelGGL((k_dual<M, DT, 2>), grid, block, 0, s, g, rp, ext, (DT *)y, sc);   \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_dual<M, DT, 1>), grid, block, 0, s, g, rp, ext, (DT *)y, sc);   \
+  } while (0)
 #define AV_DUAL_M(DT)                        \
   do {                                       \
     if (method == kTV) AV_DUAL(kTV, DT);     \
@@ -271,9 +288,17 @@ void launch_dual(int method, int ds, const Grid3 &g, const RegParams &rp, const 
 void launch_primal(int method, int ds, const Grid3 &g, const RegParams &rp, float2 *x, float2 *ext,
                    const void *y, const float2 *khz, const DevScalars *sc, hipStream_t s) {
   check_grid(g);
-  dim3 grid((unsigned)step_blocks(g)), block(RT);
-#define AV_PRIMAL(M, DT) \
-  hipLaunchKernelGGL((k_primal<M, DT>), grid, block, 0, s, g, rp, x, ext, (const DT *)y, khz, sc)
+  const bool pairs = g.W % 2 == 0;
+  dim3 grid((unsigned)(pairs ? pair_blocks(g) : step_blocks(g))), block(RT);
+#define AV_PRIMAL(M, DT)                                                                     \
+  do {                                                                                       \
+    if (pairs)                                                                               \
+      hipLaunchKernelGGL((k_primal<M, DT, 2>), grid, block, 0, s, g, rp, x, ext, (const DT *)y, \
+                         khz, sc);                                                           \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_primal<M, DT, 1>), grid, block, 0, s, g, rp, x, ext, (const DT *)y, \
+                         khz, sc);                                                           \
+  } while (0)
 #define AV_PRIMAL_M(DT)                          \
   do {                                           \
     if (method == kTV) AV_PRIMAL(kTV, DT);       \
@@ -290,6 +315,7 @@ void launch_primal(int method, int ds, const Grid3 &g, const RegParams &rp, floa
 }
 
 int step_blocks(const Grid3 &g) { return (int)(tiles_per_frame(g) * g.T); }
+int pair_blocks(const Grid3 &g) { return (int)(pair_tiles_per_frame(g) * g.T); }
 int norm_blocks(const Grid3 &g) { return (int)nblk(g.N); }
 
 void launch_step_reg(int method, const Grid3 &g, const RegParams &rp, const float2 *ext,

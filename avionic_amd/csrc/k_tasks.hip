@@ -9,11 +9,17 @@
 
 namespace avionic {
 
-enum Family { kVox = 0, kCol = 1, kRow = 2 };
+enum Family { kVox = 0, kCol = 1, kRow = 2, kDp1 = 3, kDp2 = 4 };
 
 template <int F>
 __device__ inline int first_block(const Task &t) {
-  return F == kVox ? t.b_vox : (F == kCol ? t.b_col : t.b_row);
+  switch (F) {
+    case kVox: return t.b_vox;
+    case kCol: return t.b_col;
+    case kRow: return t.b_row;
+    case kDp1: return t.b_dp1;
+    default: return t.b_dp2;
+  }
 }
 
 // Last task whose first block does not exceed b (tasks are in block order;
@@ -50,23 +56,34 @@ __device__ inline bool task_adapt(const DevScalars *sc) {
   return sc->schedule == 1 || (sc->schedule == 0 && (k < 10 || (k % 50) == 0));
 }
 
-template <int M, typename DT>
+// Dual and primal updates, one voxel (VPL = 1, over the kDp1 blocks) or one
+// pair of voxels of a row (VPL = 2, over the kDp2 blocks) per lane.
+__device__ inline bool lane_voxels(const Task &T, int blk, Vox &v) {
+  return vox_of_block<false>(T.g, blk - T.b_dp1, threadIdx.x, v);
+}
+__device__ inline bool lane_voxels(const Task &T, int blk, Vox2 &v) {
+  return vox2_of_block(T.g, blk - T.b_dp2, threadIdx.x, v);
+}
+
+template <int M, typename DT, int VPL>
 __global__ __launch_bounds__(RT) void kt_dual(const Task *__restrict__ tasks, int n) {
+  typedef typename std::conditional<VPL == 2, Vox2, Vox>::type V;
   const int blk = blockIdx.x;
-  const Task &T = tasks[find_task<kVox>(tasks, n, blk)];
+  const Task &T = tasks[find_task<VPL == 2 ? kDp2 : kDp1>(tasks, n, blk)];
   if (!task_active(T.sc)) return;
-  Vox v;
-  if (!vox_of_block<false>(T.g, blk - T.b_vox, threadIdx.x, v)) return;
+  V v;
+  if (!lane_voxels(T, blk, v)) return;
   dual_voxel<M, DT>(T.g, T.rp, T.ext, (DT *)T.y, (float)T.sc->sigma, v);
 }
 
-template <int M, typename DT>
+template <int M, typename DT, int VPL>
 __global__ __launch_bounds__(RT) void kt_primal(const Task *__restrict__ tasks, int n) {
+  typedef typename std::conditional<VPL == 2, Vox2, Vox>::type V;
   const int blk = blockIdx.x;
-  const Task &T = tasks[find_task<kVox>(tasks, n, blk)];
+  const Task &T = tasks[find_task<VPL == 2 ? kDp2 : kDp1>(tasks, n, blk)];
   if (!task_active(T.sc)) return;
-  Vox v;
-  if (!vox_of_block<false>(T.g, blk - T.b_vox, threadIdx.x, v)) return;
+  V v;
+  if (!lane_voxels(T, blk, v)) return;
   primal_voxel<M, DT>(T.g, T.rp, T.x, T.ext, (const DT *)T.y, T.khz, (float)T.sc->tau, v);
 }
 
@@ -315,35 +332,45 @@ static void launch_row(const TaskGrid &gr, const Task *d_tasks, int n, hipStream
   }
 }
 
+// Dual (primal = false) or primal update over every task, with the grid's
+// voxels per lane.
+template <int M, typename DT, int VPL>
+static void launch_dp(bool primal, int nblk, hipStream_t s, const Task *t, int n) {
+  if (primal)
+    hipLaunchKernelGGL((kt_primal<M, DT, VPL>), dim3(nblk), dim3(RT), 0, s, t, n);
+  else
+    hipLaunchKernelGGL((kt_dual<M, DT, VPL>), dim3(nblk), dim3(RT), 0, s, t, n);
+}
+template <typename DT, int VPL>
+static void launch_dp_m(int method, bool primal, int nblk, hipStream_t s, const Task *t, int n) {
+  if (method == kTV) launch_dp<kTV, DT, VPL>(primal, nblk, s, t, n);
+  else if (method == kTGV2) launch_dp<kTGV2, DT, VPL>(primal, nblk, s, t, n);
+  else if (method == kICTGV2) launch_dp<kICTGV2, DT, VPL>(primal, nblk, s, t, n);
+  else launch_dp<kICTV, DT, VPL>(primal, nblk, s, t, n);
+}
+template <int VPL>
+static void launch_dp_v(int method, int ds, bool primal, int nblk, hipStream_t s, const Task *t,
+                        int n) {
+  if (ds == kDualF32) launch_dp_m<float2, VPL>(method, primal, nblk, s, t, n);
+  else if (ds == kDualBF16) launch_dp_m<bf16x2, VPL>(method, primal, nblk, s, t, n);
+  else launch_dp_m<q16x2, VPL>(method, primal, nblk, s, t, n);
+}
+static void launch_dual_primal(bool primal, int method, int ds, const TaskGrid &gr, hipStream_t s,
+                               const Task *t, int n) {
+  if (gr.dp2 > 0) launch_dp_v<2>(method, ds, primal, gr.dp2, s, t, n);
+  if (gr.dp1 > 0) launch_dp_v<1>(method, ds, primal, gr.dp1, s, t, n);
+}
+
 void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
                    hipStream_t s) {
   if (n <= 0) return;
   const dim3 bRT(RT), bNT(NT), bNC(NTC);
-#define AV_BYMETHOD2(KER, ...)                                                       \
-  do {                                                                               \
-    if (ds == kDualF32) {                                                            \
-      if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
-      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
-      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTV, float2>), __VA_ARGS__);                           \
-    } else if (ds == kDualBF16) {                                                    \
-      if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
-      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
-      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTV, bf16x2>), __VA_ARGS__);                           \
-    } else {                                                                         \
-      if (method == kTV) hipLaunchKernelGGL((KER<kTV, q16x2>), __VA_ARGS__);          \
-      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, q16x2>), __VA_ARGS__);  \
-      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTV, q16x2>), __VA_ARGS__);                           \
-    }                                                                                \
-  } while (0)
   switch (id) {
-    case kKDual: if (gr.vox) AV_BYMETHOD2(kt_dual, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
+    case kKDual: launch_dual_primal(false, method, ds, gr, s, d_tasks, n); break;
     case kKColFwd: launch_col_fwd<false>(gr, d_tasks, n, s); break;
     case kKRowDual: launch_row<kRowDualUpdate>(gr, d_tasks, n, s); break;
     case kKColAdj: launch_col_adj(gr, d_tasks, n, s); break;
-    case kKPrimal: if (gr.vox) AV_BYMETHOD2(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n); break;
+    case kKPrimal: launch_dual_primal(true, method, ds, gr, s, d_tasks, n); break;
     case kKStepReg:
       if (gr.vox) {
         if (method == kTV) hipLaunchKernelGGL(kt_step_reg<kTV>, dim3(gr.vox), bRT, 0, s, d_tasks, n);
@@ -357,7 +384,6 @@ void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const
     case kKTail: hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n); break;
     default: break;
   }
-#undef AV_BYMETHOD2
   AV_LAUNCH_CHECK();
 }
 
@@ -378,25 +404,6 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
     AV_HIP(hipStreamWaitEvent(s, st.join, 0));
   };
   const dim3 bRT(RT), bNT(NT), bNC(NTC);
-#define AV_BYMETHOD(KER, ...)                                                        \
-  do {                                                                               \
-    if (ds == kDualF32) {                                                            \
-      if (method == kTV) hipLaunchKernelGGL((KER<kTV, float2>), __VA_ARGS__);         \
-      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, float2>), __VA_ARGS__); \
-      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, float2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTV, float2>), __VA_ARGS__);                           \
-    } else if (ds == kDualBF16) {                                                    \
-      if (method == kTV) hipLaunchKernelGGL((KER<kTV, bf16x2>), __VA_ARGS__);         \
-      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, bf16x2>), __VA_ARGS__); \
-      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, bf16x2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTV, bf16x2>), __VA_ARGS__);                           \
-    } else {                                                                         \
-      if (method == kTV) hipLaunchKernelGGL((KER<kTV, q16x2>), __VA_ARGS__);          \
-      else if (method == kTGV2) hipLaunchKernelGGL((KER<kTGV2, q16x2>), __VA_ARGS__);  \
-      else if (method == kICTGV2) hipLaunchKernelGGL((KER<kICTGV2, q16x2>), __VA_ARGS__);  \
-      else hipLaunchKernelGGL((KER<kICTV, q16x2>), __VA_ARGS__);                           \
-    }                                                                                \
-  } while (0)
   // dual update (reads ext, writes y) beside the data-term chain (reads ext,
   // writes kcol / z / zhat / khz); the primal update needs both
   if (two && gr.col < col_resident_blocks()) {
@@ -411,10 +418,10 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
     AV_HIP(hipEventRecord(st.fork, s));
     launch_col_fwd<false>(gr, d_tasks, n, s);
     AV_HIP(hipStreamWaitEvent(s2, st.fork, 0));
-    if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
+    launch_dual_primal(false, method, ds, gr, s2, d_tasks, n);
   } else {
     fork();
-    if (gr.vox > 0) AV_BYMETHOD(kt_dual, dim3(gr.vox), bRT, 0, s2, d_tasks, n);
+    launch_dual_primal(false, method, ds, gr, s2, d_tasks, n);
     launch_col_fwd<false>(gr, d_tasks, n, s);
   }
   AV_LAUNCH_CHECK();
@@ -423,7 +430,7 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   AV_LAUNCH_CHECK();
   if (hooks.fn) hooks.fn(hooks.ctx, 0, s);
   join();
-  if (gr.vox > 0) AV_BYMETHOD(kt_primal, dim3(gr.vox), bRT, 0, s, d_tasks, n);
+  launch_dual_primal(true, method, ds, gr, s, d_tasks, n);
   AV_LAUNCH_CHECK();
   if (with_step) {
     // regulariser sums (read ext, x) beside K(ext - x); the tail needs both
@@ -443,7 +450,6 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   }
   hipLaunchKernelGGL(kt_tail, dim3(n), bRT, 0, s, d_tasks, n);
   AV_LAUNCH_CHECK();
-#undef AV_BYMETHOD
 }
 
 }  // namespace avionic

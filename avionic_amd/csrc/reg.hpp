@@ -61,6 +61,8 @@ void launch_primal(int method, int ds, const Grid3 &g, const RegParams &rp, floa
 // Step-rule partial sums over the primal difference ext - x:
 // partial[2b] = weighted sum |K_reg delta|^2, partial[2b+1] = sum |delta|^2.
 int step_blocks(const Grid3 &g);
+// Blocks of the two-voxels-per-lane dual / primal updates (2 RT voxels each).
+int pair_blocks(const Grid3 &g);
 // Partial sums written by launch_reg_norm / launch_gstar_terms (one per
 // RT consecutive voxels).
 int norm_blocks(const Grid3 &g);

@@ -37,8 +37,14 @@ struct Task {
   double *part_reg;    // 2 per voxel block
   double *part_k;      // K-norm partials of the step rule (npart_k of them)
   int nvox_blocks, ncol_blocks, nrow_blocks;
+  // Dual / primal update blocks: a task runs them with one voxel per lane
+  // (ndp1_blocks = nvox_blocks, ndp2_blocks = 0) or, when its width is even,
+  // with a pair of voxels of a row per lane (ndp1_blocks = 0, ndp2_blocks =
+  // pair_blocks), the same choice at every iteration.
+  int ndp1_blocks, ndp2_blocks;
   int npart_k;                // Cartesian: nrow_blocks; other operators: their own
   int b_vox, b_col, b_row;    // first block of this task per kernel family
+  int b_dp1, b_dp2;
   int schedule;               // 0 reference schedule, 1 every iteration
 };
 
@@ -46,6 +52,8 @@ struct Task {
 // tasks: device array; the host mirror gives the grid sizes.
 struct TaskGrid {
   int vox = 0, col = 0, row = 0;
+  // Dual / primal blocks of the one-voxel-per-lane and the pair tasks.
+  int dp1 = 0, dp2 = 0;
   // Blocks [lo, hi) spanning the tasks of each column kind (col_kind) and of
   // each row kind (row_kind); the engine orders its tasks by kind, so each
   // span holds only that kind's blocks.

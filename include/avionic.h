@@ -426,6 +426,9 @@ typedef struct avionic_engine_config {
   size_t resident_bytes_cap; /* bytes of running requests; 0: no limit */
   int max_queue;            /* waiting requests beyond which submit returns
                                AVIONIC_E_CAPACITY; < 0: no limit */
+  int voxels_per_lane;      /* dual / primal updates: 1 voxel per lane, or
+                               0: a pair of voxels of a row per lane (16- and
+                               8-byte accesses) for requests of even width */
 } avionic_engine_config;
 
 /* Engine counters (avionic_engine_counters). */

@@ -110,6 +110,10 @@ PLAN_CASES = [  # method, T, H, W, C, maxIt, centered
     (1, 2, 416, 168, 3, 12, False),
     (0, 2, 96, 768, 2, 12, True),
     (2, 2, 143, 176, 2, 12, False),
+    # odd widths: these requests run the dual / primal updates with one voxel
+    # per lane beside the pair-per-lane ones
+    (1, 2, 64, 45, 2, 12, False),
+    (2, 2, 30, 27, 2, 12, True),
 ]
 
 
@@ -371,3 +375,29 @@ def test_engine_full_size_other_regularisers(gpu, method):
     err = float(np.linalg.norm(alone - ref) / np.linalg.norm(ref))
     print(f"[full size] {method} q16 duals, {its} iterations: NRMSE vs oracle {err:.2e}", flush=True)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
+def test_engine_voxel_pairs_equal_one_voxel_form(gpu, method):
+    """The dual / primal updates with a pair of voxels per lane give the same
+    bits as the one-voxel-per-lane form (voxels_per_lane=1), for every method
+    and dual storage, over step-rule and fixed-step iterations."""
+    import torch
+    import avionic_amd as A
+    T, H, W, C = 4, 24, 40, 3
+    data, b1, mask = _inputs(T, H, W, C, seed=70 + method)
+    for ds in (0, 1, 2):
+        p = _params(method, 25, ds, mask, W, H, T)
+        outs = []
+        for vpl in (1, 0):
+            e = A.Engine(use_graphs=False, max_batch=1, voxels_per_lane=vpl)
+            rid = e.submit(A.Request(method, _dev(torch, data), _dev(torch, b1),
+                                     mask=_dev(torch, mask, np.float32), params=p))
+            out = torch.empty(T, H, W, dtype=torch.complex64, device="cuda")
+            while not e.poll(rid, out):
+                e.step()
+            outs.append(out.cpu().numpy())
+        print(f"[pairs] method {method} dual storage {ds}: max |x| {np.abs(outs[0]).max():.3g}",
+              flush=True)
+        assert np.isfinite(outs[0]).all()
+        assert np.array_equal(outs[0], outs[1]), (method, ds)
