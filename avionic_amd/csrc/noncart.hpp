@@ -53,6 +53,14 @@ struct NcGeom {
   const int *cptr;         // [T][Gx*Gy+1] cell -> point list offsets
   const int2 *ce;          // per cell entry: point index m, and the bits of its
                            // weight * sqrt(density) / sqrt(HW)
+  // The same entries grouped by adjoint column tile (XBa grid columns): for
+  // frame t and tile X (nXa = ceil(Gx / XBa) tiles), the entries of cells
+  // (r, X XBa + b), r < Gy, b < XBa, in cell order k = r XBa + b, start at
+  // tce[tstart[t nXa + X]]; tcell[(t nXa + X) (XBa Gy + 1) + k] is cell k's
+  // first entry relative to that start (the last value: the tile's count).
+  const int *tstart;
+  const int *tcell;
+  const int2 *tce;
   FftPlan px, py;          // lengths Gx, Gy
 };
 

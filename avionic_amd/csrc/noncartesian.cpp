@@ -312,6 +312,38 @@ class NoncartesianOperator : public Operator {
     g_.rwx = rwx_.as<float4>();
     g_.cptr = cptr_.as<int>();
     g_.ce = ce_.as<int2>();
+    {
+      // the spreading entries regrouped by adjoint column tile (NcGeom::tce)
+      const int XB = g_.XBa, nX = (Gx + XB - 1) / XB, ncell = XB * Gy;
+      std::vector<int> tstart((size_t)T * nX), tcell((size_t)T * nX * (ncell + 1));
+      std::vector<int2> tce;
+      tce.reserve(ce.size());
+      for (int t = 0; t < T; ++t) {
+        const int *cp = &cptr[(size_t)t * (G2 + 1)];
+        for (int X = 0; X < nX; ++X) {
+          tstart[(size_t)t * nX + X] = (int)tce.size();
+          int *tc = &tcell[((size_t)t * nX + X) * (ncell + 1)];
+          int n = 0;
+          for (int r = 0; r < Gy; ++r)
+            for (int b = 0; b < XB; ++b) {
+              tc[r * XB + b] = n;
+              const int gx = X * XB + b;
+              if (gx >= Gx) continue;
+              const long cell = (long)r * Gx + gx;
+              for (int e = cp[cell]; e < cp[cell + 1]; ++e) tce.push_back(ce[e]);
+              n += cp[cell + 1] - cp[cell];
+            }
+          tc[ncell] = n;
+        }
+      }
+      up(tstart_, tstart.data(), sizeof(int) * tstart.size());
+      up(tcell_, tcell.data(), sizeof(int) * tcell.size());
+      up(tce_, tce.data(), sizeof(int2) * tce.size());
+      AV_HIP(hipStreamSynchronize(nullptr));
+      g_.tstart = tstart_.as<int>();
+      g_.tcell = tcell_.as<int>();
+      g_.tce = tce_.as<int2>();
+    }
     scratch();
     const NcTiles tiles = tune_tiles(g_, tmp_.as<float2>(), part_.as<float2>());
     g_.CG = tiles.cg;
@@ -379,7 +411,8 @@ class NoncartesianOperator : public Operator {
   int spf_, nfe_;
   std::vector<float> traj_h_, dens_h_;
   NcGeom g_{};
-  DevBuf dxi_, dyi_, sw_, rptr_, rhdr_, rwx_, cptr_, ce_, twx_, twy_, tmp_, part_;
+  DevBuf dxi_, dyi_, sw_, rptr_, rhdr_, rwx_, cptr_, ce_, tstart_, tcell_, tce_, twx_, twy_, tmp_,
+      part_;
   DevBuf q64_, w64_, sw64_, cew64_, dxi64_, dyi64_;  // FP64 tables
 
   void scratch() {
