@@ -572,13 +572,14 @@ def fft2(a, sign=-1, centered=False, unitary=False):
 class _EngineConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", _SZ), ("use_graphs", _I), ("max_batch", _I),
                 ("iterations_per_tick", _I), ("resident_bytes_cap", _SZ), ("max_queue", _I),
-                ("voxels_per_lane", _I)]
+                ("voxels_per_lane", _I), ("fixed_launch", _I)]
 
 
 class _EngineCounts(ctypes.Structure):
     _fields_ = [("held", _I), ("running", _I), ("queued", _I), ("bytes_held", _SZ),
                 ("bytes_running", _SZ), ("hbm_budget_bytes", _SZ),
-                ("graph_captures", ctypes.c_longlong), ("capture_ms", _D)]
+                ("graph_captures", ctypes.c_longlong), ("capture_ms", _D),
+                ("launch_tunings", ctypes.c_longlong)]
 
 
 class _ReqDesc(ctypes.Structure):
@@ -667,14 +668,18 @@ class Engine:
     (hipGraph-replayed when use_graphs).  max_queue: waiting requests beyond
     which submit returns None (default 0: no queue; < 0: unbounded).
     voxels_per_lane: 1 runs the dual / primal updates with one voxel per lane;
-    0 (default) with a pair of voxels per lane for requests of even width."""
+    0 (default) with a pair of voxels per lane for requests of even width.
+    fixed_launch: keep the default launch variant; by default a launch set
+    that has run 8 plain iterations unchanged times its variants (one voxel
+    or a pair per lane, column pass before or after the dual update; all give
+    the same bits) and keeps the fastest."""
 
     def __init__(self, hbm_budget_bytes=0, use_graphs=True, max_batch=0, iterations_per_tick=1,
-                 resident_bytes_cap=0, max_queue=0, voxels_per_lane=0):
+                 resident_bytes_cap=0, max_queue=0, voxels_per_lane=0, fixed_launch=False):
         L = _engine_lib()
         cfg = _EngineConfig(int(hbm_budget_bytes), int(bool(use_graphs)), int(max_batch),
                             int(iterations_per_tick), int(resident_bytes_cap), int(max_queue),
-                            int(voxels_per_lane))
+                            int(voxels_per_lane), int(bool(fixed_launch)))
         h = ctypes.c_void_p()
         _check(L.avionic_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -8,6 +8,7 @@
 // and adaptation schedule on the device, so a request's result does not
 // depend on what else is resident.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -76,6 +77,17 @@ size_t slot_bytes(int method, int ds, long N, long K, long CHW, int maxit) {
   return b;
 }
 
+// Launch variants of one iteration (TaskGrid::lanes, TaskGrid::col_first);
+// every variant gives the same bits.  v = 2 [lanes == 1] + [column pass
+// after the dual update].
+constexpr int kVariants = 4;
+inline int variant_lanes(int v) { return (v & 2) ? 1 : 2; }
+inline int variant_col_first(int v) { return (v & 1) ? 0 : 1; }
+inline int variant_of(int lanes, bool col_first) { return (lanes == 1 ? 2 : 0) + (col_first ? 0 : 1); }
+// Variant timing: a launch set is timed once it has run this many plain
+// iterations unchanged, over this many plain iterations per candidate.
+constexpr int kTuneAfter = 8, kTuneSamples = 3;
+
 struct Group {
   int method = 0;
   int ds = 0;  // dual storage
@@ -83,11 +95,26 @@ struct Group {
   bool dirty = true;
   DevBuf table;
   TaskGrid grid;
-  hipGraphExec_t g_step = nullptr, g_plain = nullptr;
-  void drop_graphs() {
-    if (g_step) (void)hipGraphExecDestroy(g_step);
-    if (g_plain) (void)hipGraphExecDestroy(g_plain);
-    g_step = g_plain = nullptr;
+  hipGraphExec_t g_step[kVariants] = {}, g_plain[kVariants] = {};
+  int variant = 0;                 // variant launched
+  bool tuning = false;             // candidates still being timed
+  int stable = 0, trial = 0;       // plain iterations since the rebuild / timed
+  std::vector<int> cands;          // candidate variants of the launch set
+  std::vector<float> samples[kVariants];
+  std::array<long, 8> sig{};       // launch-set signature (decision cache key)
+  void drop_graphs(int keep = -1) {
+    for (int v = 0; v < kVariants; ++v) {
+      if (v == keep) continue;
+      if (g_step[v]) (void)hipGraphExecDestroy(g_step[v]);
+      if (g_plain[v]) (void)hipGraphExecDestroy(g_plain[v]);
+      g_step[v] = g_plain[v] = nullptr;
+    }
+  }
+  TaskGrid grid_of(int v) const {
+    TaskGrid gr = grid;
+    gr.lanes = variant_lanes(v);
+    gr.col_first = variant_col_first(v);
+    return gr;
   }
 };
 
@@ -111,6 +138,9 @@ struct avionic_engine {
   std::deque<Slot *> queue;  // submitted, not yet admitted (FIFO)
   long long captures = 0;    // graph captures + instantiations
   double capture_ms = 0.0;   // host time spent in them
+  long long tunings = 0;     // launch sets whose variants were timed
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;  // variant timing
+  std::map<std::array<long, 8>, int> tuned;     // signature -> fastest variant
   std::map<long long, std::unique_ptr<Slot>> slots;
   std::map<int, Group> groups;  // key = method * 3 + dual storage
 
@@ -152,6 +182,8 @@ struct avionic_engine {
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_t0) (void)hipEventDestroy(ev_t0);
+    if (ev_t1) (void)hipEventDestroy(ev_t1);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (side) (void)hipStreamDestroy(side);
     if (ev_req) (void)hipEventDestroy(ev_req);
@@ -194,6 +226,32 @@ struct avionic_engine {
     g.grid.vox = bv;
     g.grid.dp1 = bd1;
     g.grid.dp2 = bd2;
+    // launch variants: pairs of voxels unless configured off or no task has
+    // an even width (then both lane variants launch the same kernels); the
+    // column pass before or after the dual update (two streams only)
+    const int lanes0 = (cfg.voxels_per_lane == 1 || bd2 == 0) ? 1 : 2;
+    const bool first0 = default_col_first(g.grid);
+    g.cands.clear();
+    for (int lanes : {2, 1}) {
+      if (lanes == 2 && lanes0 == 1) continue;
+      if (lanes == 1 && lanes0 == 2 && cfg.fixed_launch) continue;
+      for (bool first : {true, false}) {
+        if (first != first0 && (cfg.fixed_launch || !side)) continue;
+        g.cands.push_back(variant_of(lanes, first));
+      }
+    }
+    g.sig = {g.method, g.ds, (long)g.slots.size(), g.grid.vox, g.grid.col, g.grid.row, bd1, bd2};
+    g.variant = variant_of(lanes0, first0);
+    g.tuning = false;
+    g.stable = g.trial = 0;
+    for (auto &v : g.samples) v.clear();
+    if (g.cands.size() > 1) {
+      auto it = tuned.find(g.sig);
+      if (it != tuned.end())
+        g.variant = it->second;
+      else
+        g.tuning = true;
+    }
     g.grid.col = bc;
     g.grid.row = br;
     if (!tasks.empty()) {
@@ -267,24 +325,65 @@ struct avionic_engine {
         hooks.ctx = &g;
         break;
       }
+    // Variant timing: once the launch set has run kTuneAfter plain
+    // iterations unchanged, its next plain iterations cycle through the
+    // candidates, each timed with events (a host wait per timed iteration),
+    // and the one with the lowest median is kept for this signature.
+    int v = g.variant;
+    bool timed = false;
+    if (g.tuning && !with_step) {
+      if (g.stable < kTuneAfter) {
+        g.stable += 1;
+      } else {
+        v = g.cands[g.trial % g.cands.size()];
+        timed = true;
+      }
+    }
+    const TaskGrid gr = g.grid_of(v);
+    if (timed) AV_HIP(hipEventRecord(ev_t0, stream));
     if (!cfg.use_graphs) {
-      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st, hooks);
-      return;
+      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, gr, with_step, st, hooks);
+    } else {
+      hipGraphExec_t &ex = with_step ? g.g_step[v] : g.g_plain[v];
+      if (!ex) {
+        const auto t0 = std::chrono::steady_clock::now();
+        hipGraph_t graph = nullptr;
+        AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        launch_iteration(g.method, g.ds, g.table.as<Task>(), n, gr, with_step, st, hooks);
+        AV_HIP(hipStreamEndCapture(stream, &graph));
+        AV_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        captures += 1;
+        capture_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (timed) AV_HIP(hipEventRecord(ev_t0, stream));  // capture time is not launch time
+      }
+      AV_HIP(hipGraphLaunch(ex, stream));
     }
-    hipGraphExec_t &ex = with_step ? g.g_step : g.g_plain;
-    if (!ex) {
-      const auto t0 = std::chrono::steady_clock::now();
-      hipGraph_t graph = nullptr;
-      AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      launch_iteration(g.method, g.ds, g.table.as<Task>(), n, g.grid, with_step, st, hooks);
-      AV_HIP(hipStreamEndCapture(stream, &graph));
-      AV_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
-      captures += 1;
-      capture_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
-                        .count();
+    if (!timed) return;
+    AV_HIP(hipEventRecord(ev_t1, stream));
+    AV_HIP(hipEventSynchronize(ev_t1));
+    float ms = 0.f;
+    AV_HIP(hipEventElapsedTime(&ms, ev_t0, ev_t1));
+    g.samples[v].push_back(ms);
+    g.trial += 1;
+    if (g.trial < kTuneSamples * (int)g.cands.size()) return;
+    float best_ms = 0.f;
+    int best = g.cands[0];
+    for (int c : g.cands) {
+      std::vector<float> t = g.samples[c];
+      std::sort(t.begin(), t.end());
+      const float med = t[t.size() / 2];
+      if (c == g.cands[0] || med < best_ms) {
+        best_ms = med;
+        best = c;
+      }
     }
-    AV_HIP(hipGraphLaunch(ex, stream));
+    g.variant = best;
+    g.tuning = false;
+    tuned[g.sig] = best;
+    tunings += 1;
+    g.drop_graphs(best);
   }
 };
 
@@ -369,6 +468,8 @@ int avionic_engine_create(const avionic_engine_config *cfg, avionic_engine **eng
     AV_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     AV_HIP(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
     AV_HIP(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+    AV_HIP(hipEventCreate(&e->ev_t0));
+    AV_HIP(hipEventCreate(&e->ev_t1));
     AV_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     AV_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
     AV_HIP(hipEventCreateWithFlags(&e->ev_req, hipEventDisableTiming));
@@ -527,7 +628,7 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
     t.part_reg = s->preg.as<double>();
     t.part_k = s->pk.as<double>();
     t.nvox_blocks = nvb;
-    const bool pairs = e->cfg.voxels_per_lane != 1 && g.W % 2 == 0;
+    const bool pairs = g.W % 2 == 0;
     t.ndp1_blocks = pairs ? 0 : nvb;
     t.ndp2_blocks = pairs ? pair_blocks(g) : 0;
     t.ncol_blocks = s->cart ? t.cg.nxbf * t.cg.T : 0;
@@ -667,6 +768,7 @@ int avionic_engine_counters(const avionic_engine *e, avionic_engine_counts *c) {
     c->hbm_budget_bytes = e->cfg.hbm_budget_bytes;
     c->graph_captures = e->captures;
     c->capture_ms = e->capture_ms;
+    c->launch_tunings = e->tunings;
   });
 }
 

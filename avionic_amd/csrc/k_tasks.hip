@@ -56,34 +56,37 @@ __device__ inline bool task_adapt(const DevScalars *sc) {
   return sc->schedule == 1 || (sc->schedule == 0 && (k < 10 || (k % 50) == 0));
 }
 
-// Dual and primal updates, one voxel (VPL = 1, over the kDp1 blocks) or one
-// pair of voxels of a row (VPL = 2, over the kDp2 blocks) per lane.
+// Dual and primal updates over the blocks of family F: kDp2 runs a pair of
+// voxels of a row per lane (tasks of even width), kDp1 (tasks of odd width)
+// and kVox (every task) one voxel per lane.
+template <int F>
 __device__ inline bool lane_voxels(const Task &T, int blk, Vox &v) {
-  return vox_of_block<false>(T.g, blk - T.b_dp1, threadIdx.x, v);
+  return vox_of_block<false>(T.g, blk - first_block<F>(T), threadIdx.x, v);
 }
+template <int F>
 __device__ inline bool lane_voxels(const Task &T, int blk, Vox2 &v) {
   return vox2_of_block(T.g, blk - T.b_dp2, threadIdx.x, v);
 }
 
-template <int M, typename DT, int VPL>
+template <int M, typename DT, int F>
 __global__ __launch_bounds__(RT) void kt_dual(const Task *__restrict__ tasks, int n) {
-  typedef typename std::conditional<VPL == 2, Vox2, Vox>::type V;
+  typedef typename std::conditional<F == kDp2, Vox2, Vox>::type V;
   const int blk = blockIdx.x;
-  const Task &T = tasks[find_task<VPL == 2 ? kDp2 : kDp1>(tasks, n, blk)];
+  const Task &T = tasks[find_task<F>(tasks, n, blk)];
   if (!task_active(T.sc)) return;
   V v;
-  if (!lane_voxels(T, blk, v)) return;
+  if (!lane_voxels<F>(T, blk, v)) return;
   dual_voxel<M, DT>(T.g, T.rp, T.ext, (DT *)T.y, (float)T.sc->sigma, v);
 }
 
-template <int M, typename DT, int VPL>
+template <int M, typename DT, int F>
 __global__ __launch_bounds__(RT) void kt_primal(const Task *__restrict__ tasks, int n) {
-  typedef typename std::conditional<VPL == 2, Vox2, Vox>::type V;
+  typedef typename std::conditional<F == kDp2, Vox2, Vox>::type V;
   const int blk = blockIdx.x;
-  const Task &T = tasks[find_task<VPL == 2 ? kDp2 : kDp1>(tasks, n, blk)];
+  const Task &T = tasks[find_task<F>(tasks, n, blk)];
   if (!task_active(T.sc)) return;
   V v;
-  if (!lane_voxels(T, blk, v)) return;
+  if (!lane_voxels<F>(T, blk, v)) return;
   primal_voxel<M, DT>(T.g, T.rp, T.x, T.ext, (const DT *)T.y, T.khz, (float)T.sc->tau, v);
 }
 
@@ -295,6 +298,8 @@ static int col_resident_blocks() {
   return n;
 }
 
+bool default_col_first(const TaskGrid &gr) { return gr.col < col_resident_blocks(); }
+
 // Column passes: one launch per kind of column plan present among the tasks.
 template <bool DELTA, int K = 0>
 static void launch_col_fwd(const TaskGrid &gr, const Task *d_tasks, int n, hipStream_t s) {
@@ -332,33 +337,37 @@ static void launch_row(const TaskGrid &gr, const Task *d_tasks, int n, hipStream
   }
 }
 
-// Dual (primal = false) or primal update over every task, with the grid's
-// voxels per lane.
-template <int M, typename DT, int VPL>
+// Dual (primal = false) or primal update over every task, in the grid's
+// launch variant (TaskGrid::lanes).
+template <int M, typename DT, int F>
 static void launch_dp(bool primal, int nblk, hipStream_t s, const Task *t, int n) {
   if (primal)
-    hipLaunchKernelGGL((kt_primal<M, DT, VPL>), dim3(nblk), dim3(RT), 0, s, t, n);
+    hipLaunchKernelGGL((kt_primal<M, DT, F>), dim3(nblk), dim3(RT), 0, s, t, n);
   else
-    hipLaunchKernelGGL((kt_dual<M, DT, VPL>), dim3(nblk), dim3(RT), 0, s, t, n);
+    hipLaunchKernelGGL((kt_dual<M, DT, F>), dim3(nblk), dim3(RT), 0, s, t, n);
 }
-template <typename DT, int VPL>
+template <typename DT, int F>
 static void launch_dp_m(int method, bool primal, int nblk, hipStream_t s, const Task *t, int n) {
-  if (method == kTV) launch_dp<kTV, DT, VPL>(primal, nblk, s, t, n);
-  else if (method == kTGV2) launch_dp<kTGV2, DT, VPL>(primal, nblk, s, t, n);
-  else if (method == kICTGV2) launch_dp<kICTGV2, DT, VPL>(primal, nblk, s, t, n);
-  else launch_dp<kICTV, DT, VPL>(primal, nblk, s, t, n);
+  if (method == kTV) launch_dp<kTV, DT, F>(primal, nblk, s, t, n);
+  else if (method == kTGV2) launch_dp<kTGV2, DT, F>(primal, nblk, s, t, n);
+  else if (method == kICTGV2) launch_dp<kICTGV2, DT, F>(primal, nblk, s, t, n);
+  else launch_dp<kICTV, DT, F>(primal, nblk, s, t, n);
 }
-template <int VPL>
+template <int F>
 static void launch_dp_v(int method, int ds, bool primal, int nblk, hipStream_t s, const Task *t,
                         int n) {
-  if (ds == kDualF32) launch_dp_m<float2, VPL>(method, primal, nblk, s, t, n);
-  else if (ds == kDualBF16) launch_dp_m<bf16x2, VPL>(method, primal, nblk, s, t, n);
-  else launch_dp_m<q16x2, VPL>(method, primal, nblk, s, t, n);
+  if (ds == kDualF32) launch_dp_m<float2, F>(method, primal, nblk, s, t, n);
+  else if (ds == kDualBF16) launch_dp_m<bf16x2, F>(method, primal, nblk, s, t, n);
+  else launch_dp_m<q16x2, F>(method, primal, nblk, s, t, n);
 }
 static void launch_dual_primal(bool primal, int method, int ds, const TaskGrid &gr, hipStream_t s,
                                const Task *t, int n) {
-  if (gr.dp2 > 0) launch_dp_v<2>(method, ds, primal, gr.dp2, s, t, n);
-  if (gr.dp1 > 0) launch_dp_v<1>(method, ds, primal, gr.dp1, s, t, n);
+  if (gr.lanes == 1) {
+    if (gr.vox > 0) launch_dp_v<kVox>(method, ds, primal, gr.vox, s, t, n);
+    return;
+  }
+  if (gr.dp2 > 0) launch_dp_v<kDp2>(method, ds, primal, gr.dp2, s, t, n);
+  if (gr.dp1 > 0) launch_dp_v<kDp1>(method, ds, primal, gr.dp1, s, t, n);
 }
 
 void launch_kernel(int id, int method, int ds, const Task *d_tasks, int n, const TaskGrid &gr,
@@ -406,7 +415,8 @@ void launch_iteration(int method, int ds, const Task *d_tasks, int n, const Task
   const dim3 bRT(RT), bNT(NT), bNC(NTC);
   // dual update (reads ext, writes y) beside the data-term chain (reads ext,
   // writes kcol / z / zhat / khz); the primal update needs both
-  if (two && gr.col < col_resident_blocks()) {
+  const bool col_first = gr.col_first >= 0 ? gr.col_first == 1 : default_col_first(gr);
+  if (two && col_first) {
     // The column pass of up to a few waves of workgroups (one reconstruction
     // of 256x256x30x16: 480 workgroups; eight: 3840) is submitted first, so
     // that its workgroups start and the dual fills the rest of the CUs; the

@@ -37,10 +37,11 @@ struct Task {
   double *part_reg;    // 2 per voxel block
   double *part_k;      // K-norm partials of the step rule (npart_k of them)
   int nvox_blocks, ncol_blocks, nrow_blocks;
-  // Dual / primal update blocks: a task runs them with one voxel per lane
-  // (ndp1_blocks = nvox_blocks, ndp2_blocks = 0) or, when its width is even,
-  // with a pair of voxels of a row per lane (ndp1_blocks = 0, ndp2_blocks =
-  // pair_blocks), the same choice at every iteration.
+  // Dual / primal update blocks when a launch runs pairs of voxels
+  // (TaskGrid::lanes 2): a task of even width runs ndp2_blocks = pair_blocks
+  // blocks of a pair of voxels of a row per lane (ndp1_blocks = 0), a task of
+  // odd width its ndp1_blocks = nvox_blocks one-voxel blocks (ndp2_blocks =
+  // 0).  With lanes 1 every task runs its nvox_blocks one-voxel blocks.
   int ndp1_blocks, ndp2_blocks;
   int npart_k;                // Cartesian: nrow_blocks; other operators: their own
   int b_vox, b_col, b_row;    // first block of this task per kernel family
@@ -54,6 +55,11 @@ struct TaskGrid {
   int vox = 0, col = 0, row = 0;
   // Dual / primal blocks of the one-voxel-per-lane and the pair tasks.
   int dp1 = 0, dp2 = 0;
+  // Launch variant (both give the same bits): lanes 2 runs pairs of voxels
+  // for the tasks of even width, lanes 1 one voxel per lane for every task;
+  // col_first 1 submits the column pass before the dual update, 0 after it,
+  // -1 decides by the column pass's grid size.
+  int lanes = 2, col_first = -1;
   // Blocks [lo, hi) spanning the tasks of each column kind (col_kind) and of
   // each row kind (row_kind); the engine orders its tasks by kind, so each
   // span holds only that kind's blocks.
@@ -80,6 +86,9 @@ struct IterHooks {
 };
 void launch_iteration(int method, int dual_storage, const Task *d_tasks, int n, const TaskGrid &grid,
                       bool with_step, const IterStreams &st, const IterHooks &hooks = IterHooks{});
+// The column-pass order col_first = -1 resolves to: the column pass first
+// when its grid is at most a few waves of workgroups on this device.
+bool default_col_first(const TaskGrid &grid);
 
 // Kernel families of one iteration, in launch order.
 enum KernelId {

@@ -429,6 +429,13 @@ typedef struct avionic_engine_config {
   int voxels_per_lane;      /* dual / primal updates: 1 voxel per lane, or
                                0: a pair of voxels of a row per lane (16- and
                                8-byte accesses) for requests of even width */
+  int fixed_launch;         /* 1: each launch set runs the default launch
+                               variant; 0: a launch set that has run 8 plain
+                               iterations unchanged times its variants (pairs
+                               or one voxel per lane; the column pass before
+                               or after the dual update; all give the same
+                               bits) over its next plain iterations and keeps
+                               the fastest for launch sets of that signature */
 } avionic_engine_config;
 
 /* Engine counters (avionic_engine_counters). */
@@ -439,6 +446,7 @@ typedef struct avionic_engine_counts {
   size_t bytes_held, bytes_running, hbm_budget_bytes;
   long long graph_captures; /* iteration graphs captured and instantiated */
   double capture_ms;        /* host time spent capturing them */
+  long long launch_tunings; /* launch sets whose variants were timed */
 } avionic_engine_counts;
 
 typedef struct avionic_request_desc {

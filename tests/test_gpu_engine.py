@@ -401,3 +401,41 @@ def test_engine_voxel_pairs_equal_one_voxel_form(gpu, method):
               flush=True)
         assert np.isfinite(outs[0]).all()
         assert np.array_equal(outs[0], outs[1]), (method, ds)
+
+
+def test_engine_launch_variant_tuning_keeps_results(gpu):
+    """A launch set that runs unchanged times its launch variants (a pair or
+    one voxel per lane, column pass before or after the dual update) and
+    keeps the fastest; every variant gives the same bits, so the results
+    equal those of an engine that keeps the default variant.  A later launch
+    set of the same signature reuses the decision."""
+    import torch
+    import avionic_amd as A
+    T, H, W, C = 4, 32, 48, 3
+    data, b1, mask = _inputs(T, H, W, C, seed=91)
+    p = _params(2, 40, 2, mask, W, H, T)
+
+    def run(engine, nreq):
+        ids = [engine.submit(A.Request(2, _dev(torch, data), _dev(torch, b1),
+                                       mask=_dev(torch, mask, np.float32), params=p))
+               for _ in range(nreq)]
+        outs = [torch.empty(T, H, W, dtype=torch.complex64, device="cuda") for _ in ids]
+        done = set()
+        while len(done) < len(ids):
+            engine.step()
+            for i, rid in enumerate(ids):
+                if i not in done and engine.poll(rid, outs[i]):
+                    done.add(i)
+        return [o.cpu().numpy() for o in outs]
+
+    tuned = A.Engine(use_graphs=True)
+    fixed = A.Engine(use_graphs=True, fixed_launch=True)
+    a = run(tuned, 2)
+    b = run(fixed, 2)
+    assert tuned.counters()["launch_tunings"] == 1
+    assert fixed.counters()["launch_tunings"] == 0
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    c = run(tuned, 2)
+    assert tuned.counters()["launch_tunings"] == 1  # same signature: decision reused
+    assert np.array_equal(c[0], a[0])
