@@ -240,7 +240,12 @@ struct avionic_engine {
         g.cands.push_back(variant_of(lanes, first));
       }
     }
-    g.sig = {g.method, g.ds, (long)g.slots.size(), g.grid.vox, g.grid.col, g.grid.row, bd1, bd2};
+    // signature: the launch set's method, dual storage and the binary orders
+    // of magnitude of its grids (launch sets of similar sizes share a
+    // decision; a serving mix re-times only when its sizes move a lot)
+    auto lg = [](long v) { long k = 0; while (v > 1) { v >>= 1; ++k; } return k; };
+    g.sig = {g.method, g.ds, lanes0, lg(g.grid.vox), lg(g.grid.col), lg(g.grid.row), lg(bd1),
+             lg(bd2)};
     g.variant = variant_of(lanes0, first0);
     g.tuning = false;
     g.stable = g.trial = 0;

@@ -435,7 +435,8 @@ typedef struct avionic_engine_config {
                                or one voxel per lane; the column pass before
                                or after the dual update; all give the same
                                bits) over its next plain iterations and keeps
-                               the fastest for launch sets of that signature */
+                               the fastest for launch sets of its method, dual
+                               storage and grid sizes (to a factor of 2) */
 } avionic_engine_config;
 
 /* Engine counters (avionic_engine_counters). */
