@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
 # No packed-FP32 instructions in device code (the host pass warns that it does
 # not know this feature; harmless).
 HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-            "-packed-fp32-ops", "-munsafe-fp-atomics"]
+            "-packed-fp32-ops"]
 
 
 def hipcc():

@@ -804,16 +804,17 @@ int avionic_engine_time_kernels(avionic_engine *e, int reps, double *ms, const c
       }
     AV_CHECK(g != nullptr, "no resident requests");
     if (g->dirty) e->rebuild(*g);
+    const TaskGrid gr = g->grid_of(g->variant);  // the variant the group runs
     const size_t m = std::min(*n, (size_t)kNumKernels);
     hipEvent_t a, b;
     AV_HIP(hipEventCreate(&a));
     AV_HIP(hipEventCreate(&b));
     for (size_t k = 0; k < m; ++k) {
-      launch_kernel((int)k, g->method, g->ds, g->table.as<Task>(), (int)g->slots.size(), g->grid,
+      launch_kernel((int)k, g->method, g->ds, g->table.as<Task>(), (int)g->slots.size(), gr,
                     e->stream);  // warm
       AV_HIP(hipEventRecord(a, e->stream));
       for (int r = 0; r < reps; ++r)
-        launch_kernel((int)k, g->method, g->ds, g->table.as<Task>(), (int)g->slots.size(), g->grid,
+        launch_kernel((int)k, g->method, g->ds, g->table.as<Task>(), (int)g->slots.size(), gr,
                       e->stream);
       AV_HIP(hipEventRecord(b, e->stream));
       AV_HIP(hipEventSynchronize(b));

@@ -18,9 +18,10 @@
 //   3. sample pass: the four partials of a point, scaled, feed K u (or the z
 //      update, or a norm reduction).
 // K^H as two passes:
-//   1. column pass (grid (ceil(Gx/XB), T*C)): each grid cell gathers its
-//      trajectory points from a cell -> point table (the spreading, without
-//      atomics), length-Gy inverse transform, keeps the H image rows ->
+//   1. column pass (grid (ceil(Gx/XB), T*C)): the spreading, by gathers
+//      without atomics: the lanes form the products of the column tile's
+//      cell -> point entries (tce) into LDS, each grid cell sums its own in
+//      entry order; length-Gy inverse transform, keeps the H image rows ->
 //      tmp [t][c][H][Gx];
 //   2. row pass (grid (ceil(H/RB), T), coil loop inside): length-Gx inverse
 //      transforms, crop to W, conj(b1_c) coil sum in registers, deapodise.
