@@ -24,6 +24,7 @@
 #include <fstream>
 #include <functional>
 #include <map>
+#include <set>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -88,6 +89,40 @@ Dims parse_dims(const std::string &s) {
   return d;
 }
 
+// Keys a parameter file may set: the reference's configuration options and
+// its command-line options by long name (options_parser.cpp:79-123,
+// 173-321).  The reference reads the file with boost's parse_config_file,
+// which throws on any other key (options_parser.cpp:393); so does this reader.
+bool known_cfg_key(const std::string &key) {
+  static const std::set<std::string> top = {
+      "help", "debugstep", "verbose", "dims", "nonuniform", "normalize", "extradata",
+      "adaptlambda", "dens", "params", "sens", "uzero", "labeldata", "rawdata", "forceOSRemoval",
+      "tpat", "slice", "gpudevice", "dataBS", "maskBS", "finalOutBS", "method", "maxIt",
+      "stopPDGap", "kdata", "mask", "output"};
+  static const std::map<std::string, std::set<std::string>> sections = {
+      {"coil", {"uH1mu", "uReg", "uNrIt", "uTau", "uSigma", "uSigmaTauRatio", "uAlpha0", "uAlpha1",
+                "b1Reg", "b1NrIt", "b1Tau", "b1Sigma", "b1SigmaTauRatio", "b1FinalReg",
+                "b1FinalNrIt"}},
+      {"tv", {"dx", "dy", "dt", "sigma", "tau", "sigmaTauRatio", "timeSpaceWeight", "lambda"}},
+      {"tvtemp", {"dt", "sigma", "tau", "sigmaTauRatio", "lambda"}},
+      {"tgv2", {"dx", "dy", "dt", "sigma", "tau", "sigmaTauRatio", "timeSpaceWeight", "lambda",
+                "alpha0", "alpha1"}},
+      {"tgv2_3D", {"dx", "dy", "dz", "sigma", "tau", "sigmaTauRatio", "lambda", "alpha0", "alpha1"}},
+      {"ictv", {"dx", "dy", "dt", "sigma", "tau", "sigmaTauRatio", "timeSpaceWeight", "lambda",
+                "alpha1", "alpha", "timeSpaceWeight2", "dx2", "dy2", "dt2"}},
+      {"ictgv2", {"dx", "dy", "dt", "sigma", "tau", "sigmaTauRatio", "timeSpaceWeight", "lambda",
+                  "alpha0", "alpha1", "alpha", "timeSpaceWeight2", "dx2", "dy2", "dt2"}},
+      {"asltgv2recon4D", {"dx", "dy", "dz", "dt", "sigma", "tau", "sigmaTauRatio",
+                          "timeSpaceWeight", "lambda_l", "lambda_c", "alpha0", "alpha1", "alpha"}},
+      {"gpunufft", {"kernelWidth", "sectorWidth", "osf"}},
+      {"h1", {"mu", "relTol", "absTol", "maxIt"}},
+      {"adaptlambda", {"k", "d"}}};
+  const size_t dot = key.find('.');
+  if (dot == std::string::npos) return top.count(key) != 0;
+  auto it = sections.find(key.substr(0, dot));
+  return it != sections.end() && it->second.count(key.substr(dot + 1)) != 0;
+}
+
 // Configuration file (boost program_options syntax, config/default.cfg):
 // "key = value" lines, "[section]" headers, '#' comments.
 void read_cfg(const std::string &path, std::map<std::string, std::string> &cfg) {
@@ -107,7 +142,9 @@ void read_cfg(const std::string &path, std::map<std::string, std::string> &cfg) 
     const size_t eq = line.find('=');
     if (eq == std::string::npos) die("malformed line in " + path + ": " + line);
     const std::string key = trim(line.substr(0, eq)), val = trim(line.substr(eq + 1));
-    cfg[section.empty() ? key : section + "." + key] = val;
+    const std::string full = section.empty() ? key : section + "." + key;
+    if (!known_cfg_key(full)) die("unrecognised option '" + full + "' in " + path);
+    cfg[full] = val;
   }
 }
 

@@ -114,3 +114,10 @@ def test_cli_rejects_what_it_cannot_do(tmp_path):
     # no lambda and no adaptation
     r = _run(["-d", dims, "-s", str(tmp_path / "b1.bin")] + files + ["--dry-run"], ok=False)
     assert r.returncode == 2 and "lambda" in r.stderr
+    # a key the reference's parameter file does not define (boost's
+    # parse_config_file throws on it, options_parser.cpp:393)
+    for bad in ("[tgv2]\nlamda = 2.5\n", "[tgv3]\nlambda = 2.5\n", "maxIter = 5\n"):
+        (tmp_path / "bad.cfg").write_text(bad)
+        r = _run(["-d", dims, "-s", str(tmp_path / "b1.bin"), "-p", str(tmp_path / "bad.cfg")] +
+                 files + ["--dry-run"], ok=False)
+        assert r.returncode != 0 and "unrecognised option" in r.stderr, bad
