@@ -52,6 +52,7 @@ extern "C" int avionic_bin_read(const char *path, double *re, double *im, size_t
   if (!re) return AVIONIC_OK;
   std::fseek(fh.f, start, SEEK_SET);
   const size_t m = std::min<size_t>(capacity, len);
+  if (m == 0) return AVIONIC_OK;
   if (std::fread(re, 8, m, fh.f) != m) return fail(std::string("read error in ") + path);
   if (im) {
     if (cplx & 1) {
@@ -72,8 +73,10 @@ extern "C" int avionic_bin_write(const char *path, const double *re, const doubl
   if (!fh.f) return fail(std::string("cannot create ") + path);
   const uint8_t cplx = im ? 1 : 0;
   const uint32_t len = (uint32_t)n;
+  // (no payload call for n = 0: re / im may then be null, which fwrite's
+  // pointer argument must never be)
   bool ok = std::fwrite(&cplx, 1, 1, fh.f) == 1 && std::fwrite(&len, 4, 1, fh.f) == 1 &&
-            std::fwrite(re, 8, n, fh.f) == n && (!im || std::fwrite(im, 8, n, fh.f) == n);
+            (n == 0 || (std::fwrite(re, 8, n, fh.f) == n && (!im || std::fwrite(im, 8, n, fh.f) == n)));
   if (!ok) return fail(std::string("write error in ") + path);
   return AVIONIC_OK;
 }
