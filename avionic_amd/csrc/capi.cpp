@@ -6,11 +6,16 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/avionic.h"
 #include "coil.hpp"
 #include "solver.hpp"
 
 namespace avionic {
+TraceRange::TraceRange(const char *name) { roctxRangePushA(name); }
+TraceRange::~TraceRange() { roctxRangePop(); }
+
 double ellipke_host(double m);
 bool time_space_weights(double tsw, double &ds, double &dt);
 class NoncartesianOperator;

@@ -32,6 +32,15 @@ struct Error : std::runtime_error {
 
 #define AV_LAUNCH_CHECK() AV_HIP(hipGetLastError())
 
+// A named range on the host timeline for profilers (rocprofv3
+// --marker-trace shows it beside the kernels); ends with its scope.
+struct TraceRange {
+  explicit TraceRange(const char *name);
+  ~TraceRange();
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
+
 // ---------------------------------------------------------------- complex
 __host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

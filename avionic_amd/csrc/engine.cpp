@@ -192,6 +192,7 @@ struct avionic_engine {
   }
 
   void rebuild(Group &g) {
+    TraceRange tr("avionic_engine_rebuild");
     g.drop_graphs();
     // tasks ordered by column kind, then row kind: each kind's kernel is
     // launched over its own span of blocks only
@@ -351,6 +352,7 @@ struct avionic_engine {
     } else {
       hipGraphExec_t &ex = with_step ? g.g_step[v] : g.g_plain[v];
       if (!ex) {
+        TraceRange tr("avionic_engine_capture");
         const auto t0 = std::chrono::steady_clock::now();
         hipGraph_t graph = nullptr;
         AV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -658,6 +660,7 @@ int avionic_engine_submit(avionic_engine *e, const avionic_request_desc *d, long
 int avionic_engine_step(avionic_engine *e, void *stream) {
   int resident = 0;
   const int rc = eguard([&] {
+    TraceRange tr("avionic_engine_step");
     AV_CHECK(e, "engine is null");
     std::lock_guard<std::mutex> lock(e->mu);
     hipStream_t us = (hipStream_t)stream;

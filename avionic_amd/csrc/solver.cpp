@@ -364,6 +364,7 @@ double Recon::operator_norm(const float2 *b1, int iterations, hipStream_t s) {
 
 // IterativeReconstruction: tv.cpp:99-200, tgv2.cpp:180-338, ictgv2.cpp:294-492.
 void Recon::run(const float2 *data, float2 *xio, const float2 *b1, hipStream_t s) {
+  TraceRange tr("avionic_recon_run");
   prepare_weights();
   AV_CHECK(params.adaptSchedule >= 0 && params.adaptSchedule <= 2, "adaptSchedule must be 0, 1 or 2");
   if (params.adaptSchedule == 2) {
