@@ -374,6 +374,37 @@ __device__ inline void fft_io(float2 *buf, const FftPlan &p, int B, int bstride,
                                                      p.ns[n - 1], sign, tw, tid, nl, st);
 }
 
+// A plan known at compile time: length L = R0 R1 R2, stages in that order
+// (R2 = 1: two stages).  fft_io_ct runs the stages fft_io runs on
+// make_fft_plan's plan of the same radices (same butterflies, twiddle indices
+// and Stockham positions), with the length, the stage strides and, for
+// BFAST, the transform count folded into constants: the per-element index
+// divisions and remainders of the runtime plan become multiplies and shifts.
+// `tw` holds the length-L twiddle table (stage_twiddles of that plan).
+template <int L_, int R0_, int R1_, int R2_ = 1>
+struct FftCt {
+  static constexpr int L = L_, R0 = R0_, R1 = R1_, R2 = R2_;
+  static_assert(R0_ * R1_ * R2_ == L_, "radices must multiply to the length");
+};
+// BMAX: most transforms per call (sizes the per-lane register arrays).  For
+// BFAST the transform count is the constant BMAX and `B` is ignored.
+template <int NT, class P, int BMAX, bool BFAST, bool FROMG, bool TOG, class LD, class ST>
+__device__ __forceinline__ void fft_io_ct(float2 *buf, int B, int bstride, int istride, float sign,
+                                          int tid, const float2 *tw, LD &ld, ST &st) {
+  constexpr int L = P::L, E = L * BMAX;
+  const int nb = BFAST ? BMAX : B;
+  NoStore ns;
+  NoLoad nl;
+  if constexpr (P::R2 == 1) {
+    fft_stage_io<P::R0, NT, E, BFAST, FROMG, false>(buf, L, nb, bstride, istride, 1, sign, tw, tid, ld, ns);
+    fft_stage_io<P::R1, NT, E, BFAST, false, TOG>(buf, L, nb, bstride, istride, P::R0, sign, tw, tid, nl, st);
+  } else {
+    fft_stage_io<P::R0, NT, E, BFAST, FROMG, false>(buf, L, nb, bstride, istride, 1, sign, tw, tid, ld, ns);
+    fft_stage_io<P::R1, NT, E, BFAST, false, false>(buf, L, nb, bstride, istride, P::R0, sign, tw, tid, nl, ns);
+    fft_stage_io<P::R2, NT, E, BFAST, false, TOG>(buf, L, nb, bstride, istride, P::R0 * P::R1, sign, tw, tid, nl, st);
+  }
+}
+
 // Copy a plan's twiddles to LDS (L <= kMaxFftLen) and return the LDS view.
 __device__ inline const float2 *stage_twiddles(const FftPlan &p, float2 *lds, int tid, int nt) {
   for (int i = tid; i < p.L; i += nt) lds[i] = p.tw[i];

@@ -25,6 +25,18 @@ constexpr int row_slots_nc(bool wide) { return max_slots(NCE, NCR, wide); }
 
 __device__ inline int wrap_up(int v, int G) { return v >= G ? v - G : v; }
 
+// Grid transforms: on the geometry's runtime plans (NcPlanRt), or compiled
+// for grid length 384 in both directions (a 256 x 256 image at oversampling
+// 1.5), radices 16, 8, 3 as make_fft_plan orders them, with kNcCtCols grid
+// columns per column-pass block (pick_cols at that length).  nc_ct384 picks
+// the compiled kernels; they run the runtime plan's stages with the index
+// arithmetic folded into constants.
+struct NcPlanRt {
+  static constexpr int L = 0;
+};
+using NcPlan384 = FftCt<384, 16, 8, 3>;
+constexpr int kNcCtCols = 8;
+
 // ------------------------------------------------------------ K, column pass
 struct NcColFwdLoad {
   const float2 *ut, *us, *bc;
@@ -50,7 +62,7 @@ struct NcColFwdStore {
   }
 };
 
-template <bool WIDE>
+template <bool WIDE, class PL>
 __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__restrict__ u,
                                                     const float2 *__restrict__ usub,
                                                     const float2 *__restrict__ b1,
@@ -60,6 +72,8 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
   const int tid = threadIdx.x;
   const int t = blockIdx.y;
   const long HW = (long)g.H * g.W;
+  constexpr bool CT = PL::L != 0;
+  const int XB = CT ? kNcCtCols : g.XBf;
   const float2 *tw = stage_twiddles(g.py, twl, tid, NCF);
   __syncthreads();
   NcColFwdLoad ld;
@@ -70,16 +84,20 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
   ld.dyi = g.dyi;
   ld.W = g.W;
   ld.H = g.H;
-  ld.Gy = g.Gy;
-  ld.xb0 = blockIdx.x * g.XBf;
-  NcColFwdStore st{nullptr, g.W, (int)blockIdx.x * g.XBf};
+  ld.Gy = CT ? PL::L : g.Gy;
+  ld.xb0 = blockIdx.x * XB;
+  NcColFwdStore st{nullptr, g.W, (int)blockIdx.x * XB};
   // coils [c0, c1) of this block (blockIdx.z picks the coil group)
   const int cpg = (g.C + gridDim.z - 1) / gridDim.z;
   const int c0 = blockIdx.z * cpg, c1 = min(g.C, c0 + cpg);
   for (int c = c0; c < c1; ++c) {
     ld.bc = b1 + c * HW;
     st.tc = tmp + ((long)t * g.C + c) * g.Gy * g.W;
-    fft_io<NCF, NCE, true, true, true, WIDE>(tile, g.py, g.XBf, 1, g.XBf, -1.f, tid, tw, ld, st);
+    if constexpr (CT)
+      fft_io_ct<NCF, PL, kNcCtCols, true, true, true>(tile, kNcCtCols, 1, kNcCtCols, -1.f, tid, tw,
+                                                      ld, st);
+    else
+      fft_io<NCF, NCE, true, true, true, WIDE>(tile, g.py, XB, 1, XB, -1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
@@ -94,7 +112,7 @@ struct NcRowFwdLoad {
   }
 };
 
-template <bool WIDE>
+template <bool WIDE, class PL>
 __global__ __launch_bounds__(NCR) void k_nc_row_fwd(NcGeom g, const float2 *__restrict__ tmp,
                                                     float2 *__restrict__ part) {
   __shared__ float2 tile[NCE];
@@ -104,12 +122,16 @@ __global__ __launch_bounds__(NCR) void k_nc_row_fwd(NcGeom g, const float2 *__re
   const int t = plane / g.C;
   const int gy0 = blockIdx.x * g.RBf;
   const int nr = min(g.RBf, g.Gy - gy0);
-  const int Gx = g.Gx;
+  constexpr bool CT = PL::L != 0;
+  const int Gx = CT ? PL::L : g.Gx;
   const float2 *tw = stage_twiddles(g.px, twl, tid, NCR);
   __syncthreads();
   NcRowFwdLoad ld{tmp + ((long)plane * g.Gy + gy0) * g.W, g.W, Gx};
   NoStore ns;
-  fft_io<NCR, NCE, false, true, false, WIDE>(tile, g.px, nr, Gx, 1, -1.f, tid, tw, ld, ns);
+  if constexpr (CT)
+    fft_io_ct<NCR, PL, NCE / PL::L, false, true, false>(tile, nr, PL::L, 1, -1.f, tid, tw, ld, ns);
+  else
+    fft_io<NCR, NCE, false, true, false, WIDE>(tile, g.px, nr, Gx, 1, -1.f, tid, tw, ld, ns);
   // tile[b * Gx + i] holds grid row gy0 + b in natural order.
   const int *rp = g.rptr + (long)t * (g.Gy + 1);
   const int e0 = rp[gy0], e1 = rp[gy0 + nr];
@@ -203,7 +225,7 @@ struct NcColAdjStore {
 // so every application gives the same bits.  (Running a block over several
 // coils of a frame, to read the tile's tables once, measured slower: 0.43
 // against 0.37 ms at the 256x256x30x12 radial geometry.)
-template <bool WIDE>
+template <bool WIDE, class PL>
 __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__restrict__ z,
                                                     float2 *__restrict__ tmp) {
   constexpr int CPL = NCE / NCC;  // cells per lane (XBa Gy <= NCE)
@@ -216,7 +238,8 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
   const int t = plane / g.C;
   const int X = blockIdx.x, nX = gridDim.x;
   const float2 *tw = stage_twiddles(g.py, twl, tid, NCC);
-  const int XB = g.XBa, ncell = XB * g.Gy;
+  constexpr bool CT = PL::L != 0;
+  const int XB = CT ? kNcCtCols : g.XBa, ncell = XB * (CT ? PL::L : g.Gy);
   const int *tc = g.tcell + ((long)t * nX + X) * (ncell + 1);
   for (int k = tid; k <= ncell; k += NCC) cps[k] = tc[k];
   const int2 *ent = g.tce + g.tstart[t * nX + X];
@@ -258,9 +281,14 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
     if (k < ncell) tile[k] = acc[j];
   }
   __syncthreads();
-  NcColAdjStore st{tmp + (long)plane * g.H * g.Gx, g.H, g.Gx, g.Gy, X * XB};
+  NcColAdjStore st{tmp + (long)plane * g.H * g.Gx, g.H, CT ? PL::L : g.Gx, CT ? PL::L : g.Gy,
+                   X * XB};
   NoLoad nl;
-  fft_io<NCC, NCE, true, false, true, WIDE>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
+  if constexpr (CT)
+    fft_io_ct<NCC, PL, kNcCtCols, true, false, true>(tile, kNcCtCols, 1, kNcCtCols, +1.f, tid, tw,
+                                                     nl, st);
+  else
+    fft_io<NCC, NCE, true, false, true, WIDE>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
 }
 
 // ------------------------------------------------------------ K^H, row pass
@@ -290,7 +318,7 @@ struct NcRowAdjStore {
   }
 };
 
-template <bool WIDE>
+template <bool WIDE, class PL>
 __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__restrict__ tmp,
                                                     const float2 *__restrict__ b1,
                                                     float2 *__restrict__ img) {
@@ -301,9 +329,11 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
   const int y0 = blockIdx.x * g.RBa;
   const int nr = min(g.RBa, g.H - y0);
   const long HW = (long)g.H * g.W;
+  constexpr bool CT = PL::L != 0;
+  const int Gx = CT ? PL::L : g.Gx;
   const float2 *tw = stage_twiddles(g.px, twl, tid, NCR);
   __syncthreads();
-  NcRowAdjLoad ld{nullptr, g.Gx};
+  NcRowAdjLoad ld{nullptr, Gx};
   NcRowAdjStore<row_slots_nc(WIDE)> st;
 #pragma unroll
   for (int k = 0; k < row_slots_nc(WIDE); ++k) st.acc[k] = czero();
@@ -311,13 +341,19 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
   st.dxi = g.dxi;
   st.dyi = g.dyi;
   st.W = g.W;
-  st.Gx = g.Gx;
+  st.Gx = Gx;
   st.y0 = y0;
   for (int c = 0; c < g.C; ++c) {
-    ld.tr = tmp + (((long)t * g.C + c) * g.H + y0) * g.Gx;
+    ld.tr = tmp + (((long)t * g.C + c) * g.H + y0) * Gx;
     st.bc = b1 + c * HW;
     st.last = (c == g.C - 1);
-    fft_io<NCR, NCE, false, true, true, WIDE>(tile, g.px, nr, g.Gx, 1, +1.f, tid, tw, ld, st);
+    if constexpr (CT) {
+      static_assert(((NCE / PL::L) * PL::L / PL::R2 + NCR - 1) / NCR * PL::R2 <= row_slots_nc(WIDE),
+                    "coil-sum slots of the last stage");
+      fft_io_ct<NCR, PL, NCE / PL::L, false, true, true>(tile, nr, PL::L, 1, +1.f, tid, tw, ld, st);
+    }
+    else
+      fft_io<NCR, NCE, false, true, true, WIDE>(tile, g.px, nr, Gx, 1, +1.f, tid, tw, ld, st);
     __syncthreads();
   }
 }
@@ -327,18 +363,34 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
 // ---------------------------------------------------------------------------
 // host side
 
+namespace {
+// Both grid plans are make_fft_plan's length-384 plan (16, 8, 3) and the
+// column tiles have the compiled width: the compile-time kernels apply.
+bool nc_ct384(const NcGeom &g) {
+  auto is384 = [](const FftPlan &p) {
+    return p.L == 384 && p.nst == 3 && p.radix[0] == 16 && p.radix[1] == 8 && p.radix[2] == 3;
+  };
+  return !g.wide && is384(g.px) && is384(g.py) && g.XBf == kNcCtCols && g.XBa == kNcCtCols &&
+         g.RBf <= NCE / 384 && g.RBa <= NCE / 384;
+}
+}  // namespace
+
 void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub,
                             const float2 *b1, float2 *tmp, float2 *part, hipStream_t s) {
   const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG), gr((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C);
-  if (g.wide)
-    hipLaunchKernelGGL(k_nc_col_fwd<true>, gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
+  if (nc_ct384(g))
+    hipLaunchKernelGGL((k_nc_col_fwd<false, NcPlan384>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
+  else if (g.wide)
+    hipLaunchKernelGGL((k_nc_col_fwd<true, NcPlanRt>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
   else
-    hipLaunchKernelGGL(k_nc_col_fwd<false>, gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
+    hipLaunchKernelGGL((k_nc_col_fwd<false, NcPlanRt>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
   AV_LAUNCH_CHECK();
-  if (g.wide)
-    hipLaunchKernelGGL(k_nc_row_fwd<true>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+  if (nc_ct384(g))
+    hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlan384>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+  else if (g.wide)
+    hipLaunchKernelGGL((k_nc_row_fwd<true, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
   else
-    hipLaunchKernelGGL(k_nc_row_fwd<false>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+    hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
   AV_LAUNCH_CHECK();
 }
 
@@ -365,15 +417,19 @@ void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float
                        float2 *img, hipStream_t s) {
   const size_t cps_bytes = sizeof(int) * ((size_t)g.XBa * g.Gy + 1);
   const dim3 gc((g.Gx + g.XBa - 1) / g.XBa, g.T * g.C), gr((g.H + g.RBa - 1) / g.RBa, g.T);
-  if (g.wide)
-    hipLaunchKernelGGL(k_nc_col_adj<true>, gc, dim3(NCC), cps_bytes, s, g, z, tmp);
+  if (nc_ct384(g))
+    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlan384>), gc, dim3(NCC), cps_bytes, s, g, z, tmp);
+  else if (g.wide)
+    hipLaunchKernelGGL((k_nc_col_adj<true, NcPlanRt>), gc, dim3(NCC), cps_bytes, s, g, z, tmp);
   else
-    hipLaunchKernelGGL(k_nc_col_adj<false>, gc, dim3(NCC), cps_bytes, s, g, z, tmp);
+    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlanRt>), gc, dim3(NCC), cps_bytes, s, g, z, tmp);
   AV_LAUNCH_CHECK();
-  if (g.wide)
-    hipLaunchKernelGGL(k_nc_row_adj<true>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
+  if (nc_ct384(g))
+    hipLaunchKernelGGL((k_nc_row_adj<false, NcPlan384>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
+  else if (g.wide)
+    hipLaunchKernelGGL((k_nc_row_adj<true, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
   else
-    hipLaunchKernelGGL(k_nc_row_adj<false>, gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
+    hipLaunchKernelGGL((k_nc_row_adj<false, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
   AV_LAUNCH_CHECK();
 }
 

@@ -82,11 +82,14 @@ def test_adjointness_and_determinism(gpu):
     assert bool((op.ForwardOperation(z, b1) == khz).all())
 
 
-def test_launch_shapes_do_not_change_results(gpu, monkeypatch):
+# 64 x 64 (grid 96) runs the runtime-plan kernels; 256 x 256 at osf 1.5 (grid
+# 384) the kernels compiled for that grid length.
+@pytest.mark.parametrize("W,H,C,T,nfe,spf", [(64, 64, 6, 2, 128, 9), (256, 256, 5, 1, 512, 5)])
+def test_launch_shapes_do_not_change_results(gpu, monkeypatch, W, H, C, T, nfe, spf):
     """The coil-group count of the forward column pass and the rows per
     adjoint row block are tuned per geometry; every shape gives bit-identical
     K and K^H."""
-    W, H, C, T, nfe, spf = 64, 64, 6, 2, 128, 9
+    print(f"[gpu noncart shapes] W={W} H={H} C={C} T={T}", flush=True)
     tr, dcf = radial(nfe, spf, T)
     x, b1, z = gpu._t(crandn(T, H, W)), gpu._t(crandn(C, H, W)), gpu._t(crandn(T, C, nfe * spf))
     outs = []
