@@ -38,21 +38,42 @@ using NcPlan384 = FftCt<384, 16, 8, 3>;
 constexpr int kNcCtCols = 8;
 
 // ------------------------------------------------------------ K, column pass
+// The first stage's input b1_c (u - usub) dxi dyi, zero outside the image.
+// With SLOTS (the compiled kernels) a lane keeps (u - usub) dxi dyi of its
+// first-stage elements in registers from the block's first coil (`first`)
+// and loads only the coil map for the others: the lane -> element mapping of
+// a stage is the same for every coil.  The product is formed the same way
+// either way.
+template <int SLOTS>
 struct NcColFwdLoad {
   const float2 *ut, *us, *bc;
   const float *dxi, *dyi;
   int W, H, Gy, xb0;
-  bool delta;
-  __device__ float2 operator()(int b, int i, int) const {
+  bool delta, first;
+  float2 img[SLOTS > 0 ? SLOTS : 1];
+  __device__ float2 operator()(int b, int i, int slot) {
     const int x = xb0 + b;
     if (x >= W) return czero();
     const int y = wrap_up(i + H / 2, Gy);
     if (y >= H) return czero();
     const unsigned si = (unsigned)(y * W + x);
-    const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
-    return cmul(bc[si], cscale(v, dxi[x] * dyi[y]));
+    if constexpr (SLOTS > 0) {
+      if (first) img[slot] = cscale(delta ? csub(ut[si], us[si]) : ut[si], dxi[x] * dyi[y]);
+      return cmul(bc[si], img[slot]);
+    } else {
+      const float2 v = delta ? csub(ut[si], us[si]) : ut[si];
+      return cmul(bc[si], cscale(v, dxi[x] * dyi[y]));
+    }
   }
 };
+// First-stage elements per lane of the compiled column forward pass.
+template <class PL>
+constexpr int nc_colf_slots() {
+  if constexpr (PL::L == 0)
+    return 0;
+  else
+    return (kNcCtCols * PL::L / PL::R0 + AVIONIC_NC_COLF_THREADS - 1) / AVIONIC_NC_COLF_THREADS * PL::R0;
+}
 struct NcColFwdStore {
   float2 *tc;  // tmp rows of (t, c): [Gy][W]
   int W, xb0;
@@ -76,7 +97,7 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
   const int XB = CT ? kNcCtCols : g.XBf;
   const float2 *tw = stage_twiddles(g.py, twl, tid, NCF);
   __syncthreads();
-  NcColFwdLoad ld;
+  NcColFwdLoad<nc_colf_slots<PL>()> ld;
   ld.ut = u + t * HW;
   ld.us = usub ? usub + t * HW : nullptr;
   ld.delta = usub != nullptr;
@@ -91,6 +112,7 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
   const int cpg = (g.C + gridDim.z - 1) / gridDim.z;
   const int c0 = blockIdx.z * cpg, c1 = min(g.C, c0 + cpg);
   for (int c = c0; c < c1; ++c) {
+    ld.first = c == c0;
     ld.bc = b1 + c * HW;
     st.tc = tmp + ((long)t * g.C + c) * g.Gy * g.W;
     if constexpr (CT)
