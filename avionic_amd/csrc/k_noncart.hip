@@ -238,26 +238,33 @@ struct NcColAdjStore {
 };
 
 // Spreading by gathers, balanced over the lanes: a block takes its column
-// tile's entries (NcGeom::tce, in cell order) in chunks of up to NCE; the
-// lanes form the products z[m] w of a chunk, each lane NCC apart (their
-// index and z loads issued together), into the LDS tile; then every cell sums
-// the products of its own entries in entry order from LDS.  Dense cells at
-// the k-space centre (a hundred entries) then cost LDS reads, not a chain of
-// dependent global loads on one lane.  The sums keep each cell's entry order,
-// so every application gives the same bits.  (Running a block over several
-// coils of a frame, to read the tile's tables once, measured slower: 0.43
-// against 0.37 ms at the 256x256x30x12 radial geometry.)
-template <bool WIDE, class PL>
+// tile's entries (NcGeom::tce, in cell order) in chunks; the lanes form the
+// products z[m] w of a chunk, each lane NCC apart (their index and z loads
+// issued together), into the LDS tile; then every cell sums the products of
+// its own entries in entry order from LDS.  Dense cells at the k-space
+// centre (a hundred entries) then cost LDS reads, not a chain of dependent
+// global loads on one lane.  The sums keep each cell's entry order, so every
+// application gives the same bits.
+// NZ coils of a frame per block (2 in the compiled kernels): the tile's
+// offsets are staged and its entries loaded once for both, each entry's
+// products of both coils sit in the tile's two halves, and the two
+// transforms run one after the other.  Every coil's sums are the one-coil
+// sums.  (Running a block over all coils of a frame, one after the other,
+// measured slower: 0.43 against 0.37 ms at the 256x256x30x12 radial
+// geometry.)
+template <bool WIDE, class PL, int NZ>
 __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__restrict__ z,
                                                     float2 *__restrict__ tmp) {
   constexpr int CPL = NCE / NCC;  // cells per lane (XBa Gy <= NCE)
   constexpr int U = 4;            // products per lane issued together
+  constexpr int CH = NCE / NZ;    // entries per chunk
   __shared__ float2 tile[NCE];
   __shared__ float2 twl[nc_table_len(WIDE)];
   extern __shared__ int cps[];  // XBa * Gy + 1 ints, sized at launch
   const int tid = threadIdx.x;
-  const int plane = blockIdx.y;
-  const int t = plane / g.C;
+  const int ngrp = (g.C + NZ - 1) / NZ;  // coil groups per frame
+  const int t = blockIdx.y / ngrp, cz = (blockIdx.y % ngrp) * NZ;
+  const int nz = min(NZ, g.C - cz);  // coils of this block
   const int X = blockIdx.x, nX = gridDim.x;
   const float2 *tw = stage_twiddles(g.py, twl, tid, NCC);
   constexpr bool CT = PL::L != 0;
@@ -265,25 +272,31 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
   const int *tc = g.tcell + ((long)t * nX + X) * (ncell + 1);
   for (int k = tid; k <= ncell; k += NCC) cps[k] = tc[k];
   const int2 *ent = g.tce + g.tstart[t * nX + X];
-  const float2 *zp = z + (long)plane * g.M;
+  const float2 *zp = z + ((long)t * g.C + cz) * g.M;  // coil cz + q at zp + q M
   __syncthreads();
   const int E = cps[ncell];
-  float2 acc[CPL];
+  float2 acc[NZ][CPL];
 #pragma unroll
-  for (int j = 0; j < CPL; ++j) acc[j] = czero();
-  for (int c0 = 0; c0 < E; c0 += NCE) {
-    const int n = min(NCE, E - c0);
+  for (int q = 0; q < NZ; ++q)
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[q][j] = czero();
+  for (int c0 = 0; c0 < E; c0 += CH) {
+    const int n = min(CH, E - c0);
     for (int e0 = 0; e0 < n; e0 += U * NCC) {
       int2 mw[U];
-      float2 zv[U];
+      float2 zv[NZ][U];
 #pragma unroll
       for (int u = 0; u < U; ++u) mw[u] = ent[c0 + min(e0 + u * NCC + tid, n - 1)];
 #pragma unroll
-      for (int u = 0; u < U; ++u) zv[u] = zp[mw[u].x];
+      for (int q = 0; q < NZ; ++q)
+#pragma unroll
+        for (int u = 0; u < U; ++u) zv[q][u] = zp[(long)(q < nz ? q : 0) * g.M + mw[u].x];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int e = e0 + u * NCC + tid;
-        if (e < n) tile[e] = cscale(zv[u], __int_as_float(mw[u].y));
+        if (e < n)
+#pragma unroll
+          for (int q = 0; q < NZ; ++q) tile[q * CH + e] = cscale(zv[q][u], __int_as_float(mw[u].y));
       }
     }
     __syncthreads();
@@ -292,25 +305,32 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
       const int k = tid + j * NCC;
       if (k < ncell) {
         const int e1 = min(cps[k + 1], c0 + n);
-        for (int e = max(cps[k], c0); e < e1; ++e) acc[j] = cadd(acc[j], tile[e - c0]);
+        for (int e = max(cps[k], c0); e < e1; ++e)
+#pragma unroll
+          for (int q = 0; q < NZ; ++q) acc[q][j] = cadd(acc[q][j], tile[q * CH + e - c0]);
       }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < CPL; ++j) {
-    const int k = tid + j * NCC;
-    if (k < ncell) tile[k] = acc[j];
+  for (int q = 0; q < NZ; ++q) {
+    if (q >= nz) break;  // uniform over the block
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      const int k = tid + j * NCC;
+      if (k < ncell) tile[k] = acc[q][j];
+    }
+    __syncthreads();
+    NcColAdjStore st{tmp + ((long)t * g.C + cz + q) * g.H * g.Gx, g.H, CT ? PL::L : g.Gx,
+                     CT ? PL::L : g.Gy, X * XB};
+    NoLoad nl;
+    if constexpr (CT)
+      fft_io_ct<NCC, PL, kNcCtCols, true, false, true>(tile, kNcCtCols, 1, kNcCtCols, +1.f, tid,
+                                                       tw, nl, st);
+    else
+      fft_io<NCC, NCE, true, false, true, WIDE>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
+    __syncthreads();  // every lane has read the tile before the next coil writes it
   }
-  __syncthreads();
-  NcColAdjStore st{tmp + (long)plane * g.H * g.Gx, g.H, CT ? PL::L : g.Gx, CT ? PL::L : g.Gy,
-                   X * XB};
-  NoLoad nl;
-  if constexpr (CT)
-    fft_io_ct<NCC, PL, kNcCtCols, true, false, true>(tile, kNcCtCols, 1, kNcCtCols, +1.f, tid, tw,
-                                                     nl, st);
-  else
-    fft_io<NCC, NCE, true, false, true, WIDE>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
 }
 
 // ------------------------------------------------------------ K^H, row pass
@@ -438,13 +458,17 @@ int launch_nc_samples(int mode, const NcGeom &g, const float2 *part, float2 *out
 void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float2 *tmp,
                        float2 *img, hipStream_t s) {
   const size_t cps_bytes = sizeof(int) * ((size_t)g.XBa * g.Gy + 1);
-  const dim3 gc((g.Gx + g.XBa - 1) / g.XBa, g.T * g.C), gr((g.H + g.RBa - 1) / g.RBa, g.T);
+  const int nXa = (g.Gx + g.XBa - 1) / g.XBa;
+  const dim3 gr((g.H + g.RBa - 1) / g.RBa, g.T);
   if (nc_ct384(g))
-    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlan384>), gc, dim3(NCC), cps_bytes, s, g, z, tmp);
+    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlan384, 2>), dim3(nXa, g.T * ((g.C + 1) / 2)),
+                       dim3(NCC), cps_bytes, s, g, z, tmp);
   else if (g.wide)
-    hipLaunchKernelGGL((k_nc_col_adj<true, NcPlanRt>), gc, dim3(NCC), cps_bytes, s, g, z, tmp);
+    hipLaunchKernelGGL((k_nc_col_adj<true, NcPlanRt, 1>), dim3(nXa, g.T * g.C), dim3(NCC),
+                       cps_bytes, s, g, z, tmp);
   else
-    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlanRt>), gc, dim3(NCC), cps_bytes, s, g, z, tmp);
+    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlanRt, 1>), dim3(nXa, g.T * g.C), dim3(NCC),
+                       cps_bytes, s, g, z, tmp);
   AV_LAUNCH_CHECK();
   if (nc_ct384(g))
     hipLaunchKernelGGL((k_nc_row_adj<false, NcPlan384>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
