@@ -125,51 +125,74 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
 }
 
 // --------------------------------------------------------------- K, row pass
+// Transform b of a block: coil cz + b / RB, grid row gy0 + b % RB of that
+// coil's tmp plane (zero past the block's coils or rows).
+template <int NZ>
 struct NcRowFwdLoad {
-  const float2 *tr;  // tmp rows gy0.. of the plane: [.][W]
-  int W, Gx;
+  const float2 *tr;  // tmp row gy0 of plane (t, cz): [.][W]; coil planes GyW apart
+  long plane;        // Gy W
+  int W, Gx, RB, nr, nz;
   __device__ float2 operator()(int b, int i, int) const {
     const int x = wrap_up(i + W / 2, Gx);
-    return x < W ? tr[(unsigned)(b * W + x)] : czero();
+    if constexpr (NZ == 1) {
+      return x < W ? tr[(unsigned)(b * W + x)] : czero();
+    } else {
+      const int q = b / RB, r = b - q * RB;
+      return x < W && q < nz && r < nr ? tr[q * plane + (unsigned)(r * W + x)] : czero();
+    }
   }
 };
 
-template <bool WIDE, class PL>
+// NZ coils of a frame per block (2 in the compiled kernels, RB = 5 grid rows
+// each): the tile holds NZ x RB transformed rows, and the row table's entries
+// of those rows are loaded once for all coils.  Each coil's partials are the
+// one-coil partials.
+template <bool WIDE, class PL, int NZ>
 __global__ __launch_bounds__(NCR) void k_nc_row_fwd(NcGeom g, const float2 *__restrict__ tmp,
                                                     float2 *__restrict__ part) {
   __shared__ float2 tile[NCE];
   __shared__ float2 twl[nc_table_len(WIDE)];
-  const int tid = threadIdx.x;
-  const int plane = blockIdx.y;
-  const int t = plane / g.C;
-  const int gy0 = blockIdx.x * g.RBf;
-  const int nr = min(g.RBf, g.Gy - gy0);
   constexpr bool CT = PL::L != 0;
+  static_assert(CT || NZ == 1, "coil groups need the compiled kernels");
+  const int tid = threadIdx.x;
+  const int RB = CT ? NCE / PL::L / NZ : g.RBf;  // grid rows per coil
+  const int ngrp = (g.C + NZ - 1) / NZ;
+  const int t = blockIdx.y / ngrp, cz = (blockIdx.y % ngrp) * NZ;
+  const int nz = min(NZ, g.C - cz);
+  const int gy0 = blockIdx.x * RB;
+  const int nr = min(RB, g.Gy - gy0);
   const int Gx = CT ? PL::L : g.Gx;
   const float2 *tw = stage_twiddles(g.px, twl, tid, NCR);
   __syncthreads();
-  NcRowFwdLoad ld{tmp + ((long)plane * g.Gy + gy0) * g.W, g.W, Gx};
+  const long plane = (long)g.Gy * g.W;
+  NcRowFwdLoad<NZ> ld{tmp + ((long)t * g.C + cz) * plane + (long)gy0 * g.W, plane, g.W, Gx, RB, nr,
+                  nz};
   NoStore ns;
   if constexpr (CT)
-    fft_io_ct<NCR, PL, NCE / PL::L, false, true, false>(tile, nr, PL::L, 1, -1.f, tid, tw, ld, ns);
+    fft_io_ct<NCR, PL, NCE / PL::L, false, true, false>(tile, NZ * RB, PL::L, 1, -1.f, tid, tw, ld,
+                                                        ns);
   else
     fft_io<NCR, NCE, false, true, false, WIDE>(tile, g.px, nr, Gx, 1, -1.f, tid, tw, ld, ns);
-  // tile[b * Gx + i] holds grid row gy0 + b in natural order.
+  // tile[(q RB + r) Gx + i] holds grid row gy0 + r of coil cz + q in natural order.
   const int *rp = g.rptr + (long)t * (g.Gy + 1);
   const int e0 = rp[gy0], e1 = rp[gy0 + nr];
-  float2 *pp = part + (long)plane * g.M * kNcTaps;
+  float2 *pp = part + ((long)t * g.C + cz) * g.M * kNcTaps;  // coil cz + q: M kNcTaps further
   for (int e = e0 + tid; e < e1; e += NCR) {
     // (partial index m*4 + dy, grid row, first grid column, row weight) and
     // the column weights: no load here depends on another
     const int4 h = g.rhdr[e];
     const float4 wx = g.rwx[e];
-    const float2 *row = tile + (h.y - gy0) * Gx;
     const int qx = h.z;
-    float2 acc = cscale(row[qx], wx.x);
-    acc = cadd(acc, cscale(row[wrap_up(qx + 1, Gx)], wx.y));
-    acc = cadd(acc, cscale(row[wrap_up(qx + 2, Gx)], wx.z));
-    acc = cadd(acc, cscale(row[wrap_up(qx + 3, Gx)], wx.w));
-    pp[(unsigned)h.x] = cscale(acc, __int_as_float(h.w));
+#pragma unroll
+    for (int q = 0; q < NZ; ++q) {
+      if (q >= nz) break;
+      const float2 *row = tile + (q * RB + h.y - gy0) * Gx;
+      float2 acc = cscale(row[qx], wx.x);
+      acc = cadd(acc, cscale(row[wrap_up(qx + 1, Gx)], wx.y));
+      acc = cadd(acc, cscale(row[wrap_up(qx + 2, Gx)], wx.z));
+      acc = cadd(acc, cscale(row[wrap_up(qx + 3, Gx)], wx.w));
+      pp[(long)q * g.M * kNcTaps + (unsigned)h.x] = cscale(acc, __int_as_float(h.w));
+    }
   }
 }
 
@@ -245,10 +268,11 @@ struct NcColAdjStore {
 // centre (a hundred entries) then cost LDS reads, not a chain of dependent
 // global loads on one lane.  The sums keep each cell's entry order, so every
 // application gives the same bits.
-// NZ coils of a frame per block (2 in the compiled kernels): the tile's
-// offsets are staged and its entries loaded once for both, each entry's
-// products of both coils sit in the tile's two halves, and the two
-// transforms run one after the other.  Every coil's sums are the one-coil
+// NZ coils of a frame per block (3 in the compiled kernels; 2 and 4 measured
+// slower at the 256x256x30x12 radial geometry): the tile's offsets are
+// staged and its entries loaded once for all of them, each entry's products
+// of the NZ coils sit in NZ parts of the tile, and the transforms run one
+// after the other.  Every coil's sums are the one-coil
 // sums.  (Running a block over all coils of a frame, one after the other,
 // measured slower: 0.43 against 0.37 ms at the 256x256x30x12 radial
 // geometry.)
@@ -419,7 +443,9 @@ bool nc_ct384(const NcGeom &g) {
 
 void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub,
                             const float2 *b1, float2 *tmp, float2 *part, hipStream_t s) {
-  const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG), gr((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C);
+  const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG);
+  const int rb2 = NCE / 384 / 2;  // grid rows per coil of the compiled two-coil row pass
+  const dim3 gr((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C), gr2((g.Gy + rb2 - 1) / rb2, g.T * ((g.C + 1) / 2));
   if (nc_ct384(g))
     hipLaunchKernelGGL((k_nc_col_fwd<false, NcPlan384>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
   else if (g.wide)
@@ -428,11 +454,11 @@ void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub
     hipLaunchKernelGGL((k_nc_col_fwd<false, NcPlanRt>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
   AV_LAUNCH_CHECK();
   if (nc_ct384(g))
-    hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlan384>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+    hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlan384, 2>), gr2, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
   else if (g.wide)
-    hipLaunchKernelGGL((k_nc_row_fwd<true, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+    hipLaunchKernelGGL((k_nc_row_fwd<true, NcPlanRt, 1>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
   else
-    hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+    hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlanRt, 1>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
   AV_LAUNCH_CHECK();
 }
 
@@ -461,7 +487,7 @@ void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float
   const int nXa = (g.Gx + g.XBa - 1) / g.XBa;
   const dim3 gr((g.H + g.RBa - 1) / g.RBa, g.T);
   if (nc_ct384(g))
-    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlan384, 2>), dim3(nXa, g.T * ((g.C + 1) / 2)),
+    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlan384, 3>), dim3(nXa, g.T * ((g.C + 2) / 3)),
                        dim3(NCC), cps_bytes, s, g, z, tmp);
   else if (g.wide)
     hipLaunchKernelGGL((k_nc_col_adj<true, NcPlanRt, 1>), dim3(nXa, g.T * g.C), dim3(NCC),
