@@ -120,6 +120,15 @@ struct CtStage {
   }
 };
 
+// First-stage elements per lane of a compile-time plan (0: runtime plan).
+template <class P>
+constexpr int col_first_slots() {
+  if constexpr (P::H == 0)
+    return 0;
+  else
+    return CtStage<P::R0, P::H, P::XB, 1>::MAXB * P::R0;
+}
+
 // Outputs v of stage S0 become inputs w of stage S1 through the LDS tile
 // (element (b, i) at b + i B).  Callers sync before writing the tile again.
 template <int B, class S0, class S1>
@@ -174,14 +183,14 @@ __device__ __forceinline__ void col_fft_ct(float2 *tile, float sign, int tid, co
   for (int c = 0; c < S0::MAXB; ++c)
     if (S0::lane(tid, c, b, j))
 #pragma unroll
-      for (int r = 0; r < S0::R; ++r) raw[c][r] = ld.raw(b, S0::in_index(j, r));
+      for (int r = 0; r < S0::R; ++r) raw[c][r] = ld.raw(b, S0::in_index(j, r), c * S0::R + r);
   __builtin_amdgcn_sched_barrier(0);
   float2 v[S0::MAXB][S0::R];
 #pragma unroll
   for (int c = 0; c < S0::MAXB; ++c)
     if (S0::lane(tid, c, b, j)) {
 #pragma unroll
-      for (int r = 0; r < S0::R; ++r) v[c][r] = ld.finish(raw[c][r], b);
+      for (int r = 0; r < S0::R; ++r) v[c][r] = ld.finish(raw[c][r], b, c * S0::R + r);
       S0::butterfly(v[c], j, tw, sign);
     }
   float2 w[S1::MAXB][S1::R];
@@ -211,14 +220,21 @@ __device__ __forceinline__ void col_fft_ct(float2 *tile, float sign, int tid, co
 // branch whose value is used after it gets its own wait, which serialises the
 // lane's loads.
 // Index products use 24-bit multiplies (rows and columns are below 2^12).
-template <bool DELTA, int HS>
+// SLOTS (compile-time plans: the first stage's elements per lane): the lane
+// keeps its image values (u, or u - usub) in registers from the block's
+// first coil (`first`) and loads only the coil map for the others; the lane
+// -> element mapping of a stage is the same for every coil, and the product
+// is formed the same way, so the values are those of the uncached form.
+template <bool DELTA, int HS, int SLOTS = 0>
 struct ColFwdLoad {
   const float2 *ut, *st, *bc;
   int H, W, xb0, centered;
+  bool first = true;
+  float2 img[SLOTS > 0 ? SLOTS : 1];
   struct Raw {
     float2 b, u, s;
   };
-  __device__ Raw raw(int b, int i) const {
+  __device__ Raw raw(int b, int i, int) const {
     const int Hh = HS ? HS : H;
     const bool cen = centered != 0;
     const int x = xb0 + b;
@@ -226,12 +242,18 @@ struct ColFwdLoad {
     const int sy = cen ? wrap_half(i, Hh) : i;
     const int sx = cen ? wrap_half(xc, W) : xc;
     const unsigned si = __umul24((unsigned)sy, (unsigned)W) + (unsigned)sx;
+    if (SLOTS > 0 && !first) return Raw{gld(bc, si), czero(), czero()};
     return Raw{gld(bc, si), gld(ut, si), DELTA ? gld(st, si) : czero()};
   }
-  __device__ float2 finish(const Raw &r, int b) const {
-    return xb0 + b < W ? cmul(r.b, DELTA ? csub(r.u, r.s) : r.u) : czero();
+  __device__ float2 finish(const Raw &r, int b, int slot) {
+    if constexpr (SLOTS > 0) {
+      if (first) img[slot] = DELTA ? csub(r.u, r.s) : r.u;
+      return xb0 + b < W ? cmul(r.b, img[slot]) : czero();
+    } else {
+      return xb0 + b < W ? cmul(r.b, DELTA ? csub(r.u, r.s) : r.u) : czero();
+    }
   }
-  __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i), b); }
+  __device__ float2 operator()(int b, int i, int slot) { return finish(raw(b, i, slot), b, slot); }
 };
 template <int HS>
 struct ColFwdStore {
@@ -269,7 +291,7 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
   const int *rmap = stage_rowmap(g, t, rml, tid, NTC);
   __syncthreads();
   const int r0 = g.row_off[t], S = g.row_off[t + 1] - r0;
-  ColFwdLoad<DELTA, P::H> ld;
+  ColFwdLoad<DELTA, P::H, col_first_slots<P>()> ld;
   ld.ut = u + t * HW;
   ld.st = DELTA ? usub + t * HW : nullptr;
   ld.H = g.H;
@@ -278,6 +300,7 @@ __device__ inline void col_forward_fused(const CartGeom &g, int xbi, int t, cons
   ld.centered = g.centered;
   ColFwdStore<P::H> st{nullptr, rmap, g.H, g.W, xbi * XB, g.centered};
   for (int c = 0; c < g.C; ++c) {
+    ld.first = c == 0;
     ld.bc = b1 + c * HW;
     st.kc = kcol + ((long)g.C * r0 + (long)c * S) * g.W;
     if constexpr (P::H == 0)
@@ -302,7 +325,7 @@ struct ColAdjLoad {
     float2 z;
     bool use;
   };
-  __device__ Raw raw(int b, int i) const {
+  __device__ Raw raw(int b, int i, int) const {
     const int Hh = HS ? HS : H;
     const bool cen = centered != 0;
     const int x = xb0 + b;
@@ -311,8 +334,8 @@ struct ColAdjLoad {
     const bool use = x < W && k >= 0;
     return Raw{gld(zh, use ? __umul24((unsigned)k, (unsigned)W) + (unsigned)x : 0u), use};
   }
-  __device__ float2 finish(const Raw &r, int) const { return r.use ? r.z : czero(); }
-  __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i), 0); }
+  __device__ float2 finish(const Raw &r, int, int) const { return r.use ? r.z : czero(); }
+  __device__ float2 operator()(int b, int i, int) const { return finish(raw(b, i, 0), 0, 0); }
 };
 // The coil sum stays in registers, one accumulator per output of the lane.
 // PRE: the coil map values are loaded by prefetch() ahead of the last stage.
