@@ -41,6 +41,9 @@ CASES = [
     (16, 16, 1, 1, 16, 3, 2, 1.25),
     (64, 64, 4, 2, 128, 9, 3, 1.25),
     (256, 256, 2, 2, 512, 21, 3, 1.5),
+    # grid 384: the compiled kernels, with coil groups that do not divide C = 5 (adjoint column
+    # blocks of 3 + 2 coils, forward row blocks of 2 + 2 + 1)
+    (256, 256, 5, 1, 384, 7, 3, 1.5),
     # 384 x 384 at the reference's default osf = 2, J = 3 (CUDA/config/default.cfg:119-121,
     # noncartesian_operator.cc:248-255): a 768-point grid, the wide kernel variants
     (384, 384, 1, 1, 768, 7, 3, 2.0),
