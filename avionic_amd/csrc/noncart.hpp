@@ -9,19 +9,22 @@
 // neighbourhood with separable Kaiser-Bessel weights.
 //
 // K as three passes, none of which writes the oversampled grid:
-//   1. column pass (grid (ceil(W/XB), T), coil loop inside): image columns ->
-//      zero-padded length-Gy transforms -> tmp [t][c][Gy][W];
-//   2. row pass (grid (ceil(Gy/RB), T*C)): zero-padded length-Gx transforms of
-//      RB grid rows in LDS; every trajectory point touching one of those rows
-//      reads its J neighbours there and writes one partial sum per row it
-//      touches -> part [t][c][M][4];
+//   1. column pass (grid (ceil(W/XB), T, CG), a coil loop over each of the CG
+//      coil groups inside): image columns -> zero-padded length-Gy transforms
+//      -> tmp [t][c][Gy][W];
+//   2. row pass (grid (ceil(Gy/RB), T*C); the kernels compiled for grid length
+//      384 take two coils of a frame per block): zero-padded length-Gx
+//      transforms of RB grid rows in LDS; every trajectory point touching one
+//      of those rows reads its J neighbours there and writes one partial sum
+//      per row it touches -> part [t][c][M][4];
 //   3. sample pass: the four partials of a point, scaled, feed K u (or the z
 //      update, or a norm reduction).
 // K^H as two passes:
-//   1. column pass (grid (ceil(Gx/XB), T*C)): the spreading, by gathers
-//      without atomics: the lanes form the products of the column tile's
-//      cell -> point entries (tce) into LDS, each grid cell sums its own in
-//      entry order; length-Gy inverse transform, keeps the H image rows ->
+//   1. column pass (grid (ceil(Gx/XB), T*C); the compiled kernels take three
+//      coils of a frame per block): the spreading, by gathers without
+//      atomics: the lanes form the products of the column tile's cell ->
+//      point entries (tce) into LDS, each grid cell sums its own in entry
+//      order; length-Gy inverse transform, keeps the H image rows ->
 //      tmp [t][c][H][Gx];
 //   2. row pass (grid (ceil(H/RB), T), coil loop inside): length-Gx inverse
 //      transforms, crop to W, conj(b1_c) coil sum in registers, deapodise.
