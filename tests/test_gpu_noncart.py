@@ -44,6 +44,8 @@ CASES = [
     # grid 384: the compiled kernels, with coil groups that do not divide C = 5 (adjoint column
     # blocks of 3 + 2 coils, forward row blocks of 2 + 2 + 1)
     (256, 256, 5, 1, 384, 7, 3, 1.5),
+    # grid 384 from a 192 x 192 image at osf 2: the compiled kernels with more zero padding
+    (192, 192, 3, 2, 256, 13, 3, 2.0),
     # 384 x 384 at the reference's default osf = 2, J = 3 (CUDA/config/default.cfg:119-121,
     # noncartesian_operator.cc:248-255): a 768-point grid, the wide kernel variants
     (384, 384, 1, 1, 768, 7, 3, 2.0),
