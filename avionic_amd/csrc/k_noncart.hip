@@ -26,15 +26,17 @@ constexpr int row_slots_nc(bool wide) { return max_slots(NCE, NCR, wide); }
 __device__ inline int wrap_up(int v, int G) { return v >= G ? v - G : v; }
 
 // Grid transforms: on the geometry's runtime plans (NcPlanRt), or compiled
-// for grid length 384 in both directions (a 256 x 256 image at oversampling
-// 1.5), radices 16, 8, 3 as make_fft_plan orders them, with kNcCtCols grid
-// columns per column-pass block (pick_cols at that length).  nc_ct384 picks
-// the compiled kernels; they run the runtime plan's stages with the index
-// arithmetic folded into constants.
+// for a grid length in both directions: 384 (a 256 x 256 image at
+// oversampling 1.5; radices 16, 8, 3 as make_fft_plan orders them) or 512
+// (256 x 256 at 2, the reference's default oversampling; 16, 16, 2), with
+// kNcCtCols grid columns per column-pass block (pick_cols at both lengths).
+// nc_ct_len picks the compiled kernels; they run the runtime plan's stages
+// with the index arithmetic folded into constants.
 struct NcPlanRt {
   static constexpr int L = 0;
 };
 using NcPlan384 = FftCt<384, 16, 8, 3>;
+using NcPlan512 = FftCt<512, 16, 16, 2>;
 constexpr int kNcCtCols = 8;
 
 // ------------------------------------------------------------ K, column pass
@@ -430,32 +432,63 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
 // host side
 
 namespace {
-// Both grid plans are make_fft_plan's length-384 plan (16, 8, 3) and the
-// column tiles have the compiled width: the compile-time kernels apply.
-bool nc_ct384(const NcGeom &g) {
-  auto is384 = [](const FftPlan &p) {
-    return p.L == 384 && p.nst == 3 && p.radix[0] == 16 && p.radix[1] == 8 && p.radix[2] == 3;
+// The compiled grid length that applies (0: none): both grid plans are
+// make_fft_plan's plan of that length and the tiles have the compiled sizes.
+int nc_ct_len(const NcGeom &g) {
+  auto is = [](const FftPlan &p, int L, int r0, int r1, int r2) {
+    return p.L == L && p.nst == 3 && p.radix[0] == r0 && p.radix[1] == r1 && p.radix[2] == r2;
   };
-  return !g.wide && is384(g.px) && is384(g.py) && g.XBf == kNcCtCols && g.XBa == kNcCtCols &&
-         g.RBf <= NCE / 384 && g.RBa <= NCE / 384;
+  if (g.wide || g.XBf != kNcCtCols || g.XBa != kNcCtCols) return 0;
+  for (int L : {384, 512}) {
+    const bool plan = L == 384 ? is(g.px, 384, 16, 8, 3) && is(g.py, 384, 16, 8, 3)
+                               : is(g.px, 512, 16, 16, 2) && is(g.py, 512, 16, 16, 2);
+    if (plan && g.RBf <= NCE / L && g.RBa <= NCE / L) return L;
+  }
+  return 0;
+}
+
+// K's grid passes on a compiled plan: row blocks of two coils, NCE / L / 2
+// grid rows each.
+template <class PL>
+void nc_forward_ct(const NcGeom &g, const float2 *u, const float2 *usub, const float2 *b1,
+                   float2 *tmp, float2 *part, hipStream_t s) {
+  constexpr int rb2 = NCE / PL::L / 2;
+  const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG);
+  const dim3 gr2((g.Gy + rb2 - 1) / rb2, g.T * ((g.C + 1) / 2));
+  hipLaunchKernelGGL((k_nc_col_fwd<false, PL>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
+  AV_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_nc_row_fwd<false, PL, 2>), gr2, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
+  AV_LAUNCH_CHECK();
+}
+// K^H's grid passes on a compiled plan: adjoint column blocks of three coils.
+template <class PL>
+void nc_adjoint_ct(const NcGeom &g, const float2 *z, const float2 *b1, float2 *tmp, float2 *img,
+                   hipStream_t s) {
+  const size_t cps_bytes = sizeof(int) * ((size_t)g.XBa * g.Gy + 1);
+  const int nXa = (g.Gx + g.XBa - 1) / g.XBa;
+  hipLaunchKernelGGL((k_nc_col_adj<false, PL, 3>), dim3(nXa, g.T * ((g.C + 2) / 3)), dim3(NCC),
+                     cps_bytes, s, g, z, tmp);
+  AV_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_nc_row_adj<false, PL>), dim3((g.H + g.RBa - 1) / g.RBa, g.T), dim3(NCR), 0,
+                     s, g, (const float2 *)tmp, b1, img);
+  AV_LAUNCH_CHECK();
 }
 }  // namespace
 
 void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub,
                             const float2 *b1, float2 *tmp, float2 *part, hipStream_t s) {
-  const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG);
-  const int rb2 = NCE / 384 / 2;  // grid rows per coil of the compiled two-coil row pass
-  const dim3 gr((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C), gr2((g.Gy + rb2 - 1) / rb2, g.T * ((g.C + 1) / 2));
-  if (nc_ct384(g))
-    hipLaunchKernelGGL((k_nc_col_fwd<false, NcPlan384>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
-  else if (g.wide)
+  switch (nc_ct_len(g)) {
+    case 384: nc_forward_ct<NcPlan384>(g, u, usub, b1, tmp, part, s); return;
+    case 512: nc_forward_ct<NcPlan512>(g, u, usub, b1, tmp, part, s); return;
+    default: break;
+  }
+  const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG), gr((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C);
+  if (g.wide)
     hipLaunchKernelGGL((k_nc_col_fwd<true, NcPlanRt>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
   else
     hipLaunchKernelGGL((k_nc_col_fwd<false, NcPlanRt>), gc, dim3(NCF), 0, s, g, u, usub, b1, tmp);
   AV_LAUNCH_CHECK();
-  if (nc_ct384(g))
-    hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlan384, 2>), gr2, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
-  else if (g.wide)
+  if (g.wide)
     hipLaunchKernelGGL((k_nc_row_fwd<true, NcPlanRt, 1>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
   else
     hipLaunchKernelGGL((k_nc_row_fwd<false, NcPlanRt, 1>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, part);
@@ -483,22 +516,22 @@ int launch_nc_samples(int mode, const NcGeom &g, const float2 *part, float2 *out
 
 void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float2 *tmp,
                        float2 *img, hipStream_t s) {
+  switch (nc_ct_len(g)) {
+    case 384: nc_adjoint_ct<NcPlan384>(g, z, b1, tmp, img, s); return;
+    case 512: nc_adjoint_ct<NcPlan512>(g, z, b1, tmp, img, s); return;
+    default: break;
+  }
   const size_t cps_bytes = sizeof(int) * ((size_t)g.XBa * g.Gy + 1);
   const int nXa = (g.Gx + g.XBa - 1) / g.XBa;
   const dim3 gr((g.H + g.RBa - 1) / g.RBa, g.T);
-  if (nc_ct384(g))
-    hipLaunchKernelGGL((k_nc_col_adj<false, NcPlan384, 3>), dim3(nXa, g.T * ((g.C + 2) / 3)),
-                       dim3(NCC), cps_bytes, s, g, z, tmp);
-  else if (g.wide)
+  if (g.wide)
     hipLaunchKernelGGL((k_nc_col_adj<true, NcPlanRt, 1>), dim3(nXa, g.T * g.C), dim3(NCC),
                        cps_bytes, s, g, z, tmp);
   else
     hipLaunchKernelGGL((k_nc_col_adj<false, NcPlanRt, 1>), dim3(nXa, g.T * g.C), dim3(NCC),
                        cps_bytes, s, g, z, tmp);
   AV_LAUNCH_CHECK();
-  if (nc_ct384(g))
-    hipLaunchKernelGGL((k_nc_row_adj<false, NcPlan384>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
-  else if (g.wide)
+  if (g.wide)
     hipLaunchKernelGGL((k_nc_row_adj<true, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);
   else
     hipLaunchKernelGGL((k_nc_row_adj<false, NcPlanRt>), gr, dim3(NCR), 0, s, g, (const float2 *)tmp, b1, img);

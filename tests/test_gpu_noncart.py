@@ -46,6 +46,8 @@ CASES = [
     (256, 256, 5, 1, 384, 7, 3, 1.5),
     # grid 384 from a 192 x 192 image at osf 2: the compiled kernels with more zero padding
     (192, 192, 3, 2, 256, 13, 3, 2.0),
+    # grid 512: 256 x 256 at the reference's default osf 2, the kernels compiled for length 512
+    (256, 256, 4, 2, 512, 9, 3, 2.0),
     # 384 x 384 at the reference's default osf = 2, J = 3 (CUDA/config/default.cfg:119-121,
     # noncartesian_operator.cc:248-255): a 768-point grid, the wide kernel variants
     (384, 384, 1, 1, 768, 7, 3, 2.0),
@@ -87,14 +89,16 @@ def test_adjointness_and_determinism(gpu):
     assert bool((op.ForwardOperation(z, b1) == khz).all())
 
 
-# 64 x 64 (grid 96) runs the runtime-plan kernels; 256 x 256 at osf 1.5 (grid
-# 384) the kernels compiled for that grid length.
-@pytest.mark.parametrize("W,H,C,T,nfe,spf", [(64, 64, 6, 2, 128, 9), (256, 256, 5, 1, 512, 5)])
-def test_launch_shapes_do_not_change_results(gpu, monkeypatch, W, H, C, T, nfe, spf):
+# 64 x 64 (grid 96) runs the runtime-plan kernels; 256 x 256 at osf 1.5 and 2
+# (grids 384 and 512) the kernels compiled for those grid lengths.
+@pytest.mark.parametrize("W,H,C,T,nfe,spf,osf", [(64, 64, 6, 2, 128, 9, 1.5),
+                                                 (256, 256, 5, 1, 512, 5, 1.5),
+                                                 (256, 256, 5, 1, 512, 5, 2.0)])
+def test_launch_shapes_do_not_change_results(gpu, monkeypatch, W, H, C, T, nfe, spf, osf):
     """The coil-group count of the forward column pass and the rows per
     adjoint row block are tuned per geometry; every shape gives bit-identical
     K and K^H."""
-    print(f"[gpu noncart shapes] W={W} H={H} C={C} T={T}", flush=True)
+    print(f"[gpu noncart shapes] W={W} H={H} C={C} T={T} osf={osf}", flush=True)
     tr, dcf = radial(nfe, spf, T)
     x, b1, z = gpu._t(crandn(T, H, W)), gpu._t(crandn(C, H, W)), gpu._t(crandn(T, C, nfe * spf))
     outs = []
@@ -103,7 +107,7 @@ def test_launch_shapes_do_not_change_results(gpu, monkeypatch, W, H, C, T, nfe, 
             monkeypatch.delenv("AVIONIC_NC_TILES", raising=False)
         else:
             monkeypatch.setenv("AVIONIC_NC_TILES", tiles)
-        op = make_op(gpu, W, H, C, T, nfe, spf, tr, dcf, 3, 1.5)
+        op = make_op(gpu, W, H, C, T, nfe, spf, tr, dcf, 3, osf)
         outs.append((op.BackwardOperation(x, b1), op.ForwardOperation(z, b1)))
     for kx, khz in outs[1:]:
         assert bool((kx == outs[0][0]).all()) and bool((khz == outs[0][1]).all())
