@@ -193,6 +193,9 @@ struct avionic_engine {
 
   void rebuild(Group &g) {
     TraceRange tr("avionic_engine_rebuild");
+    // The previous step's iterations may still run on the stream: its graphs
+    // are destroyed, and the task table rewritten, only once they are done.
+    AV_HIP(hipStreamSynchronize(stream));
     g.drop_graphs();
     // tasks ordered by column kind, then row kind: each kind's kernel is
     // launched over its own span of blocks only
@@ -264,6 +267,8 @@ struct avionic_engine {
       g.table.ensure(sizeof(Task) * tasks.size());
       AV_HIP(hipMemcpyAsync(g.table.p, tasks.data(), sizeof(Task) * tasks.size(),
                             hipMemcpyHostToDevice, stream));
+      // `tasks` (pageable) is freed on return: the copy completes first
+      AV_HIP(hipStreamSynchronize(stream));
     }
     g.dirty = false;
   }
