@@ -247,6 +247,28 @@ def test_engine_radial_matches_eager_and_oracle(gpu, method):
     np.testing.assert_allclose(log[0], ref["sigma"], rtol=1e-5)
 
 
+@pytest.mark.parametrize("osf", [1.5, 2.0])
+def test_engine_radial_compiled_grids_match_eager(gpu, osf):
+    """The engine at the grid lengths with compiled gridding passes (384 and
+    512 for a 256 x 256 image): equal to the eager PDRecon path, which the
+    operator cases and the full-size fixture pin against the oracle."""
+    W = H = 256
+    C, T, nfe, spf, J = 2, 2, 256, 13, 3
+    print(f"[gpu noncart engine] compiled grid, osf={osf}", flush=True)
+    data, b1, tr, dcf = _radial_problem(W, H, C, T, nfe, spf, J, osf, 23)
+    p = O.default_params("tgv2")
+    k, d = O.adapt_lambda_defaults("tgv2")
+    p.update(maxIt=40, lambda_=W / (spf * np.pi / 2) * k + d)
+    eager, sig, _ = _recon_gpu(gpu, 1, p, data, b1, tr, dcf, nfe, spf, J, osf)
+    op = make_op(gpu, W, H, C, T, nfe, spf, tr, dcf, J, osf)
+    req = gpu.A.Request(1, gpu._t(data), gpu._t(b1), x0=gpu._t(np.zeros((T, H, W))),
+                        params=_engine_params(p), op=op)
+    (x,), (log,) = _engine_run(gpu, [req])
+    assert np.isfinite(x).all()
+    assert rel(x, eager) < 1e-6, rel(x, eager)
+    np.testing.assert_allclose(log[0], sig, rtol=1e-6)
+
+
 def test_engine_mixes_cartesian_and_radial(gpu):
     """One TGV2 batch of a Cartesian request and two radial requests sharing
     one operator: each result equals the same request run alone."""
