@@ -26,18 +26,25 @@ constexpr int row_slots_nc(bool wide) { return max_slots(NCE, NCR, wide); }
 __device__ inline int wrap_up(int v, int G) { return v >= G ? v - G : v; }
 
 // Grid transforms: on the geometry's runtime plans (NcPlanRt), or compiled
-// for a grid length in both directions: 384 (a 256 x 256 image at
-// oversampling 1.5; radices 16, 8, 3 as make_fft_plan orders them) or 512
-// (256 x 256 at 2, the reference's default oversampling; 16, 16, 2), with
-// kNcCtCols grid columns per column-pass block (pick_cols at both lengths).
-// nc_ct_len picks the compiled kernels; they run the runtime plan's stages
-// with the index arithmetic folded into constants.
+// for a grid length in both directions, with the radices in make_fft_plan's
+// order and XB grid columns per column-pass block (pick_cols at that length):
+// 384 (a 256 x 256 image at oversampling 1.5; 16, 8, 3), 512 (256 x 256 at
+// 2, the reference's default oversampling; 16, 16, 2), 192 and 256 (128 x 128
+// at 1.5 and 2; 16, 4, 3 and 16, 16).  nc_ct_len picks the compiled kernels;
+// they run the runtime plan's stages with the index arithmetic folded into
+// constants.
 struct NcPlanRt {
-  static constexpr int L = 0;
+  static constexpr int L = 0, XB = 0;
 };
-using NcPlan384 = FftCt<384, 16, 8, 3>;
-using NcPlan512 = FftCt<512, 16, 16, 2>;
-constexpr int kNcCtCols = 8;
+template <class F, int XB_>
+struct NcCt : F {
+  static constexpr int XB = XB_;
+  static constexpr int RL = F::R2 > 1 ? F::R2 : F::R1;  // last radix
+};
+using NcPlan384 = NcCt<FftCt<384, 16, 8, 3>, 8>;
+using NcPlan512 = NcCt<FftCt<512, 16, 16, 2>, 8>;
+using NcPlan192 = NcCt<FftCt<192, 16, 4, 3>, 16>;
+using NcPlan256 = NcCt<FftCt<256, 16, 16>, 16>;
 
 // ------------------------------------------------------------ K, column pass
 // The first stage's input b1_c (u - usub) dxi dyi, zero outside the image.
@@ -74,7 +81,7 @@ constexpr int nc_colf_slots() {
   if constexpr (PL::L == 0)
     return 0;
   else
-    return (kNcCtCols * PL::L / PL::R0 + AVIONIC_NC_COLF_THREADS - 1) / AVIONIC_NC_COLF_THREADS * PL::R0;
+    return (PL::XB * PL::L / PL::R0 + AVIONIC_NC_COLF_THREADS - 1) / AVIONIC_NC_COLF_THREADS * PL::R0;
 }
 struct NcColFwdStore {
   float2 *tc;  // tmp rows of (t, c): [Gy][W]
@@ -96,7 +103,7 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
   const int t = blockIdx.y;
   const long HW = (long)g.H * g.W;
   constexpr bool CT = PL::L != 0;
-  const int XB = CT ? kNcCtCols : g.XBf;
+  const int XB = CT ? PL::XB : g.XBf;
   const float2 *tw = stage_twiddles(g.py, twl, tid, NCF);
   __syncthreads();
   NcColFwdLoad<nc_colf_slots<PL>()> ld;
@@ -118,7 +125,7 @@ __global__ __launch_bounds__(NCF) void k_nc_col_fwd(NcGeom g, const float2 *__re
     ld.bc = b1 + c * HW;
     st.tc = tmp + ((long)t * g.C + c) * g.Gy * g.W;
     if constexpr (CT)
-      fft_io_ct<NCF, PL, kNcCtCols, true, true, true>(tile, kNcCtCols, 1, kNcCtCols, -1.f, tid, tw,
+      fft_io_ct<NCF, PL, PL::XB, true, true, true>(tile, PL::XB, 1, PL::XB, -1.f, tid, tw,
                                                       ld, st);
     else
       fft_io<NCF, NCE, true, true, true, WIDE>(tile, g.py, XB, 1, XB, -1.f, tid, tw, ld, st);
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
   const int X = blockIdx.x, nX = gridDim.x;
   const float2 *tw = stage_twiddles(g.py, twl, tid, NCC);
   constexpr bool CT = PL::L != 0;
-  const int XB = CT ? kNcCtCols : g.XBa, ncell = XB * (CT ? PL::L : g.Gy);
+  const int XB = CT ? PL::XB : g.XBa, ncell = XB * (CT ? PL::L : g.Gy);
   const int *tc = g.tcell + ((long)t * nX + X) * (ncell + 1);
   for (int k = tid; k <= ncell; k += NCC) cps[k] = tc[k];
   const int2 *ent = g.tce + g.tstart[t * nX + X];
@@ -351,7 +358,7 @@ __global__ __launch_bounds__(NCC) void k_nc_col_adj(NcGeom g, const float2 *__re
                      CT ? PL::L : g.Gy, X * XB};
     NoLoad nl;
     if constexpr (CT)
-      fft_io_ct<NCC, PL, kNcCtCols, true, false, true>(tile, kNcCtCols, 1, kNcCtCols, +1.f, tid,
+      fft_io_ct<NCC, PL, PL::XB, true, false, true>(tile, PL::XB, 1, PL::XB, +1.f, tid,
                                                        tw, nl, st);
     else
       fft_io<NCC, NCE, true, false, true, WIDE>(tile, g.py, XB, 1, XB, +1.f, tid, tw, nl, st);
@@ -416,7 +423,7 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
     st.bc = b1 + c * HW;
     st.last = (c == g.C - 1);
     if constexpr (CT) {
-      static_assert(((NCE / PL::L) * PL::L / PL::R2 + NCR - 1) / NCR * PL::R2 <= row_slots_nc(WIDE),
+      static_assert(((NCE / PL::L) * PL::L / PL::RL + NCR - 1) / NCR * PL::RL <= row_slots_nc(WIDE),
                     "coil-sum slots of the last stage");
       fft_io_ct<NCR, PL, NCE / PL::L, false, true, true>(tile, nr, PL::L, 1, +1.f, tid, tw, ld, st);
     }
@@ -434,16 +441,25 @@ __global__ __launch_bounds__(NCR) void k_nc_row_adj(NcGeom g, const float2 *__re
 namespace {
 // The compiled grid length that applies (0: none): both grid plans are
 // make_fft_plan's plan of that length and the tiles have the compiled sizes.
-int nc_ct_len(const NcGeom &g) {
-  auto is = [](const FftPlan &p, int L, int r0, int r1, int r2) {
-    return p.L == L && p.nst == 3 && p.radix[0] == r0 && p.radix[1] == r1 && p.radix[2] == r2;
+template <class PL>
+bool nc_is(const NcGeom &g) {
+  auto is = [](const FftPlan &p) {
+    const int r[3] = {PL::R0, PL::R1, PL::R2};
+    const int n = PL::R2 > 1 ? 3 : 2;
+    if (p.L != PL::L || p.nst != n) return false;
+    for (int k = 0; k < n; ++k)
+      if (p.radix[k] != r[k]) return false;
+    return true;
   };
-  if (g.wide || g.XBf != kNcCtCols || g.XBa != kNcCtCols) return 0;
-  for (int L : {384, 512}) {
-    const bool plan = L == 384 ? is(g.px, 384, 16, 8, 3) && is(g.py, 384, 16, 8, 3)
-                               : is(g.px, 512, 16, 16, 2) && is(g.py, 512, 16, 16, 2);
-    if (plan && g.RBf <= NCE / L && g.RBa <= NCE / L) return L;
-  }
+  return is(g.px) && is(g.py) && g.XBf == PL::XB && g.XBa == PL::XB && g.RBf <= NCE / PL::L &&
+         g.RBa <= NCE / PL::L;
+}
+int nc_ct_len(const NcGeom &g) {
+  if (g.wide) return 0;
+  if (nc_is<NcPlan384>(g)) return 384;
+  if (nc_is<NcPlan512>(g)) return 512;
+  if (nc_is<NcPlan192>(g)) return 192;
+  if (nc_is<NcPlan256>(g)) return 256;
   return 0;
 }
 
@@ -480,6 +496,8 @@ void launch_nc_forward_grid(const NcGeom &g, const float2 *u, const float2 *usub
   switch (nc_ct_len(g)) {
     case 384: nc_forward_ct<NcPlan384>(g, u, usub, b1, tmp, part, s); return;
     case 512: nc_forward_ct<NcPlan512>(g, u, usub, b1, tmp, part, s); return;
+    case 192: nc_forward_ct<NcPlan192>(g, u, usub, b1, tmp, part, s); return;
+    case 256: nc_forward_ct<NcPlan256>(g, u, usub, b1, tmp, part, s); return;
     default: break;
   }
   const dim3 gc((g.W + g.XBf - 1) / g.XBf, g.T, g.CG), gr((g.Gy + g.RBf - 1) / g.RBf, g.T * g.C);
@@ -519,6 +537,8 @@ void launch_nc_adjoint(const NcGeom &g, const float2 *z, const float2 *b1, float
   switch (nc_ct_len(g)) {
     case 384: nc_adjoint_ct<NcPlan384>(g, z, b1, tmp, img, s); return;
     case 512: nc_adjoint_ct<NcPlan512>(g, z, b1, tmp, img, s); return;
+    case 192: nc_adjoint_ct<NcPlan192>(g, z, b1, tmp, img, s); return;
+    case 256: nc_adjoint_ct<NcPlan256>(g, z, b1, tmp, img, s); return;
     default: break;
   }
   const size_t cps_bytes = sizeof(int) * ((size_t)g.XBa * g.Gy + 1);

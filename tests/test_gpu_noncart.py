@@ -48,6 +48,9 @@ CASES = [
     (192, 192, 3, 2, 256, 13, 3, 2.0),
     # grid 512: 256 x 256 at the reference's default osf 2, the kernels compiled for length 512
     (256, 256, 4, 2, 512, 9, 3, 2.0),
+    # grids 192 and 256: 128 x 128 at osf 1.5 and 2, compiled with 16-column tiles
+    (128, 128, 5, 2, 256, 11, 3, 1.5),
+    (128, 128, 3, 2, 256, 11, 3, 2.0),
     # 384 x 384 at the reference's default osf = 2, J = 3 (CUDA/config/default.cfg:119-121,
     # noncartesian_operator.cc:248-255): a 768-point grid, the wide kernel variants
     (384, 384, 1, 1, 768, 7, 3, 2.0),
@@ -89,11 +92,14 @@ def test_adjointness_and_determinism(gpu):
     assert bool((op.ForwardOperation(z, b1) == khz).all())
 
 
-# 64 x 64 (grid 96) runs the runtime-plan kernels; 256 x 256 at osf 1.5 and 2
-# (grids 384 and 512) the kernels compiled for those grid lengths.
+# 64 x 64 (grid 96) runs the runtime-plan kernels; 256 x 256 and 128 x 128 at
+# osf 1.5 and 2 (grids 384, 512, 192, 256) the kernels compiled for those grid
+# lengths.
 @pytest.mark.parametrize("W,H,C,T,nfe,spf,osf", [(64, 64, 6, 2, 128, 9, 1.5),
                                                  (256, 256, 5, 1, 512, 5, 1.5),
-                                                 (256, 256, 5, 1, 512, 5, 2.0)])
+                                                 (256, 256, 5, 1, 512, 5, 2.0),
+                                                 (128, 128, 5, 1, 256, 7, 1.5),
+                                                 (128, 128, 5, 1, 256, 7, 2.0)])
 def test_launch_shapes_do_not_change_results(gpu, monkeypatch, W, H, C, T, nfe, spf, osf):
     """The coil-group count of the forward column pass and the rows per
     adjoint row block are tuned per geometry; every shape gives bit-identical
@@ -247,14 +253,14 @@ def test_engine_radial_matches_eager_and_oracle(gpu, method):
     np.testing.assert_allclose(log[0], ref["sigma"], rtol=1e-5)
 
 
-@pytest.mark.parametrize("osf", [1.5, 2.0])
-def test_engine_radial_compiled_grids_match_eager(gpu, osf):
-    """The engine at the grid lengths with compiled gridding passes (384 and
-    512 for a 256 x 256 image): equal to the eager PDRecon path, which the
-    operator cases and the full-size fixture pin against the oracle."""
-    W = H = 256
+@pytest.mark.parametrize("n,osf", [(256, 1.5), (256, 2.0), (128, 1.5), (128, 2.0)])
+def test_engine_radial_compiled_grids_match_eager(gpu, n, osf):
+    """The engine at the grid lengths with compiled gridding passes (384, 512,
+    192, 256): equal to the eager PDRecon path, which the operator cases and
+    the full-size fixture pin against the oracle."""
+    W = H = n
     C, T, nfe, spf, J = 2, 2, 256, 13, 3
-    print(f"[gpu noncart engine] compiled grid, osf={osf}", flush=True)
+    print(f"[gpu noncart engine] compiled grid, n={n} osf={osf}", flush=True)
     data, b1, tr, dcf = _radial_problem(W, H, C, T, nfe, spf, J, osf, 23)
     p = O.default_params("tgv2")
     k, d = O.adapt_lambda_defaults("tgv2")
