@@ -474,6 +474,9 @@ def main():
                          "small-request leg (0: skip)")
     ap.add_argument("--radial-iters", type=int, default=200,
                     help="iterations of the radial TGV2 leg (0: skip)")
+    ap.add_argument("--radial-osf", type=float, default=None,
+                    help="grid oversampling of the radial leg (default: the workload's 1.5; the "
+                         "reference's default configuration uses 2)")
     ap.add_argument("--no-nrmse", action="store_true",
                     help="skip running the batched requests to 1000 iterations for the NRMSE "
                          "(shorter profiler runs)")
@@ -559,7 +562,11 @@ def main():
 
     radial = None
     if args.radial_iters > 0 and not args.no_legs:
-        log(f"radial TGV2 leg ({synth.RADIAL.name}, {args.radial_iters} iterations)")
+        if args.radial_osf is not None:
+            import dataclasses
+            synth.RADIAL = dataclasses.replace(synth.RADIAL, osf=float(args.radial_osf))
+        log(f"radial TGV2 leg ({synth.RADIAL.name}, osf {synth.RADIAL.osf}, "
+            f"{args.radial_iters} iterations)")
         radial = radial_leg(A, synth, torch, dev, args.radial_iters)
 
     serving = None
